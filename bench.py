@@ -1,0 +1,204 @@
+"""Benchmark: seamless-clone Poisson blend to 1e-6 relative residual on one GPU.
+
+Workload (one step = one full solve through the C ABI, inputs resident in HBM):
+4096x4096 RGBA f32, fbm source/target (frequency 0.125, 6 octaves, gain 0.5,
+seeds 101+c / 201+c, mapped to [0, 1]), alpha 1, seeded 12-vertex star mask
+covering ~30% of the image; R, G, B solved independently (alpha has a zero
+initial residual and passes through at iteration 0, as in the reference).
+
+Prints ONE JSON line: ms per megapixel to tolerance (lower is better), the
+sweep kernel's roofline, and the CPU baselines timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/megapixel to 1e-6 rel. residual, learned warm-start vs plain SOR; HBM GB/s"
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_UNKNOWN_SWEEP = 12.0  # FP32 e read + e write + rhs read
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--size", type=int, default=4096)
+    p.add_argument("--half-sweeps", type=int, default=0)
+    p.add_argument("--omega", type=float, default=0.0)
+    p.add_argument("--tol", type=float, default=1e-6)
+    p.add_argument("--cpu-seconds", type=float, default=20.0,
+                   help="budget of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-tight", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baselines(size, omega, gpu_sweeps, budget_s):
+    """Time the oracle on this host on a bounded sample of the same workload:
+    the OpenMP red-black SOR stand-in for cpu.optimized (all cores, capped at
+    16 threads) and the single-core restatement of cpu.reference (lexicographic
+    SOR, omega 1.8). Both run whole sweeps of the full-size R channel; the
+    per-sweep time is scaled to the sweep count the solve needs."""
+    import torch
+
+    from oracle import oracle as orc
+    from paintop_amd.inputs import seamless_clone_case
+    from tests.helpers import classify, laplacian
+
+    src, tgt, msk = seamless_clone_case(size, channels=4)
+    s = src[..., 0].cpu().numpy()
+    t = tgt[..., 0].cpu().numpy().astype(np.float64)
+    m = msk.cpu().numpy()
+    del src, tgt, msk
+    torch.cuda.empty_cache()
+    cells = classify(m)
+    rhs = np.where(cells == 1, laplacian(s), 0.0)
+    threads = min(os.cpu_count() or 1, 16)
+    # all-cores stand-in: time n sweeps
+    n = 4
+    t0 = time.perf_counter()
+    _, rep, used = orc.rbsor_omp(cells, rhs, t, 0.0, n, 1e-30, omega, threads=threads)
+    per_sweep = (time.perf_counter() - t0) / n
+    n2 = max(2, min(200, int(budget_s / max(per_sweep, 1e-6))))
+    t0 = time.perf_counter()
+    _, rep, used = orc.rbsor_omp(cells, rhs, t, 0.0, n2, 1e-30, omega, threads=threads)
+    per_sweep = (time.perf_counter() - t0) / n2
+    mp = size * size / 1e6
+    opt_ms_per_mp = per_sweep * gpu_sweeps * 3 * 1e3 / mp
+    # single-core cpu.reference restatement (omega 1.8): per-sweep time on a crop
+    crop = min(size, 1024)
+    y0 = (size - crop) // 2
+    cc = cells[y0:y0 + crop, y0:y0 + crop].copy()
+    cc[0, :] = cc[-1, :] = 0
+    cc[:, 0] = cc[:, -1] = 0
+    rr = rhs[y0:y0 + crop, y0:y0 + crop].copy()
+    tt = t[y0:y0 + crop, y0:y0 + crop].copy()
+    nref = 3
+    t0 = time.perf_counter()
+    orc.solve(cc, rr, tt, 0.0, nref, 1e-30, 1.8)
+    ref_sweep_full = (time.perf_counter() - t0) / nref * (size * size) / (crop * crop)
+    return {
+        "value": round(opt_ms_per_mp, 3),
+        "unit": "ms/MP",
+        "cores": int(used),
+        "kind": "port",
+        "sample": (f"OpenMP red-black SOR (oracle/rbsor_omp.c, the cpu.optimized stand-in) on "
+                   f"the full {size}x{size} R channel, {n2} sweeps timed ({per_sweep * 1e3:.1f} "
+                   f"ms/sweep incl. residual), scaled to {gpu_sweeps} sweeps x 3 channels"),
+        "cpu_reference_ms_per_sweep_full_image": round(ref_sweep_full * 1e3, 1),
+        "cpu_reference_note": ("single-core lexicographic SOR (omega 1.8) restated from "
+                               "poisson.rs, timed on a 1024^2 crop and scaled by pixel count; "
+                               "at omega 1.8 it cannot reach 1e-6 on this workload within the "
+                               "reference's 100000-iteration cap"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not implemented)"}))
+        sys.exit(2)
+    import torch
+
+    from paintop_amd import hip
+    from paintop_amd.inputs import seamless_clone_case
+
+    info = hip.probe()
+    size = args.size
+    src, tgt, msk = seamless_clone_case(size, channels=4)
+    out = torch.empty_like(tgt)
+    stream = torch.cuda.current_stream().cuda_stream
+    desc = hip.make_desc(size, size, 4, 100000, args.tol, 0.0, hip.PH_OP_POISSON_BLEND,
+                         omega=args.omega, half_sweeps=args.half_sweeps, flags=hip.PH_FLAG_TIMING)
+
+    def step():
+        return hip.poisson_solve_device(desc, src.data_ptr(), tgt.data_ptr(), msk.data_ptr(),
+                                        out.data_ptr(), stream, want_history=False)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times, results = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = step()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+        results.append(r)
+    ms = float(np.median(times))
+    mp = size * size / 1e6
+    r = results[len(results) // 2]
+    # roofline of the sweep kernel: algorithmic bytes over its event-timed duration
+    sweep_ms = float(np.median([x.sweep_ms for x in results]))
+    alg_bytes = BYTES_PER_UNKNOWN_SWEEP * r.unknown_sweeps
+    achieved = alg_bytes / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
+    per_launch_us = sweep_ms * 1e3 / max(r.launches, 1)
+    line = {
+        "metric": METRIC,
+        "value": round(ms / mp, 4),
+        "unit": "ms/MP",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 sweeps, f64 residual/refinement",
+        "data": "synthetic (device-generated fbm + star mask)",
+        "config": {
+            "workload": f"seamless-clone Poisson blend {size}x{size} RGBA f32, star mask ~30%, "
+                        f"fbm source/target, tol {args.tol:g}, plain red-black SOR start",
+            "sweeps": r.iterations,
+            "refinements": r.refinements,
+            "launches": r.launches,
+            "omega": round(r.omega, 6),
+            "half_sweeps_per_launch": r.half_sweeps,
+            "unknowns_per_channel": r.unknowns // 3 if r.unknowns else 0,
+            "converged": r.converged,
+            "final_residual": r.final_residual,
+            "device": info["name"],
+        },
+        "roofline": {
+            "kernel": f"k_sor<{r.half_sweeps}>",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "avg_launch_us": round(per_launch_us, 2),
+            "note": "achieved = 12 B x unknown-sweeps / sweep-kernel time; with K half-sweeps "
+                    "per launch the compulsory HBM traffic is 12 B per unknown per launch",
+        },
+    }
+    if not args.no_tight:
+        # error of the 1e-6 result against the same solve driven to 1e-12 in FP64 refinement
+        out_tight = torch.empty_like(tgt)
+        d2 = hip.make_desc(size, size, 4, 100000, 1e-12, 0.0, hip.PH_OP_POISSON_BLEND,
+                           omega=args.omega, half_sweeps=args.half_sweeps)
+        rt = hip.poisson_solve_device(d2, src.data_ptr(), tgt.data_ptr(), msk.data_ptr(),
+                                      out_tight.data_ptr(), stream, want_history=False)
+        err = (out - out_tight).abs().max().item()
+        line["config"]["max_abs_err_vs_fp64_1e-12"] = err
+        line["config"]["tight_sweeps"] = rt.iterations
+        del out_tight
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baselines(size, r.omega, r.iterations, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

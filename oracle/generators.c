@@ -1,0 +1,78 @@
+/*
+ * generators.c -- test-infrastructure restatement of paintop's field.fbm@1
+ * value-noise generator (crates/paintop-cpu/src/noise.rs). Used by the tests as
+ * the checker for the device-side fbm generator, and by the benchmark's
+ * cpu_baseline leg. Build: -O2 -ffp-contract=off, no fast-math.
+ */
+#include <math.h>
+
+#include "oracle.h"
+
+/* noise.rs:72-76: the SplitMix64 finalizer. */
+uint64_t or_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+/* noise.rs:86-109: fold (gx, gy) into the seed, top 24 bits -> [-1, 1]. */
+double or_lattice_value(int64_t gx, int64_t gy, uint64_t seed) {
+  uint64_t ux = (uint64_t)gx;
+  uint64_t uy = (uint64_t)gy;
+  uint64_t hx = or_mix64(seed ^ (ux * 0x9e3779b97f4a7c15ULL));
+  uint64_t h = or_mix64(hx ^ (uy * 0xc2b2ae3d27d4eb4fULL));
+  uint64_t top24 = h >> 40;
+  double unit = (double)(uint32_t)top24 / (double)(1u << 24);
+  return fma(unit, 2.0, -1.0);
+}
+
+/* noise.rs:113-115: ((t*t)*t) * fma(t, fma(t, 6, -15), 10). */
+static double fade(double t) { return t * t * t * fma(t, fma(t, 6.0, -15.0), 10.0); }
+
+/* noise.rs:118-120 */
+static double lerp(double a, double b, double t) { return fma(b - a, t, a); }
+
+/* noise.rs:127-160 */
+double or_value_noise(double x, double y, uint64_t seed) {
+  double x0 = floor(x);
+  double y0 = floor(y);
+  int64_t gx0 = (int64_t)x0;
+  int64_t gy0 = (int64_t)y0;
+  double tx = fade(x - x0);
+  double ty = fade(y - y0);
+  double v00 = or_lattice_value(gx0, gy0, seed);
+  double v10 = or_lattice_value(gx0 + 1, gy0, seed);
+  double v01 = or_lattice_value(gx0, gy0 + 1, seed);
+  double v11 = or_lattice_value(gx0 + 1, gy0 + 1, seed);
+  double top = lerp(v00, v10, tx);
+  double bottom = lerp(v01, v11, tx);
+  return lerp(top, bottom, ty);
+}
+
+/* noise.rs:218-240 */
+double or_fbm_eval(double x, double y, uint64_t seed, uint32_t octaves, double lacunarity,
+                   double gain) {
+  double frequency = 1.0, amplitude = 1.0, sum = 0.0, total = 0.0;
+  for (uint32_t o = 0; o < octaves; ++o) {
+    uint64_t octave_seed = or_mix64(seed ^ ((uint64_t)o + 1u));
+    sum = fma(amplitude, or_value_noise(x * frequency, y * frequency, octave_seed), sum);
+    total += amplitude;
+    frequency *= lacunarity;
+    amplitude *= gain;
+  }
+  return total > 0.0 ? sum / total : 0.0;
+}
+
+/* noise.rs:660-694 with the pixel-centre convention of
+ * paintop-ir/src/resource.rs:394-397 and FREQUENCY_PIXELS = 64 (noise.rs:646). */
+void or_fbm(size_t width, size_t height, double frequency, uint64_t seed, uint32_t octaves,
+            double lacunarity, double gain, float *out) {
+  double scale = frequency / 64.0;
+  for (size_t y = 0; y < height; ++y)
+    for (size_t x = 0; x < width; ++x) {
+      double px = (double)(uint32_t)x + 0.5;
+      double py = (double)(uint32_t)y + 0.5;
+      out[y * width + x] =
+          (float)or_fbm_eval(px * scale, py * scale, seed, octaves, lacunarity, gain);
+    }
+}

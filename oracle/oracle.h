@@ -1,0 +1,88 @@
+/*
+ * oracle.h -- CPU restatement of paintop's Poisson / screened-Poisson solver
+ * path, used ONLY as test infrastructure: the checker that the tests, the smoke
+ * entry point and the benchmark's cpu_baseline leg compare the HIP path against.
+ * Nothing in the product (paintop_amd/) includes, links or calls this code.
+ *
+ * What each part restates (file:line in bobisme/paintop):
+ *   or_classify_cells        crates/paintop-cpu/src/poisson.rs:329-355
+ *   or_guidance_laplacian    crates/paintop-cpu/src/poisson.rs:275-298, 364-373
+ *   or_residual_l2           crates/paintop-cpu/src/poisson.rs:305-325
+ *   or_solve                 crates/paintop-cpu/src/poisson.rs:168-265 (+ STALL_* :75-85)
+ *   or_blend_channels        crates/paintop-cpu/src/poisson_blend.rs:153-246
+ *   or_fbm                   crates/paintop-cpu/src/noise.rs:72-160, 218-240, 660-694
+ *   or_rbsor_omp             no reference counterpart: the reference has no
+ *                            cpu.optimized Poisson kernel, so this OpenMP
+ *                            red-black SOR is the stand-in timed as the
+ *                            all-cores CPU baseline.
+ *
+ * Bit-faithfulness recipe: every Rust `a.mul_add(b, c)` is `fma(a, b, c)`, the
+ * left-to-right evaluation order of every sum is kept, and the library is built
+ * with -ffp-contract=off and without fast-math, so the f64 trajectory is the
+ * one IEEE-754 fixes for the Rust source.
+ */
+#ifndef PAINTOP_ORACLE_H
+#define PAINTOP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* SolverStopReason (paintop-ir/src/resource.rs:1186-1196). */
+enum { OR_CONVERGED = 0, OR_MAX_ITERATIONS = 1, OR_STALLED = 2 };
+
+typedef struct {
+  uint32_t iterations;
+  int32_t stop_reason;
+  int32_t converged;
+  int32_t _pad;
+  double tolerance;
+  double final_residual;
+  uint32_t history_len;
+  uint32_t _pad2;
+} or_report;
+
+/* cells: 1 = Interior, 0 = Boundary. */
+void or_classify_cells(const float *mask, size_t width, size_t height, uint8_t *cells);
+
+double or_guidance_laplacian(const double *field, size_t col, size_t row, size_t width,
+                             size_t height);
+
+double or_residual_l2(size_t width, size_t height, const uint8_t *cells, const double *rhs,
+                      const double *anchor, double lambda, const double *field);
+
+/* field: out, width*height. history: out, capacity max_iterations. */
+void or_solve(size_t width, size_t height, const uint8_t *cells, const double *rhs,
+              const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
+              double omega, double *field, double *history, or_report *report);
+
+/* Full op semantics of blend_channels: interleaved f32 in, interleaved f32 out,
+ * the worst channel's report (and its history, capacity max_iterations). */
+void or_blend_channels(const float *source, const float *target, const float *mask,
+                       size_t channels, size_t width, size_t height, uint32_t max_iterations,
+                       double tolerance, double lambda, double omega, float *out,
+                       double *history, or_report *report);
+
+/* field.fbm@1 sample grid: value at pixel centre (x+0.5, y+0.5) * frequency/64. */
+uint64_t or_mix64(uint64_t z);
+double or_lattice_value(int64_t gx, int64_t gy, uint64_t seed);
+double or_value_noise(double x, double y, uint64_t seed);
+double or_fbm_eval(double x, double y, uint64_t seed, uint32_t octaves, double lacunarity,
+                   double gain);
+void or_fbm(size_t width, size_t height, double frequency, uint64_t seed, uint32_t octaves,
+            double lacunarity, double gain, float *out);
+
+/* The all-cores CPU stand-in: red-black SOR in f64 on interior cells, with the
+ * reference's per-sweep relative residual and stop rules. threads <= 0 means
+ * the OpenMP default. Returns the thread count used. */
+int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double *rhs,
+                 const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
+                 double omega, int threads, double *field, double *history, or_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+"""ctypes face of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the tests, ``__graft_entry__.smoke()`` and the
+benchmark's ``cpu_baseline`` leg import this module as the checker. The
+product package never imports it.
+
+Each function names the reference code it restates (bobisme/paintop):
+``solve`` -> crates/paintop-cpu/src/poisson.rs:168-265,
+``blend_channels`` -> crates/paintop-cpu/src/poisson_blend.rs:168-246,
+``fbm`` -> crates/paintop-cpu/src/noise.rs:72-160, 218-240, 660-694.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+STOP_NAMES = {0: "converged", 1: "max-iterations", 2: "stalled"}
+
+
+class _Report(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("stop_reason", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("_pad", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("final_residual", ctypes.c_double),
+        ("history_len", ctypes.c_uint32),
+        ("_pad2", ctypes.c_uint32),
+    ]
+
+
+@dataclass
+class SolveReport:
+    """Mirror of ``SolveReport`` (poisson.rs:126-141)."""
+
+    iterations: int
+    converged: bool
+    stop_reason: str
+    tolerance: float
+    final_residual: float
+    residual_history: list = field(default_factory=list)
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        up = ctypes.POINTER(ctypes.c_uint8)
+        sz = ctypes.c_size_t
+        L.or_classify_cells.argtypes = [fp, sz, sz, up]
+        L.or_guidance_laplacian.argtypes = [dp, sz, sz, sz, sz]
+        L.or_guidance_laplacian.restype = ctypes.c_double
+        L.or_residual_l2.argtypes = [sz, sz, up, dp, dp, ctypes.c_double, dp]
+        L.or_residual_l2.restype = ctypes.c_double
+        L.or_solve.argtypes = [sz, sz, up, dp, dp, ctypes.c_double, ctypes.c_uint32,
+                               ctypes.c_double, ctypes.c_double, dp, dp,
+                               ctypes.POINTER(_Report)]
+        L.or_blend_channels.argtypes = [fp, fp, fp, sz, sz, sz, ctypes.c_uint32,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        fp, dp, ctypes.POINTER(_Report)]
+        L.or_mix64.argtypes = [ctypes.c_uint64]
+        L.or_mix64.restype = ctypes.c_uint64
+        L.or_lattice_value.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64]
+        L.or_lattice_value.restype = ctypes.c_double
+        L.or_value_noise.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
+        L.or_value_noise.restype = ctypes.c_double
+        L.or_fbm_eval.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                  ctypes.c_uint32, ctypes.c_double, ctypes.c_double]
+        L.or_fbm_eval.restype = ctypes.c_double
+        L.or_fbm.argtypes = [sz, sz, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
+                             ctypes.c_double, ctypes.c_double, fp]
+        L.or_rbsor_omp.argtypes = [sz, sz, up, dp, dp, ctypes.c_double, ctypes.c_uint32,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp,
+                                   ctypes.POINTER(_Report)]
+        L.or_rbsor_omp.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _report(r: _Report, hist: np.ndarray) -> SolveReport:
+    return SolveReport(
+        iterations=int(r.iterations),
+        converged=bool(r.converged),
+        stop_reason=STOP_NAMES[int(r.stop_reason)],
+        tolerance=float(r.tolerance),
+        final_residual=float(r.final_residual),
+        residual_history=[float(v) for v in hist[: r.history_len]],
+    )
+
+
+def classify_cells(mask: np.ndarray) -> np.ndarray:
+    mask = np.ascontiguousarray(mask, dtype=np.float32)
+    h, w = mask.shape
+    cells = np.zeros((h, w), dtype=np.uint8)
+    lib().or_classify_cells(_p(mask, ctypes.c_float), w, h, _p(cells, ctypes.c_uint8))
+    return cells
+
+
+def guidance_laplacian(field_: np.ndarray) -> np.ndarray:
+    """rhs = Laplacian of ``field_`` at every pixel (poisson.rs:364-373)."""
+    f = np.ascontiguousarray(field_, dtype=np.float64)
+    h, w = f.shape
+    out = np.empty_like(f)
+    L = lib()
+    fp = _p(f, ctypes.c_double)
+    for r in range(h):
+        for c in range(w):
+            out[r, c] = L.or_guidance_laplacian(fp, c, r, w, h)
+    return out
+
+
+def residual_l2(cells, rhs, anchor, lam, field_) -> float:
+    cells = np.ascontiguousarray(cells, dtype=np.uint8)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    anchor = np.ascontiguousarray(anchor, dtype=np.float64)
+    field_ = np.ascontiguousarray(field_, dtype=np.float64)
+    h, w = cells.shape
+    return float(lib().or_residual_l2(w, h, _p(cells, ctypes.c_uint8), _p(rhs, ctypes.c_double),
+                                      _p(anchor, ctypes.c_double), float(lam),
+                                      _p(field_, ctypes.c_double)))
+
+
+def solve(cells, rhs, anchor, lam, max_iterations, tolerance, omega):
+    """poisson::solve restated (lexicographic SOR, f64)."""
+    cells = np.ascontiguousarray(cells, dtype=np.uint8)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    anchor = np.ascontiguousarray(anchor, dtype=np.float64)
+    h, w = cells.shape
+    out = np.empty((h, w), dtype=np.float64)
+    hist = np.zeros(max(int(max_iterations), 1), dtype=np.float64)
+    rep = _Report()
+    lib().or_solve(w, h, _p(cells, ctypes.c_uint8), _p(rhs, ctypes.c_double),
+                   _p(anchor, ctypes.c_double), float(lam), int(max_iterations),
+                   float(tolerance), float(omega), _p(out, ctypes.c_double),
+                   _p(hist, ctypes.c_double), ctypes.byref(rep))
+    return out, _report(rep, hist)
+
+
+def blend_channels(source, target, mask, lam, max_iterations, tolerance, omega=1.8):
+    """poisson_blend::blend_channels restated. source/target: (H, W, C) f32."""
+    source = np.ascontiguousarray(source, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    mask = np.ascontiguousarray(mask, dtype=np.float32)
+    if source.ndim == 2:
+        source = source[:, :, None]
+        target = target[:, :, None]
+    h, w, c = source.shape
+    out = np.empty_like(target)
+    hist = np.zeros(max(int(max_iterations), 1), dtype=np.float64)
+    rep = _Report()
+    lib().or_blend_channels(_p(source, ctypes.c_float), _p(target, ctypes.c_float),
+                            _p(mask, ctypes.c_float), c, w, h, int(max_iterations),
+                            float(tolerance), float(lam), float(omega),
+                            _p(out, ctypes.c_float), _p(hist, ctypes.c_double),
+                            ctypes.byref(rep))
+    return out, _report(rep, hist)
+
+
+def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads=0):
+    """The all-cores red-black SOR stand-in for cpu.optimized."""
+    cells = np.ascontiguousarray(cells, dtype=np.uint8)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    anchor = np.ascontiguousarray(anchor, dtype=np.float64)
+    h, w = cells.shape
+    out = np.empty((h, w), dtype=np.float64)
+    hist = np.zeros(max(int(max_iterations), 1), dtype=np.float64)
+    rep = _Report()
+    used = lib().or_rbsor_omp(w, h, _p(cells, ctypes.c_uint8), _p(rhs, ctypes.c_double),
+                              _p(anchor, ctypes.c_double), float(lam), int(max_iterations),
+                              float(tolerance), float(omega), int(threads),
+                              _p(out, ctypes.c_double), _p(hist, ctypes.c_double),
+                              ctypes.byref(rep))
+    return out, _report(rep, hist), int(used)
+
+
+def mix64(z: int) -> int:
+    return int(lib().or_mix64(z & 0xFFFFFFFFFFFFFFFF))
+
+
+def lattice_value(gx: int, gy: int, seed: int) -> float:
+    return float(lib().or_lattice_value(gx, gy, seed & 0xFFFFFFFFFFFFFFFF))
+
+
+def value_noise(x: float, y: float, seed: int) -> float:
+    return float(lib().or_value_noise(x, y, seed & 0xFFFFFFFFFFFFFFFF))
+
+
+def fbm_eval(x, y, seed, octaves, lacunarity=2.0, gain=0.5) -> float:
+    return float(lib().or_fbm_eval(x, y, seed & 0xFFFFFFFFFFFFFFFF, octaves, lacunarity, gain))
+
+
+def fbm(width, height, frequency, seed, octaves, lacunarity=2.0, gain=0.5) -> np.ndarray:
+    out = np.empty((height, width), dtype=np.float32)
+    lib().or_fbm(width, height, float(frequency), seed & 0xFFFFFFFFFFFFFFFF, int(octaves),
+                 float(lacunarity), float(gain), _p(out, ctypes.c_float))
+    return out

@@ -1,0 +1,185 @@
+// fp64_kernels.hip -- classification, system assembly, the FP64 residual of the
+// refinement loop, and the write-back of the candidate.
+//
+// All arithmetic that fixes what the solve converges to runs in FP64 with the
+// reference's operation order:
+//   classify      poisson.rs:340-355  (mask > 0.5 and not on the image border)
+//   b             poisson.rs:364-373  (fma(4, -c, ((l + r) + u) + d) of the source)
+//   b'            poisson.rs:319      (fma(lambda, -anchor, b))
+//   r             poisson.rs:318-320  (fma(lambda + 4, -u, nsum) - b')
+// The squared-residual sums are reduced in a fixed order (per-row sums, then a
+// fixed-order sum of rows), so reruns are bit-identical.
+#include <math.h>
+
+#include "ph_internal.h"
+
+namespace ph {
+
+constexpr float MASK_THRESHOLD = 0.5f;  // poisson.rs:329
+
+// Classify every image pixel and grow the bounding box of interior cells
+// (bbox: xmin, ymin, xmax, ymax; int atomics, order independent).
+__global__ void k_classify(const float *__restrict__ mask, int w, int h,
+                           uint8_t *__restrict__ cls, int *__restrict__ bbox) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= w) return;
+  const size_t idx = (size_t)y * w + x;
+  uint8_t in = 0;
+  if (w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 && y < h - 1)
+    in = mask[idx] > MASK_THRESHOLD ? 1 : 0;
+  cls[idx] = in;
+  // wave-level pre-reduction keeps the atomics few
+  const unsigned long long ball = __ballot(in);
+  if (ball) {
+    int xmin = in ? x : 0x7fffffff, xmax = in ? x : -1;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      xmin = min(xmin, __shfl_xor(xmin, off, 64));
+      xmax = max(xmax, __shfl_xor(xmax, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&bbox[0], xmin);
+      atomicMax(&bbox[2], xmax);
+      atomicMin(&bbox[1], y);
+      atomicMax(&bbox[3], y);
+    }
+  }
+}
+
+// Count interior cells of each tile's valid core (one block per tile).
+__global__ void k_tile_occupancy(const uint8_t *__restrict__ cls, int w, int h, int ox, int oy,
+                                 int K, int wv, int hb, const int2 *__restrict__ origins,
+                                 int *__restrict__ counts) {
+  const int2 o = origins[blockIdx.x];
+  int n = 0;
+  for (int r = 0; r < hb; ++r) {
+    const int iy = oy + o.y + K + r;
+    if (iy < 0 || iy >= h) continue;
+    for (int cidx = threadIdx.x; cidx < wv; cidx += blockDim.x) {
+      const int ix = ox + o.x + K + cidx;
+      if (ix >= 0 && ix < w) n += cls[(size_t)iy * w + ix];
+    }
+  }
+  __shared__ int red[256];
+  red[threadIdx.x] = n;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
+}
+
+// Fill an instance's planes from the interleaved image channel.
+__global__ void k_assemble(PhInst I, const float *__restrict__ src, const float *__restrict__ tgt,
+                           const uint8_t *__restrict__ cls_img, uint8_t *__restrict__ cls) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pitch) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  const int ix = I.ox + px, iy = I.oy + py;
+  const int W = I.img_w, H = I.img_h, C = I.channels, ch = I.channel;
+  double a = 0.0, bp = 0.0;
+  uint8_t in = 0;
+  if (px < I.pw && ix >= 0 && ix < W && iy >= 0 && iy < H) {
+    const size_t q = (size_t)iy * W + ix;
+    a = (double)tgt[q * C + ch];
+    in = cls_img[q];
+    if (in) {
+      // interior cells are never on the image border: four in-image neighbours
+      const double cc = (double)src[q * C + ch];
+      const double l = (double)src[(q - 1) * C + ch];
+      const double r = (double)src[(q + 1) * C + ch];
+      const double u = (double)src[(q - W) * C + ch];
+      const double d = (double)src[(q + W) * C + ch];
+      const double b = fma(4.0, -cc, ((l + r) + u) + d);
+      bp = fma(I.lambda, -a, b);
+    }
+  }
+  I.x[p] = a;
+  I.bp[p] = bp;
+  cls[p] = in;
+  I.e[0][p] = 0.0f;
+  I.e[1][p] = 0.0f;
+  I.g[p] = __builtin_nanf("");
+}
+
+// x += e at interior cells, e = 0 (the FP64 correction step).
+__global__ void k_apply_correction(PhInst I, int buf) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pw) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  if (I.cls[p]) {
+    I.x[p] += (double)I.e[buf][p];
+    I.e[buf][p] = 0.0f;
+  }
+}
+
+// g = -r(x) in FP64, stored FP32 (NaN at non-interior cells); per-row sums of
+// r^2 in a fixed order; the last row block of the instance adds the rows in
+// order and stores ||r|| in out[0] and ||r||^2 in out[1].
+__global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__ row_sums,
+                                                  uint32_t *__restrict__ arrival,
+                                                  double *__restrict__ out) {
+  const int py = blockIdx.x;
+  double s = 0.0;
+  for (int px = threadIdx.x; px < I.pw; px += blockDim.x) {
+    const size_t p = (size_t)py * I.pitch + px;
+    if (I.cls[p]) {
+      const double nsum = ((I.x[p - 1] + I.x[p + 1]) + I.x[p - I.pitch]) + I.x[p + I.pitch];
+      const double r = fma(I.dg64, -I.x[p], nsum) - I.bp[p];
+      I.g[p] = (float)(-r);
+      s = fma(r, r, s);
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  __shared__ uint32_t ticket;
+  if (threadIdx.x == 0) {
+    row_sums[py] = red[0];
+    __threadfence();
+    ticket = atomicAdd(arrival, 1u);
+  }
+  __syncthreads();
+  if (ticket == (uint32_t)(gridDim.x - 1)) {
+    __threadfence();
+    // fixed-order sum of the row sums: thread t takes a contiguous run of rows
+    const int n = gridDim.x;
+    const int per = (n + 255) / 256;
+    double v = 0.0;
+    for (int r = threadIdx.x * per; r < min(n, (int)(threadIdx.x + 1) * per); ++r) v += row_sums[r];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      out[0] = sqrt(red[0]);
+      out[1] = red[0];
+      *arrival = 0u;
+    }
+  }
+}
+
+// Write the solved interior cells of an instance into the interleaved
+// candidate; every other sample already holds the target's bits.
+__global__ void k_scatter(PhInst I, float *__restrict__ out) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pw) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  if (I.cls[p]) {
+    const size_t q = (size_t)(I.oy + py) * I.img_w + (I.ox + px);
+    out[q * I.channels + I.channel] = (float)I.x[p];
+  }
+}
+
+}  // namespace ph

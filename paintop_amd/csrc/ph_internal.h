@@ -1,0 +1,82 @@
+// ph_internal.h -- device-side data layout shared by the solver kernels and the
+// host orchestration of libpaintop_hip.
+//
+// One solve instance = one channel of one image region. Its unknowns live in
+// a padded rectangle ("plane") that covers the bounding box of the interior
+// cells plus the one-cell Dirichlet ring, extended by PH_PAD cells of inert
+// padding on every side so that the halo reads of the sweep tiles never leave
+// the allocation. Plane cell (px, py) is image cell (ox + px, oy + py); the
+// plane origin is chosen with (ox + oy) even, so the red/black colour of a
+// plane cell is (px + py) & 1 and equals the colour of its image cell
+// (red = even).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PH_PAD 16          // >= the largest half-sweep count per launch
+#define PH_MAX_K 16
+#define PH_STALL_WINDOW 16u       // poisson.rs:81
+#define PH_STALL_IMPROVEMENT 1e-12 // poisson.rs:85
+
+// Per-instance descriptor (read-only for the sweep kernels).
+struct PhInst {
+  float *e[2];      // FP32 correction, ping-pong (plane layout, pitch floats per row)
+  float *g;         // FP32 right-hand side of the correction equation; NaN = not interior
+  double *x;        // FP64 solution (plane layout)
+  double *bp;       // FP64 b' = b - lambda*anchor at interior cells, 0 elsewhere
+  const uint8_t *cls; // 1 = interior (plane layout, pitch bytes per row)
+  int32_t pitch;    // elements per plane row
+  int32_t pw, ph;   // plane width / height in cells
+  int32_t ox, oy;   // image coordinates of plane cell (0, 0)
+  int32_t img_w, img_h, channels, channel; // source image geometry for gather/scatter
+  float c;          // omega / (4 + lambda)
+  float dg;         // 4 + lambda (FP32)
+  double dg64;      // 4 + lambda (FP64)
+  double lambda;
+  int32_t tile_begin, tile_end; // this instance's tiles in the tile list
+};
+
+// Halt states of the per-instance control block.
+enum : uint32_t {
+  PH_RUN = 0,
+  PH_HALT_STOP = 1,     // a stop rule fired at sweep halt_sweep (estimate)
+  PH_HALT_REFINE = 2,   // the inner residual fell below the refinement threshold
+  PH_HALT_LIMIT = 3,    // sweeps_done reached sweep_limit
+  PH_HALT_REDO_DONE = 4,
+  PH_HALT_DONE = 5,     // solve finished (host side)
+};
+enum : uint32_t { PH_MODE_NORMAL = 0, PH_MODE_REDO = 1 };
+enum : uint32_t { PH_REASON_CONVERGED = 0, PH_REASON_MAXIT = 1, PH_REASON_STALLED = 2 };
+
+// Per-instance control block: the reference's stop rules (poisson.rs:202-229)
+// run here, on the device, once per sweep launch.
+struct PhCtl {
+  double r0;          // ||r(anchor start)|| in FP64: the relative-residual denominator
+  double tol;
+  double refine_abs2; // refine when the squared inner residual falls to this
+  double om1sq;       // (1 - omega)^2
+  double best;        // best relative residual so far (stall guard)
+  double prev_black;  // black pre-update residual sum of the last half-sweep of the previous launch
+  double *history;    // relative residual per sweep, capacity max_it
+  uint32_t max_it;
+  uint32_t sweep_limit;
+  uint32_t sweeps_done;  // sweeps applied to the state in e[cur_buf]
+  uint32_t sweeps_final; // sweeps whose residual is in history
+  uint32_t since;
+  uint32_t halt;
+  uint32_t reason;
+  uint32_t halt_sweep;
+  uint32_t halt_base;  // sweeps_done at the start of the launch that halted
+  uint32_t halt_buf;   // its input buffer
+  uint32_t cur_buf;
+  uint32_t mode;
+  uint32_t refine_pending;
+  uint32_t launches;
+};
+
+struct PhTile {
+  int32_t inst;
+  int32_t px0; // first loaded plane column (valid columns start at px0 + K)
+  int32_t py0; // first loaded plane row (valid rows start at py0 + K)
+  int32_t pad;
+};

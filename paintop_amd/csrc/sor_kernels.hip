@@ -1,0 +1,290 @@
+// sor_kernels.hip -- FP32 red-black SOR sweeps with temporal blocking in
+// registers, and the per-instance stop-rule control that follows each launch.
+//
+// The equation relaxed is the correction equation of the FP64 refinement loop:
+//     nsum(e) - (4 + lambda) e = g      at interior cells, e = 0 elsewhere,
+// where g = -(residual of the FP64 iterate x). Its residual
+//     r = nsum(e) - (4 + lambda) e - g
+// is the residual of x + e, i.e. of the reference's equation
+// (crates/paintop-cpu/src/poisson.rs:305-325), and one SOR update is
+//     e <- e + omega/(4+lambda) * r          (poisson.rs:256-262 rearranged).
+//
+// Schedule. A wave owns a tile: 64 lanes x CPL adjacent columns, and streams
+// down its rows. At step t it applies half-sweep s (s = 1..K, red for odd s)
+// to row t - s + 1, in increasing s, so every update reads exactly the values
+// a global red/black sweep would read; the K+2 live rows stay in registers
+// (a ring indexed at compile time), left/right neighbours across lanes come
+// from DPP wavefront shifts. Each row is read once and written once per K
+// half-sweeps: 12 bytes per unknown per launch (e in, g in, e out). The tile
+// loads a K-cell halo on every side; its valid core is written to the other
+// ping-pong buffer, so tiles never see each other's writes within a launch and
+// the result is the same for every tile shape.
+//
+// Residuals. With r_pre the residual of a cell just before its update, the
+// residual after the update is (1 - omega) r_pre exactly, and black cells do
+// not move after their own update within a sweep. So
+//     ||r(after sweep j)||^2 = (1-omega)^2 sum_black r_pre(2j)^2 + sum_red r_pre(2j+1)^2,
+// available one half-sweep late, with no extra pass. Per-lane sums are FP32 in
+// fixed row order, lanes are combined by a fixed butterfly in FP64, tiles by a
+// fixed-order sum: reruns are bit-identical.
+#include <math.h>
+
+#include "ph_internal.h"
+
+namespace ph {
+
+// DPP wavefront shifts: lane l receives lane l-1 (shr) / lane l+1 (shl); the
+// lanes shifted in from outside the wave receive 0.
+__device__ __forceinline__ float from_left(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
+// Fixed-order butterfly sum across the 64 lanes of a wave; every lane gets the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so neighbouring tiles (which share halo rows)
+// hit the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int per = (n + 7) >> 3;
+  const int xcd = b & 7, slot = b >> 3;
+  const int full = n - (per - 1) * 8;  // XCDs that get `per` tiles
+  int t;
+  if (xcd < full) t = xcd * per + slot;
+  else t = full * per + (xcd - full) * (per - 1) + slot;
+  return t;
+}
+
+// Stop rules for one finished sweep m (poisson.rs:202-229). Returns true when
+// the solve must halt at m.
+__device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
+  const double rel = sqrt(res2) / C.r0;
+  C.history[m - 1] = rel;
+  C.sweeps_final = m;
+  if (rel <= C.tol) {
+    C.reason = PH_REASON_CONVERGED;
+    C.halt_sweep = m;
+    return true;
+  }
+  if (rel < C.best * (1.0 - PH_STALL_IMPROVEMENT)) {
+    C.best = rel;
+    C.since = 0;
+  } else {
+    C.since += 1;
+    if (C.since >= PH_STALL_WINDOW) {
+      C.reason = PH_REASON_STALLED;
+      C.halt_sweep = m;
+      return true;
+    }
+  }
+  if (res2 <= C.refine_abs2) C.refine_pending = 1;
+  return false;
+}
+
+// Control for one instance after all of its tiles finished a launch. Runs on
+// one whole wave (the last tile to arrive).
+template <int K>
+__device__ void control(PhCtl &C, const PhInst &I, const double *partials, int lane) {
+  const uint32_t S = C.sweeps_done;
+  const int k_run = min(K, (int)(2 * (C.sweep_limit - S)));
+  const int nsw = k_run >> 1;
+  double A[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    double v = 0.0;
+    for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += partials[(size_t)t * K + s];
+    A[s] = wave_sum(v);
+  }
+  if (lane != 0) return;
+  C.launches += 1;
+  if (C.mode == PH_MODE_REDO) {
+    C.sweeps_done = S + nsw;
+    C.cur_buf ^= 1u;
+    C.halt = PH_HALT_REDO_DONE;
+    return;
+  }
+  const uint32_t in_buf = C.cur_buf;
+  bool stop = false;
+  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, S, C.om1sq * C.prev_black + A[0]);
+#pragma unroll
+  for (int j = 1; j < K / 2; ++j)
+    if (j < nsw && !stop) stop = finalize_sweep(C, S + j, C.om1sq * A[2 * j - 1] + A[2 * j]);
+  double last_black = A[0];
+#pragma unroll
+  for (int j = 1; j <= K / 2; ++j)
+    if (j == nsw) last_black = A[2 * j - 1];
+  C.prev_black = last_black;
+  if (stop) {
+    C.halt_base = S;
+    C.halt_buf = in_buf;
+    C.halt = PH_HALT_STOP;
+    return;
+  }
+  C.sweeps_done = S + nsw;
+  C.cur_buf = in_buf ^ 1u;
+  if (C.sweeps_done >= C.sweep_limit) C.halt = PH_HALT_LIMIT;
+  else if (C.refine_pending) C.halt = PH_HALT_REFINE;
+}
+
+// One SOR update of cell j of a lane's row, given its four neighbours.
+__device__ __forceinline__ float relax(float &e, float g, float l, float r, float u, float d,
+                                       float c, float dg) {
+  const float rp = fmaf(-dg, e, (((l + r) + u) + d) - g);
+  const float rr = (g == g) ? rp : 0.0f;  // g is NaN at non-interior cells
+  e = fmaf(c, rr, e);
+  return rr;
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, int ntiles,
+                                            const PhInst *__restrict__ insts, PhCtl *ctl,
+                                            double *__restrict__ partials,
+                                            uint32_t *__restrict__ arrivals, int hb) {
+  constexpr int CPL = 4;
+  constexpr int U = K + 2;  // live rows: t-K .. t+1
+  const int lane = threadIdx.x;
+  const int b = xcd_tile(blockIdx.x, ntiles);
+  const PhTile T = tiles[b];
+  const PhInst &I = insts[T.inst];
+  PhCtl &C = ctl[T.inst];
+
+  if (C.halt == PH_RUN) {
+    const int k_run = min(K, (int)(2 * (C.sweep_limit - C.sweeps_done)));
+    const uint32_t cur = C.cur_buf;
+    const float *__restrict__ ein = I.e[cur];
+    float *__restrict__ eout = I.e[cur ^ 1u];
+    const float *__restrict__ gp = I.g;
+    const size_t pitch = (size_t)I.pitch;
+    const int col0 = T.px0 + lane * CPL;
+    const float c = I.c, dg = I.dg;
+    bool cv[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) cv[j] = (lane * CPL + j >= K) && (lane * CPL + j < 64 * CPL - K);
+    bool lane_full = cv[0] && cv[CPL - 1];
+
+    float E[U][CPL], G[U][CPL];
+    float acc[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) acc[s] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < U; ++r)
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) E[r][j] = G[r][j] = 0.0f;
+
+    const int nrows = hb + 2 * K;
+    const int nsteps = nrows - 1;
+    const float *erow = ein + (size_t)T.py0 * pitch + col0;
+    const float *grow = gp + (size_t)T.py0 * pitch + col0;
+    float *orow = eout + (size_t)T.py0 * pitch + col0;
+    {
+      const float4 ev = *reinterpret_cast<const float4 *>(erow);
+      const float4 gv = *reinterpret_cast<const float4 *>(grow);
+      E[0][0] = ev.x; E[0][1] = ev.y; E[0][2] = ev.z; E[0][3] = ev.w;
+      G[0][0] = gv.x; G[0][1] = gv.y; G[0][2] = gv.z; G[0][3] = gv.w;
+    }
+
+    for (int t0 = 0; t0 < nsteps; t0 += U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u;
+        if (t < nsteps) {
+          {  // bring in row t+1 (t+1 <= nrows-1 always holds here)
+            constexpr int dummy = 0;
+            (void)dummy;
+            const int sl = (u + 1) % U;
+            const size_t off = (size_t)(t + 1) * pitch;
+            const float4 ev = *reinterpret_cast<const float4 *>(erow + off);
+            const float4 gv = *reinterpret_cast<const float4 *>(grow + off);
+            E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
+            G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
+          }
+#pragma unroll
+          for (int s = 1; s <= K; ++s) {
+            // stage s on row i = t - s + 1; rows above i < s never reach the valid core
+            if (s <= k_run && t >= 2 * s - 1) {
+              const int si = ((u - s + 1) % U + U) % U;
+              const int su = ((u - s) % U + U) % U;
+              const int sd = ((u - s + 2) % U + U) % U;
+              const int colour = (s - 1) & 1;
+              const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
+              const int i = t - s + 1;
+              const bool rv = (i >= K) && (i < K + hb);
+              float *Er = E[si];
+              const float *Eu = E[su];
+              const float *Ed = E[sd];
+              const float *Gr = G[si];
+              if (j0 == 0) {
+                const float l0 = from_left(Er[3]);
+                const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
+                const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
+                if (rv) {
+                  if (cv[0]) acc[s - 1] = fmaf(r0, r0, acc[s - 1]);
+                  if (cv[2]) acc[s - 1] = fmaf(r2, r2, acc[s - 1]);
+                }
+              } else {
+                const float r3n = from_right(Er[0]);
+                const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
+                const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
+                if (rv) {
+                  if (cv[1]) acc[s - 1] = fmaf(r1, r1, acc[s - 1]);
+                  if (cv[3]) acc[s - 1] = fmaf(r3, r3, acc[s - 1]);
+                }
+              }
+            }
+          }
+          // row t-K+1 has had all of its half-sweeps
+          const int w = t - K + 1;
+          if (w >= K && w < K + hb) {
+            const int sw = ((u - K + 1) % U + U) % U;
+            float *dst = orow + (size_t)w * pitch;
+            if (lane_full) {
+              *reinterpret_cast<float4 *>(dst) = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
+            } else {
+#pragma unroll
+              for (int j = 0; j < CPL; ++j)
+                if (cv[j]) dst[j] = E[sw][j];
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const double tot = wave_sum((double)acc[s]);
+      if (lane == 0) partials[(size_t)b * K + s] = tot;
+    }
+  }
+
+  // The last tile of the instance to arrive runs the instance's control.
+  __threadfence();
+  uint32_t ticket = 0;
+  if (lane == 0) ticket = atomicAdd(&arrivals[T.inst], 1u);
+  ticket = __shfl(ticket, 0, 64);
+  if (ticket == (uint32_t)(I.tile_end - I.tile_begin - 1)) {
+    __threadfence();
+    if (C.halt == PH_RUN) control<K>(C, I, partials, lane);
+    if (lane == 0) arrivals[T.inst] = 0u;
+  }
+}
+
+template __global__ void k_sor<2>(const PhTile *, int, const PhInst *, PhCtl *, double *,
+                                  uint32_t *, int);
+template __global__ void k_sor<4>(const PhTile *, int, const PhInst *, PhCtl *, double *,
+                                  uint32_t *, int);
+template __global__ void k_sor<8>(const PhTile *, int, const PhInst *, PhCtl *, double *,
+                                  uint32_t *, int);
+template __global__ void k_sor<12>(const PhTile *, int, const PhInst *, PhCtl *, double *,
+                                   uint32_t *, int);
+template __global__ void k_sor<16>(const PhTile *, int, const PhInst *, PhCtl *, double *,
+                                   uint32_t *, int);
+
+}  // namespace ph

@@ -1,0 +1,290 @@
+"""ctypes binding of libpaintop_hip.so (the C ABI in include/paintop_hip.h).
+
+This is the host side of the drop-in boundary: the structs mirror the header
+field for field, and every call goes through the C ABI. There is no CPU
+fallback: if the library is missing, loading raises ``HipLibraryMissing``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpaintop_hip.so")
+
+PH_OK, PH_ERROR, PH_UNAVAILABLE = 0, 1, 2
+PH_OP_POISSON_BLEND, PH_OP_SCREENED_POISSON = 0, 1
+PH_FLAG_TIMING, PH_FLAG_WARM_START_NET, PH_FLAG_FORCE_PLAIN = 0x1, 0x2, 0x4
+STOP_NAMES = {0: "converged", 1: "max-iterations", 2: "stalled"}
+# paintop_ir::ErrorClass in declaration order (paintop-ir/src/error.rs:23-37)
+ERROR_CLASSES = ("parse", "schema", "reference", "type", "semantic", "policy", "execution")
+
+# Every symbol the header declares; the library must export all of them.
+EXPORTED_SYMBOLS = (
+    "ph_abi_version",
+    "ph_probe",
+    "ph_force_unavailable",
+    "ph_poisson_solve",
+    "ph_poisson_solve_device",
+    "ph_fbm_device",
+    "ph_polygon_mask_device",
+    "ph_ellipse_mask_device",
+)
+
+
+class HipLibraryMissing(RuntimeError):
+    """libpaintop_hip.so is not built; run ``python -m paintop_amd.build``."""
+
+
+class PhSolveDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("channels", ctypes.c_uint32),
+        ("max_iterations", ctypes.c_uint32),
+        ("tolerance", ctypes.c_double),
+        ("lam", ctypes.c_double),
+        ("omega", ctypes.c_double),
+        ("flags", ctypes.c_uint32),
+        ("half_sweeps", ctypes.c_uint32),
+        ("op", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class PhReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("stop_reason", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("history_len", ctypes.c_uint32),
+        ("tolerance", ctypes.c_double),
+        ("final_residual", ctypes.c_double),
+        ("residual_history", ctypes.POINTER(ctypes.c_double)),
+        ("history_capacity", ctypes.c_uint32),
+        ("worst_channel", ctypes.c_uint32),
+        ("refinements", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32),
+        ("unknowns", ctypes.c_uint64),
+        ("unknown_sweeps", ctypes.c_uint64),
+        ("device_ms", ctypes.c_double),
+        ("omega_used", ctypes.c_double),
+        ("half_sweeps_used", ctypes.c_uint32),
+        ("warm_start_used", ctypes.c_uint32),
+        ("warm_start_rel", ctypes.c_double),
+        ("sweep_ms", ctypes.c_double),
+    ]
+
+
+class PhError(ctypes.Structure):
+    _fields_ = [
+        ("error_class", ctypes.c_int32),
+        ("code", ctypes.c_char * 64),
+        ("message", ctypes.c_char * 512),
+    ]
+
+
+class PhDeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 128),
+        ("arch", ctypes.c_char * 64),
+        ("compute_units", ctypes.c_int32),
+        ("device_index", ctypes.c_int32),
+        ("runtime_version", ctypes.c_int32),
+        ("driver_version", ctypes.c_int32),
+        ("hbm_bytes", ctypes.c_uint64),
+    ]
+
+
+class HipCallError(RuntimeError):
+    """A failed C-ABI call, carrying the paintop error class and code."""
+
+    def __init__(self, status: int, error_class: str, code: str, message: str):
+        super().__init__(f"{code}: {message}")
+        self.status = status
+        self.error_class = error_class
+        self.code = code
+        self.message = message
+
+
+_lib = None
+
+
+def lib():
+    """Load libpaintop_hip.so (once). Raises HipLibraryMissing if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipLibraryMissing(f"{LIB_PATH} is missing: run `python -m paintop_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.ph_abi_version.restype = ctypes.c_int32
+    L.ph_probe.argtypes = [ctypes.POINTER(PhDeviceInfo), ctypes.POINTER(PhError)]
+    L.ph_probe.restype = ctypes.c_int
+    L.ph_force_unavailable.argtypes = [ctypes.c_int32]
+    L.ph_poisson_solve.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp,
+                                   ctypes.POINTER(PhReport), ctypes.POINTER(PhError)]
+    L.ph_poisson_solve.restype = ctypes.c_int
+    L.ph_poisson_solve_device.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp, vp,
+                                          ctypes.POINTER(PhReport), ctypes.POINTER(PhError)]
+    L.ph_poisson_solve_device.restype = ctypes.c_int
+    L.ph_fbm_device.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
+                                ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_float, ctypes.c_float, vp, ctypes.POINTER(PhError)]
+    L.ph_fbm_device.restype = ctypes.c_int
+    L.ph_polygon_mask_device.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+                                         ctypes.c_int32, vp, ctypes.POINTER(PhError)]
+    L.ph_polygon_mask_device.restype = ctypes.c_int
+    L.ph_ellipse_mask_device.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, vp,
+                                         ctypes.POINTER(PhError)]
+    L.ph_ellipse_mask_device.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(status: int, err: PhError):
+    if status != PH_OK:
+        cls = int(err.error_class)
+        name = ERROR_CLASSES[cls] if 0 <= cls < len(ERROR_CLASSES) else "execution"
+        raise HipCallError(status, name, err.code.decode(), err.message.decode())
+
+
+@dataclass
+class SolveResult:
+    iterations: int
+    converged: bool
+    stop_reason: str
+    tolerance: float
+    final_residual: float
+    residual_history: list = field(default_factory=list)
+    worst_channel: int = 0
+    refinements: int = 0
+    launches: int = 0
+    unknowns: int = 0
+    unknown_sweeps: int = 0
+    device_ms: float = 0.0
+    omega: float = 0.0
+    half_sweeps: int = 0
+    warm_start_used: bool = False
+    warm_start_rel: float = 0.0
+    sweep_ms: float = 0.0
+
+
+def make_desc(width, height, channels, max_iterations, tolerance, lam=0.0, op=PH_OP_POISSON_BLEND,
+              omega=0.0, half_sweeps=0, flags=0) -> PhSolveDesc:
+    return PhSolveDesc(width=width, height=height, channels=channels,
+                       max_iterations=max_iterations, tolerance=tolerance, lam=lam,
+                       omega=omega, flags=flags, half_sweeps=half_sweeps, op=op, reserved=0)
+
+
+def _result(rep: PhReport, hist: np.ndarray) -> SolveResult:
+    return SolveResult(
+        iterations=int(rep.iterations),
+        converged=bool(rep.converged),
+        stop_reason=STOP_NAMES.get(int(rep.stop_reason), "unknown"),
+        tolerance=float(rep.tolerance),
+        final_residual=float(rep.final_residual),
+        residual_history=[float(v) for v in hist[: rep.history_len]],
+        worst_channel=int(rep.worst_channel),
+        refinements=int(rep.refinements),
+        launches=int(rep.launches),
+        unknowns=int(rep.unknowns),
+        unknown_sweeps=int(rep.unknown_sweeps),
+        device_ms=float(rep.device_ms),
+        omega=float(rep.omega_used),
+        half_sweeps=int(rep.half_sweeps_used),
+        warm_start_used=bool(rep.warm_start_used),
+        warm_start_rel=float(rep.warm_start_rel),
+        sweep_ms=float(rep.sweep_ms),
+    )
+
+
+def abi_version() -> int:
+    return int(lib().ph_abi_version())
+
+
+def probe() -> dict:
+    info, err = PhDeviceInfo(), PhError()
+    st = lib().ph_probe(ctypes.byref(info), ctypes.byref(err))
+    _check(st, err)
+    return {
+        "name": info.name.decode(),
+        "arch": info.arch.decode(),
+        "compute_units": int(info.compute_units),
+        "device_index": int(info.device_index),
+        "runtime_version": int(info.runtime_version),
+        "driver_version": int(info.driver_version),
+        "hbm_bytes": int(info.hbm_bytes),
+    }
+
+
+def force_unavailable(forced: bool) -> None:
+    lib().ph_force_unavailable(1 if forced else 0)
+
+
+def poisson_solve(desc: PhSolveDesc, source: np.ndarray, target: np.ndarray, mask: np.ndarray):
+    """Host-buffer solve: returns (candidate f32 array shaped like target, SolveResult)."""
+    source = np.ascontiguousarray(source, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.float32)
+    mask = np.ascontiguousarray(mask, dtype=np.float32)
+    out = np.empty_like(target)
+    cap = int(desc.max_iterations)
+    hist = np.zeros(max(cap, 1), dtype=np.float64)
+    rep = PhReport()
+    rep.residual_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rep.history_capacity = cap
+    err = PhError()
+    st = lib().ph_poisson_solve(ctypes.byref(desc), source.ctypes.data, target.ctypes.data,
+                                mask.ctypes.data, out.ctypes.data, ctypes.byref(rep),
+                                ctypes.byref(err))
+    _check(st, err)
+    return out, _result(rep, hist)
+
+
+def poisson_solve_device(desc: PhSolveDesc, d_source: int, d_target: int, d_mask: int,
+                         d_out: int, stream: int = 0, want_history: bool = True) -> SolveResult:
+    """Device-buffer solve on raw device pointers (e.g. torch ``data_ptr()``)."""
+    cap = int(desc.max_iterations)
+    hist = np.zeros(max(cap, 1), dtype=np.float64)
+    rep = PhReport()
+    if want_history:
+        rep.residual_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rep.history_capacity = cap
+    err = PhError()
+    st = lib().ph_poisson_solve_device(ctypes.byref(desc), d_source, d_target, d_mask, d_out,
+                                       stream or None, ctypes.byref(rep), ctypes.byref(err))
+    _check(st, err)
+    return _result(rep, hist)
+
+
+def fbm_device(d_out: int, width, height, frequency, seed, octaves, lacunarity=2.0, gain=0.5,
+               channel_stride=1, channel_offset=0, scale=1.0, bias=0.0, stream: int = 0):
+    err = PhError()
+    st = lib().ph_fbm_device(d_out, width, height, channel_stride, channel_offset, frequency,
+                             seed & 0xFFFFFFFFFFFFFFFF, octaves, lacunarity, gain, scale, bias,
+                             stream or None, ctypes.byref(err))
+    _check(st, err)
+
+
+def polygon_mask_device(d_out: int, width, height, points, even_odd=False, stream: int = 0):
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1))
+    err = PhError()
+    st = lib().ph_polygon_mask_device(d_out, width, height,
+                                      pts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      len(pts) // 2, 1 if even_odd else 0, stream or None,
+                                      ctypes.byref(err))
+    _check(st, err)
+
+
+def ellipse_mask_device(d_out: int, width, height, cx, cy, rx, ry, stream: int = 0):
+    err = PhError()
+    st = lib().ph_ellipse_mask_device(d_out, width, height, cx, cy, rx, ry, stream or None,
+                                      ctypes.byref(err))
+    _check(st, err)

@@ -1,0 +1,160 @@
+"""Seeded synthetic workloads for the Poisson path.
+
+Large images are generated on the device (``ph_fbm_device``,
+``ph_polygon_mask_device``), so a 4096x4096 RGBA pair never crosses PCIe.
+Small host-side cases use numpy. Every random draw comes from a SplitMix64
+stream whose finalizer is paintop's ``mix64`` (crates/paintop-cpu/src/noise.rs:72-76).
+
+Workloads:
+  gaussian_bump_case   256x256 Gray: Gaussian-bump target, Laplacian guidance of a
+                       second bump field, centred 80x60 ellipse mask (omega 1.9, 1e-5).
+  seamless_clone_case  4096x4096 RGBA fbm source/target (frequency 0.125, 6 octaves,
+                       gain 0.5, mapped to [0, 1]), alpha 1, 12-vertex star mask
+                       covering ~30% of the image; solved to 1e-6.
+  screened_case        8192x8192 Gray screened Poisson, lambda 0.01, full image
+                       minus a 64 px border; solved to 1e-7.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_M64 = 0xFFFFFFFFFFFFFFFF
+
+
+def mix64(z: int) -> int:
+    z &= _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & _M64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        return mix64(self.state)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / 9007199254740992.0))
+
+    def uniform_array(self, n: int, lo: float, hi: float) -> np.ndarray:
+        return np.array([self.uniform(lo, hi) for _ in range(n)], dtype=np.float64)
+
+
+# --- host-side small cases -------------------------------------------------------
+
+
+def bump_field(width: int, height: int, seed: int, bumps: int = 8) -> np.ndarray:
+    """Seeded sum of isotropic Gaussian bumps (sigma 10-40 px, amplitude
+    U(0.2, 0.8)) plus U(0, 0.02) noise, clamped to [0, 1]; f32 (H, W)."""
+    rng = SplitMix64(seed)
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    x += 0.5
+    y += 0.5
+    f = np.zeros((height, width), dtype=np.float64)
+    for _ in range(bumps):
+        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
+        sig = rng.uniform(10.0, 40.0)
+        amp = rng.uniform(0.2, 0.8)
+        f += amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2.0 * sig * sig))
+    noise = rng.uniform_array(width * height, 0.0, 0.02).reshape(height, width)
+    return np.clip(f + noise, 0.0, 1.0).astype(np.float32)
+
+
+def ellipse_mask(width: int, height: int, cx: float, cy: float, rx: float, ry: float):
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    d = ((x + 0.5 - cx) / rx) ** 2 + ((y + 0.5 - cy) / ry) ** 2
+    return (d <= 1.0).astype(np.float32)
+
+
+def gaussian_bump_case(size: int = 256):
+    """The 256x256 Gray CPU-reference case: (source, target, mask) f32 (H, W).
+    The op's guidance is the Laplacian of ``source`` (seed 2); the target and
+    Dirichlet boundary come from ``target`` (seed 1)."""
+    target = bump_field(size, size, seed=1)
+    source = bump_field(size, size, seed=2)
+    m = ellipse_mask(size, size, size / 2.0, size / 2.0, 80.0 * size / 256, 60.0 * size / 256)
+    return source, target, m
+
+
+def star_polygon(cx: float, cy: float, r_outer: float, inner_ratio: float = 0.5,
+                 points: int = 6, jitter: float = 0.1, seed: int = 301):
+    """A 2*points-vertex star, radii alternating r_outer / inner_ratio*r_outer,
+    each jittered radially by a seeded U(-jitter, +jitter) factor."""
+    rng = SplitMix64(seed)
+    verts = []
+    for k in range(2 * points):
+        r = r_outer if k % 2 == 0 else r_outer * inner_ratio
+        r *= 1.0 + rng.uniform(-jitter, jitter)
+        a = math.pi * k / points - math.pi / 2
+        verts.append((cx + r * math.cos(a), cy + r * math.sin(a)))
+    return verts
+
+
+def random_polygon(cx, cy, rx, ry, n, rng: SplitMix64):
+    """An n-vertex star-shaped polygon inside the ellipse (rx, ry)."""
+    verts = []
+    for k in range(n):
+        a = 2 * math.pi * (k + rng.uniform(-0.3, 0.3)) / n
+        s = rng.uniform(0.55, 1.0)
+        verts.append((cx + s * rx * math.cos(a), cy + s * ry * math.sin(a)))
+    return verts
+
+
+# --- device-side large cases -----------------------------------------------------
+
+
+def seamless_clone_case(size: int = 4096, device: str = "cuda", channels: int = 4,
+                        star_seed: int = 301):
+    """4096x4096 RGBA fbm blend generated in HBM; returns torch tensors
+    (source, target, mask) with source/target (H, W, C) and mask (H, W)."""
+    import torch
+
+    from . import hip
+
+    src = torch.empty((size, size, channels), dtype=torch.float32, device=device)
+    tgt = torch.empty_like(src)
+    msk = torch.empty((size, size), dtype=torch.float32, device=device)
+    stream = torch.cuda.current_stream().cuda_stream
+    for c in range(min(channels, 3)):
+        hip.fbm_device(src.data_ptr(), size, size, 0.125, 101 + c, 6, 2.0, 0.5,
+                       channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5,
+                       stream=stream)
+        hip.fbm_device(tgt.data_ptr(), size, size, 0.125, 201 + c, 6, 2.0, 0.5,
+                       channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5,
+                       stream=stream)
+    if channels == 4:
+        src[:, :, 3] = 1.0
+        tgt[:, :, 3] = 1.0
+    r_outer = math.sqrt(0.30 * size * size / 1.5)
+    verts = star_polygon(size / 2.0, size / 2.0, r_outer, 0.5, 6, 0.1, star_seed)
+    hip.polygon_mask_device(msk.data_ptr(), size, size, verts, stream=stream)
+    torch.cuda.synchronize()
+    return src, tgt, msk
+
+
+def screened_case(size: int = 8192, border: int = 64, device: str = "cuda"):
+    """8192x8192 Gray screened case: anchor = (fbm+1)/2 (seed 401), guidance =
+    (fbm2+1)/2 + N(0, 0.01) (seeds 402, 403), mask = 1 except a 64 px border."""
+    import torch
+
+    from . import hip
+
+    stream = torch.cuda.current_stream().cuda_stream
+    anchor = torch.empty((size, size), dtype=torch.float32, device=device)
+    guide = torch.empty_like(anchor)
+    hip.fbm_device(anchor.data_ptr(), size, size, 0.125, 401, 6, 2.0, 0.5, scale=0.5, bias=0.5,
+                   stream=stream)
+    hip.fbm_device(guide.data_ptr(), size, size, 0.125, 402, 6, 2.0, 0.5, scale=0.5, bias=0.5,
+                   stream=stream)
+    g = torch.Generator(device=device)
+    g.manual_seed(403)
+    guide += 0.01 * torch.randn(guide.shape, generator=g, device=device, dtype=torch.float32)
+    msk = torch.zeros_like(anchor)
+    msk[border:size - border, border:size - border] = 1.0
+    torch.cuda.synchronize()
+    return guide, anchor, msk

@@ -1,0 +1,252 @@
+"""The hip backend's implementations of the two Poisson ops.
+
+``PoissonBlend`` and ``ScreenedPoisson`` have the reference ops' contract
+(crates/paintop-cpu/src/poisson_blend.rs, crates/paintop-cpu/src/screened_poisson.rs):
+the same input/output ports, the same plan params with the same defaults,
+ranges, error codes and classes, the same candidate layout (the target's
+descriptor) and the same report (``SolverData`` of the worst channel). The
+compute runs on the GPU through the C ABI (``ph_poisson_solve``); there is no
+CPU path.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+from . import hip
+from .errors import E_GPU_UNAVAILABLE, ErrorClass, PaintopError
+from .values import ImageDescriptor, MaskDescriptor, Report, ResourceValue, SolverData
+
+POISSON_BLEND_OP_ID = "repair.poisson_blend@1"
+SCREENED_POISSON_OP_ID = "repair.screened_poisson@1"
+HIP_IMPL_ID = "hip.poisson@1"
+
+# poisson_blend.rs:53-78
+E_POISSON_BLEND_INPUT = "E_POISSON_BLEND_INPUT"
+E_POISSON_BLEND_SHAPE = "E_POISSON_BLEND_SHAPE"
+E_POISSON_BLEND_PARAM = "E_POISSON_BLEND_PARAM"
+E_POISSON_BLEND_BUFFER = "E_POISSON_BLEND_BUFFER"
+DEFAULT_MAX_ITERATIONS = 2000
+MAX_ITERATIONS_LIMIT = 100_000
+DEFAULT_TOLERANCE = 1e-6
+# screened_poisson.rs:50-63
+E_SCREENED_POISSON_INPUT = "E_SCREENED_POISSON_INPUT"
+E_SCREENED_POISSON_PARAM = "E_SCREENED_POISSON_PARAM"
+E_SCREENED_POISSON_BUFFER = "E_SCREENED_POISSON_BUFFER"
+DEFAULT_LAMBDA = 0.0
+
+_U32_MAX = 2**32 - 1
+_MANIFEST_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "manifests")
+
+
+def load_manifest(op_id: str) -> dict:
+    """The op manifest with ``hip.poisson@1`` declared beside ``cpu.reference@1``."""
+    with open(os.path.join(_MANIFEST_DIR, op_id + ".json"), encoding="utf-8") as f:
+        return json.load(f)
+
+
+# --- serde_json accessors -------------------------------------------------------
+
+
+def _as_u64(v):
+    """serde_json ``Value::as_u64``: only a non-negative JSON integer."""
+    if isinstance(v, bool) or not isinstance(v, int) or v < 0 or v > 2**64 - 1:
+        return None
+    return v
+
+
+def _as_f64(v):
+    """serde_json ``Value::as_f64``: any JSON number."""
+    if isinstance(v, bool) or not isinstance(v, (int, float)):
+        return None
+    return float(v)
+
+
+def _param_error(code: str, message: str, value) -> PaintopError:
+    return PaintopError(ErrorClass.SCHEMA, code, message, actual=json.dumps(value))
+
+
+def _resolve_iterations(params: dict, code: str, prefix: str) -> int:
+    if "max_iterations" not in params:
+        return DEFAULT_MAX_ITERATIONS
+    value = params["max_iterations"]
+    n = _as_u64(value)
+    if n is None or n > _U32_MAX:
+        raise _param_error(code, f"{prefix}max_iterations must be a positive integer", value)
+    if n == 0 or n > MAX_ITERATIONS_LIMIT:
+        raise _param_error(code, f"{prefix}max_iterations must be in 1..={MAX_ITERATIONS_LIMIT}",
+                           value)
+    return n
+
+
+def _resolve_tolerance(params: dict, code: str, prefix: str) -> float:
+    if "tolerance" not in params:
+        return DEFAULT_TOLERANCE
+    value = params["tolerance"]
+    t = _as_f64(value)
+    if t is None:
+        raise _param_error(code, f"{prefix}tolerance must be a number", value)
+    if not math.isfinite(t) or t <= 0.0 or t >= 1.0:
+        raise _param_error(code, f"{prefix}tolerance must be in (0, 1)", value)
+    return t
+
+
+def resolve_blend_controls(params: dict) -> tuple:
+    """poisson_blend.rs:102-139 -> (max_iterations, tolerance, lambda=0)."""
+    params = params or {}
+    pre = "repair.poisson_blend: "
+    n = _resolve_iterations(params, E_POISSON_BLEND_PARAM, pre)
+    t = _resolve_tolerance(params, E_POISSON_BLEND_PARAM, pre)
+    return n, t, 0.0
+
+
+def resolve_screened_controls(params: dict) -> tuple:
+    """screened_poisson.rs:66-112 -> (max_iterations, tolerance, lambda)."""
+    params = params or {}
+    pre = "repair.screened_poisson: "
+    lam = DEFAULT_LAMBDA
+    if "lambda" in params:
+        value = params["lambda"]
+        lv = _as_f64(value)
+        if lv is None:
+            raise _param_error(E_SCREENED_POISSON_PARAM, pre + "lambda must be a number", value)
+        if not math.isfinite(lv) or lv < 0.0:
+            raise _param_error(E_SCREENED_POISSON_PARAM,
+                               pre + "lambda must be a finite value >= 0", value)
+        lam = lv
+    n = _resolve_iterations(params, E_SCREENED_POISSON_PARAM, pre)
+    t = _resolve_tolerance(params, E_SCREENED_POISSON_PARAM, pre)
+    return n, t, lam
+
+
+def check_shapes(source: ImageDescriptor, target: ImageDescriptor, mask_extent, op: str):
+    """poisson_blend.rs:417-445 (both ops use E_POISSON_BLEND_SHAPE)."""
+    if source.extent != target.extent:
+        raise PaintopError(ErrorClass.SEMANTIC, E_POISSON_BLEND_SHAPE,
+                           f"{op}: the source and target images must share an extent",
+                           actual=f"source {source.extent} vs target {target.extent}")
+    if source.layout != target.layout:
+        raise PaintopError(ErrorClass.SEMANTIC, E_POISSON_BLEND_SHAPE,
+                           f"{op}: the source and target images must share a channel layout",
+                           actual=f"source {source.layout} vs target {target.layout}")
+    if mask_extent != target.extent:
+        raise PaintopError(ErrorClass.SEMANTIC, E_POISSON_BLEND_SHAPE,
+                           f"{op}: the mask must share the images' extent",
+                           actual=f"mask {mask_extent} vs image {target.extent}")
+    return target
+
+
+def solver_data(kind: str, r: hip.SolveResult) -> SolverData:
+    """poisson_blend.rs:249-266."""
+    return SolverData(kind=kind, steps=r.iterations, stability_number=0.0, stability_limit=0.0,
+                      stable=True, residual_history=list(r.residual_history), total_energy=0.0,
+                      iterations=r.iterations, stop_reason=r.stop_reason,
+                      converged=r.converged, tolerance=r.tolerance,
+                      final_residual=r.final_residual)
+
+
+class _HipPoissonOp:
+    """Shared compute of the two ops (the blend_channels role)."""
+
+    op_id = ""
+    op_name = ""
+    ports = ("source", "target", "mask")
+    input_code = ""
+    buffer_code = ""
+    kind = ""
+    ph_op = hip.PH_OP_POISSON_BLEND
+
+    def __init__(self, omega: float = 0.0, half_sweeps: int = 0):
+        # SOR factor and half-sweeps per launch: tuned when 0. These are backend
+        # tuning knobs, not plan params (the reference fixes omega = 1.8 in its
+        # own backend, poisson_blend.rs:78).
+        self.omega = omega
+        self.half_sweeps = half_sweeps
+        self.last_result: hip.SolveResult | None = None
+
+    @classmethod
+    def manifest(cls) -> dict:
+        return load_manifest(cls.op_id)
+
+    def _resolve(self, params):
+        raise NotImplementedError
+
+    def _input(self, inputs, port):
+        if port not in inputs:
+            raise PaintopError(ErrorClass.REFERENCE, self.input_code,
+                               f"{self.op_name} requires a `{port}` input value")
+        return inputs[port]
+
+    def compute(self, inputs: dict, params: dict) -> dict:
+        """OpImplementation::compute (op_impl.rs:51-70) for this op."""
+        max_it, tol, lam = self._resolve(params)
+        a, b, m = (self._input(inputs, p) for p in self.ports)
+        if not isinstance(a.descriptor, ImageDescriptor):
+            raise PaintopError(ErrorClass.TYPE, self.input_code,
+                               f"{self.op_name} `{self.ports[0]}` input must be a image resource")
+        if not isinstance(b.descriptor, ImageDescriptor):
+            raise PaintopError(ErrorClass.TYPE, self.input_code,
+                               f"{self.op_name} `{self.ports[1]}` input must be a image resource")
+        if not isinstance(m.descriptor, MaskDescriptor):
+            raise PaintopError(ErrorClass.TYPE, self.input_code,
+                               f"{self.op_name} `mask` input must be a mask resource")
+        out_desc = check_shapes(a.descriptor, b.descriptor, m.descriptor.extent, self.op_name)
+        channels = out_desc.channels
+        ext = out_desc.extent
+        desc = hip.make_desc(ext.width, ext.height, channels, max_it, tol, lam, op=self.ph_op,
+                             omega=self.omega, half_sweeps=self.half_sweeps)
+        try:
+            out, res = hip.poisson_solve(desc, a.samples, b.samples, m.samples)
+        except hip.HipCallError as e:
+            cls = ErrorClass(e.error_class)
+            raise PaintopError(cls, e.code, e.message) from None
+        self.last_result = res
+        samples = out.reshape(-1)
+        all_finite = bool(np.isfinite(samples).all())
+        try:
+            candidate = ResourceValue(out_desc, channels, samples)
+        except PaintopError:
+            raise PaintopError(ErrorClass.EXECUTION, self.buffer_code,
+                               f"{self.op_name} produced a candidate buffer of unexpected length "
+                               f"{samples.size}") from None
+        report = Report(extent=ext, channels=channels, all_finite=all_finite,
+                        solver=solver_data(self.kind, res))
+        return {"candidate": candidate, "report": ResourceValue.report_value(report)}
+
+
+class PoissonBlend(_HipPoissonOp):
+    """repair.poisson_blend@1 on the GPU (poisson_blend.rs:535-600)."""
+
+    op_id = POISSON_BLEND_OP_ID
+    op_name = "repair.poisson_blend"
+    ports = ("source", "target", "mask")
+    input_code = E_POISSON_BLEND_INPUT
+    buffer_code = E_POISSON_BLEND_BUFFER
+    kind = "poisson"
+    ph_op = hip.PH_OP_POISSON_BLEND
+
+    def _resolve(self, params):
+        return resolve_blend_controls(params)
+
+
+class ScreenedPoisson(_HipPoissonOp):
+    """repair.screened_poisson@1 on the GPU (screened_poisson.rs:362-430)."""
+
+    op_id = SCREENED_POISSON_OP_ID
+    op_name = "repair.screened_poisson"
+    ports = ("guidance", "anchor", "mask")
+    input_code = E_SCREENED_POISSON_INPUT
+    buffer_code = E_SCREENED_POISSON_BUFFER
+    kind = "screened-poisson"
+    ph_op = hip.PH_OP_SCREENED_POISSON
+
+    def _resolve(self, params):
+        return resolve_screened_controls(params)
+
+
+def gpu_unavailable(reason: str) -> PaintopError:
+    return PaintopError(ErrorClass.POLICY, E_GPU_UNAVAILABLE,
+                        f"no compatible GPU adapter is available: {reason}")

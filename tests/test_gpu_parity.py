@@ -1,0 +1,312 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: classification and every non-interior sample bit-exact; interior within
+the bounded-tier envelope of paintop's differential harness (max-abs 1e-4,
+RMS 1e-5, paintop-testkit/src/differential.rs:58-62) of the oracle run to the
+same relative residual, with the error against an exact sparse direct solve
+stated; stop reasons, iteration caps and history lengths as the reference's
+tests pin them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from paintop_amd import hip, ops
+from paintop_amd.inputs import gaussian_bump_case, star_polygon
+from paintop_amd.values import image, mask
+from tests.helpers import classify, direct_solve, laplacian, residual_norm
+
+pytestmark = pytest.mark.gpu
+
+BOUNDED_MAX_ABS = 1e-4  # differential.rs:58-62
+BOUNDED_RMS = 1e-5
+
+
+def run_op(op, a, b, m, params, layout="Gray", **tuning):
+    h, w = m.shape
+    names = ("source", "target") if isinstance(op, type) and op is ops.PoissonBlend else (
+        "guidance", "anchor")
+    inputs = {names[0]: image(w, h, a, layout), names[1]: image(w, h, b, layout),
+              "mask": mask(w, h, m)}
+    impl = op(**tuning)
+    out = impl.compute(inputs, params)
+    return out["candidate"].array(), out["report"].as_report().solver, impl.last_result
+
+
+def square_mask(w, h, lo, hi):
+    m = np.zeros((h, w), np.float32)
+    m[lo:hi, lo:hi] = 1.0
+    return m
+
+
+def test_probe_reports_gfx950(gpu):
+    info = gpu.probe()
+    assert info["arch"].startswith("gfx950") and info["compute_units"] > 0
+
+
+# ---- the reference's op tests on the GPU (poisson_blend/tests.rs) ----------------
+
+
+def test_boundary_pixels_keep_the_target(gpu):
+    w = h = 16
+    src = np.full((h, w), 0.9, np.float32)
+    tgt = np.full((h, w), 0.1, np.float32)
+    m = square_mask(w, h, 5, 11)
+    out, _, _ = run_op(ops.PoissonBlend, src, tgt, m, {})
+    assert out[..., 0][m <= 0.5].tobytes() == tgt[m <= 0.5].tobytes()
+
+
+def test_uniform_source_over_uniform_target_yields_target_constant(gpu):
+    w = h = 20
+    m = square_mask(w, h, 6, 14)
+    out, sd, _ = run_op(ops.PoissonBlend, np.full((h, w), 0.7, np.float32),
+                        np.full((h, w), 0.25, np.float32), m,
+                        {"tolerance": 1e-8, "max_iterations": 5000})
+    # flat source over flat target: r0 == 0, 0 iterations, the target itself
+    assert np.all(np.abs(out[..., 0][m > 0.5] - 0.25) < 1e-3)
+    assert sd.kind == "poisson" and sd.converged and sd.iterations == 0
+
+
+def test_linear_gradient_source_is_reproduced_inside_mask(gpu):
+    w = h = 24
+    src = np.tile(np.arange(w, dtype=np.float32), (h, 1))
+    tgt = src.copy()
+    tgt[6:18, 6:18] = 0.0  # start away from the answer so the solver works
+    m = square_mask(w, h, 6, 18)
+    out, sd, _ = run_op(ops.PoissonBlend, src, tgt, m, {"tolerance": 1e-9, "max_iterations": 8000})
+    o = out[..., 0]
+    assert sd.converged and sd.iterations > 0
+    assert all(abs((o[r, c] - o[r, c - 1]) - 1.0) < 5e-2 for r in range(7, 17)
+               for c in range(7, 17))
+
+
+def test_reruns_are_bit_identical(gpu):
+    w, h = 18, 14
+    src = (((np.arange(w * h) * 7) % 13) / 13.0).astype(np.float32).reshape(h, w)
+    tgt = np.full((h, w), 0.3, np.float32)
+    m = square_mask(w, h, 4, 12)
+    a, sa, _ = run_op(ops.PoissonBlend, src, tgt, m, {"max_iterations": 200})
+    b, sb, _ = run_op(ops.PoissonBlend, src, tgt, m, {"max_iterations": 200})
+    assert a.tobytes() == b.tobytes() and sa.residual_history == sb.residual_history
+
+
+def test_report_carries_iterative_metrics(gpu):
+    w = h = 16
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    src = (np.float32(0.01) * (x * x + y * y)).astype(np.float32)
+    m = square_mask(w, h, 4, 12)
+    _, sd, _ = run_op(ops.PoissonBlend, src, np.full((h, w), 0.2, np.float32), m,
+                      {"max_iterations": 3, "tolerance": 1e-12})
+    assert sd.iterations == 3 and sd.stop_reason == "max-iterations" and sd.converged is False
+    assert len(sd.residual_history) == 3 and sd.tolerance > 0
+    assert math.isfinite(sd.final_residual) and sd.steps == 3 and sd.stable
+
+
+@pytest.mark.parametrize("cap", [1, 2, 7, 9])
+def test_iteration_cap_is_exact(gpu, cap):
+    # characterization.rs:225: a tight cap stops at exactly the cap
+    w = h = 64
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    src = (np.float32(0.001) * (x * x + y * y)).astype(np.float32)
+    m = square_mask(w, h, 16, 48)
+    _, sd, _ = run_op(ops.PoissonBlend, src, np.full((h, w), 0.25, np.float32), m,
+                      {"max_iterations": cap, "tolerance": 1e-12})
+    assert sd.iterations == cap and sd.stop_reason == "max-iterations"
+    assert len(sd.residual_history) == cap
+
+
+def test_no_interior_region_stops_immediately(gpu):
+    w = h = 32
+    tgt = np.full((h, w), 0.5, np.float32)
+    out, sd, _ = run_op(ops.PoissonBlend, np.random.default_rng(0).random((h, w), np.float32),
+                        tgt, np.zeros((h, w), np.float32), {"max_iterations": 1000})
+    assert sd.iterations == 0 and sd.converged and sd.residual_history == []
+    assert out[..., 0].tobytes() == tgt.tobytes()
+
+
+def test_rejections_are_reported_before_launch(gpu):
+    with pytest.raises(ops.PaintopError) as ei:
+        run_op(ops.PoissonBlend, np.zeros((8, 8), np.float32), np.zeros((8, 8), np.float32),
+               np.zeros((8, 8), np.float32), {"tolerance": 2.0})
+    assert ei.value.code == "E_POISSON_BLEND_PARAM"
+
+
+# ---- screened_poisson/tests.rs on the GPU ------------------------------------------
+
+
+def bump(w, h):
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    return (np.float32(0.02) * (x * x + y * y)).astype(np.float32)
+
+
+def test_screened_large_lambda_snaps_to_anchor(gpu):
+    w = h = 20
+    m = square_mask(w, h, 6, 14)
+    out, sd, _ = run_op(ops.ScreenedPoisson, bump(w, h), np.full((h, w), 0.4, np.float32), m,
+                        {"lambda": 1.0e6, "tolerance": 1e-9, "max_iterations": 8000})
+    assert np.all(np.abs(out[..., 0][m > 0.5] - 0.4) < 1e-2) and sd.kind == "screened-poisson"
+
+
+def test_larger_lambda_moves_closer_to_anchor(gpu):
+    w = h = 20
+    m = square_mask(w, h, 6, 14)
+    anchor = np.full((h, w), 0.5, np.float32)
+
+    def dist(lam):
+        out, _, _ = run_op(ops.ScreenedPoisson, bump(w, h), anchor, m,
+                           {"lambda": lam, "tolerance": 1e-8, "max_iterations": 8000})
+        return abs(out[h // 2, w // 2, 0] - 0.5)
+
+    assert dist(10.0) < dist(0.01)
+
+
+def test_screened_boundary_and_report(gpu):
+    w = h = 16
+    g = np.tile(np.arange(w, dtype=np.float32), (h, 1))
+    anchor = np.full((h, w), 0.3, np.float32)
+    m = square_mask(w, h, 5, 11)
+    out, _, _ = run_op(ops.ScreenedPoisson, g, anchor, m, {"lambda": 5.0})
+    assert out[..., 0][m <= 0.5].tobytes() == anchor[m <= 0.5].tobytes()
+    _, sd, _ = run_op(ops.ScreenedPoisson, bump(w, h), np.full((h, w), 0.2, np.float32),
+                      square_mask(w, h, 4, 12),
+                      {"lambda": 1.0, "max_iterations": 3, "tolerance": 1e-12})
+    assert sd.kind == "screened-poisson" and sd.iterations == 3 and len(sd.residual_history) == 3
+
+
+# ---- parity with the oracle ------------------------------------------------------
+
+
+def check_parity(oracle_out, gpu_out, m, target, label):
+    inside = classify(m).astype(bool)
+    if target.ndim == 2:
+        target = target[..., None]
+        oracle_out = oracle_out.reshape(gpu_out.shape)
+    for c in range(gpu_out.shape[-1]):
+        g, o, t = gpu_out[..., c], oracle_out[..., c], target[..., c]
+        assert g[~inside].tobytes() == t[~inside].tobytes(), f"{label}: outside-mask bits ch{c}"
+        d = np.abs(g.astype(np.float64) - o.astype(np.float64))[inside]
+        if d.size:
+            print(f"[parity] {label} ch{c}: max_abs {d.max():.3e} rms "
+                  f"{np.sqrt(np.mean(d * d)):.3e}", flush=True)
+            assert d.max() <= BOUNDED_MAX_ABS, label
+            assert np.sqrt(np.mean(d * d)) <= BOUNDED_RMS, label
+
+
+def test_gaussian_bump_case_parity(gpu, oracle):
+    """256x256 Gray, omega 1.9, 1e-5 -- the CPU-reference configuration."""
+    src, tgt, m = gaussian_bump_case(256)
+    ref, ref_rep = oracle.blend_channels(src, tgt, m, 0.0, 5000, 1e-5, omega=1.9)
+    out, sd, res = run_op(ops.PoissonBlend, src, tgt, m,
+                          {"tolerance": 1e-5, "max_iterations": 5000}, omega=1.9)
+    print(f"[parity] bump: oracle {ref_rep.iterations} sweeps, gpu {sd.iterations} sweeps "
+          f"(refinements {res.refinements}, K {res.half_sweeps})", flush=True)
+    assert sd.converged and ref_rep.converged and sd.final_residual <= 1e-5
+    assert len(sd.residual_history) == sd.iterations
+    assert oracle.classify_cells(m).tobytes() == classify(m).tobytes()
+    check_parity(ref, out, m, tgt, "bump")
+    # error of both against the exact solution of the same system
+    cells = classify(m)
+    rhs = np.where(cells == 1, laplacian(src), 0.0)
+    exact = direct_solve(cells, rhs, tgt.astype(np.float64), 0.0)
+    eg = np.abs(out[..., 0] - exact)[cells == 1].max()
+    eo = np.abs(ref[..., 0] - exact)[cells == 1].max()
+    print(f"[parity] bump: max error vs direct FP64 solve: gpu {eg:.3e}, oracle {eo:.3e}",
+          flush=True)
+    # the GPU's own FP64 stopping decision holds for the returned field (f32-rounded)
+    r0 = residual_norm(cells, rhs, tgt, 0.0, tgt)
+    assert residual_norm(cells, rhs, tgt, 0.0, out[..., 0]) / r0 < 1e-4
+
+
+@pytest.mark.parametrize("lam", [0.0, 0.01, 1.0])
+def test_star_mask_rgba_parity(gpu, oracle, lam):
+    """A 192x160 RGBA blend with a seeded star mask and fbm-like inputs:
+    alpha constant (passes through at 0 iterations), three solved channels."""
+    w, h = 192, 160
+    rng = np.random.default_rng(12)
+    src = np.empty((h, w, 4), np.float32)
+    tgt = np.empty((h, w, 4), np.float32)
+    for c in range(3):
+        src[..., c] = oracle.fbm(w, h, 2.0, 101 + c, 6) * 0.5 + 0.5
+        tgt[..., c] = oracle.fbm(w, h, 2.0, 201 + c, 6) * 0.5 + 0.5
+    src[..., 3] = 1.0
+    tgt[..., 3] = 1.0
+    verts = np.array(star_polygon(w / 2, h / 2, 70.0, 0.5, 6, 0.1, 301))
+    m = np.zeros((h, w), np.float32)
+    yy, xx = np.mgrid[0:h, 0:w]
+    px, py = xx + 0.5, yy + 0.5
+    inside = np.zeros((h, w), bool)
+    n = len(verts)
+    winding = np.zeros((h, w), np.int32)
+    for i in range(n):
+        x0, y0 = verts[i]
+        x1, y1 = verts[(i + 1) % n]
+        b0, b1 = y0 <= py, y1 <= py
+        cross = b0 != b1
+        t = np.where(cross, (py - y0) / np.where(y1 != y0, y1 - y0, 1.0), 0.0)
+        cx = t * (x1 - x0) + x0
+        hit = cross & (cx > px)
+        winding += np.where(hit, np.where(b0, 1, -1), 0)
+    inside = winding != 0
+    m[inside] = 1.0
+    ref, rrep = oracle.blend_channels(src, tgt, m, lam, 100000, 1e-6, omega=1.9)
+    op = ops.ScreenedPoisson if lam > 0 else ops.PoissonBlend
+    params = {"tolerance": 1e-6, "max_iterations": 100000}
+    if lam > 0:
+        params["lambda"] = lam
+    out, sd, res = run_op(op, src, tgt, m, params, layout="Rgba")
+    print(f"[parity] star lam={lam}: oracle {rrep.iterations} gpu {sd.iterations} sweeps "
+          f"omega {res.omega:.4f} refinements {res.refinements}", flush=True)
+    assert sd.converged and rrep.converged
+    assert out[..., 3].tobytes() == tgt[..., 3].tobytes()
+    check_parity(ref, out, m, tgt, f"star lam={lam}")
+
+
+def test_results_do_not_depend_on_half_sweeps_per_launch(gpu):
+    """The register-streamed schedule applies exactly the red/black updates of
+    a global sweep: with the same omega and the same number of sweeps the field
+    is bit-identical for every K."""
+    src, tgt, m = gaussian_bump_case(128)
+    outs = {}
+    for k in (2, 4, 8, 12, 16):
+        out, sd, _ = run_op(ops.PoissonBlend, src, tgt, m, {"max_iterations": 22, "tolerance": 1e-12},
+                            omega=1.7, half_sweeps=k)
+        assert sd.iterations == 22
+        outs[k] = out.tobytes()
+    assert len(set(outs.values())) == 1
+
+
+def test_sweeps_match_fp64_red_black_reference(gpu):
+    """A few plain red-black sweeps on the GPU agree with a numpy FP64
+    red-black SOR of the same sweeps (FP32 correction arithmetic: close, not equal)."""
+    src, tgt, m = gaussian_bump_case(96)
+    cells = classify(m)
+    rhs = np.where(cells == 1, laplacian(src), 0.0)
+    u = tgt.astype(np.float64).copy()
+    omega, nsw = 1.6, 5
+    yy, xx = np.mgrid[0:96, 0:96]
+    for _ in range(nsw):
+        for colour in (0, 1):
+            sel = (cells == 1) & (((xx + yy) & 1) == colour)
+            nsum = np.zeros_like(u)
+            nsum[1:-1, 1:-1] = u[1:-1, :-2] + u[1:-1, 2:] + u[:-2, 1:-1] + u[2:, 1:-1]
+            target = (nsum - rhs) / 4.0
+            u[sel] = u[sel] + omega * (target[sel] - u[sel])
+    out, sd, _ = run_op(ops.PoissonBlend, src, tgt, m, {"max_iterations": nsw, "tolerance": 1e-12},
+                        omega=omega, half_sweeps=4)
+    assert sd.iterations == nsw
+    assert np.max(np.abs(out[..., 0] - u)) < 2e-6
+
+
+# ---- generators ---------------------------------------------------------------------
+
+
+def test_device_fbm_is_bit_identical_to_oracle(gpu, oracle):
+    import torch
+
+    for (w, h, freq, seed, octs) in [(64, 48, 0.125, 101, 6), (37, 29, 3.0, 12345, 4),
+                                      (50, 50, 1.0, 7, 32)]:
+        d = torch.empty((h, w), dtype=torch.float32, device="cuda")
+        gpu.fbm_device(d.data_ptr(), w, h, freq, seed, octs)
+        torch.cuda.synchronize()
+        assert d.cpu().numpy().tobytes() == oracle.fbm(w, h, freq, seed, octs).tobytes()

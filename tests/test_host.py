@@ -1,0 +1,201 @@
+"""CPU-side tests of the host layer: the C-ABI library loads and exports every
+symbol the header declares, parameter resolution matches the reference's
+codes and classes, the backend-selection boundary behaves like dispatch.rs,
+and without a GPU the ops fail loudly (no CPU fallback)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from paintop_amd import executor, hip, ops
+from paintop_amd.errors import ErrorClass, PaintopError
+from paintop_amd.values import image, mask
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "paintop_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = hip.lib()
+    names = declared_symbols()
+    assert names, "header declares no entry points"
+    assert set(names) == set(hip.EXPORTED_SYMBOLS)
+    for n in names:
+        assert hasattr(L, n), n
+    assert hip.abi_version() == 1
+
+
+def test_probe_forced_unavailable_is_policy_error():
+    hip.force_unavailable(True)
+    try:
+        with pytest.raises(hip.HipCallError) as ei:
+            hip.probe()
+        assert ei.value.status == hip.PH_UNAVAILABLE
+        assert ei.value.error_class == "policy" and ei.value.code == "E_GPU_UNAVAILABLE"
+    finally:
+        hip.force_unavailable(False)
+
+
+def test_op_without_device_fails_loudly_with_policy_error():
+    hip.force_unavailable(True)
+    try:
+        w = h = 8
+        inputs = {"source": image(w, h, np.zeros(64)), "target": image(w, h, np.zeros(64)),
+                  "mask": mask(w, h, np.ones(64))}
+        with pytest.raises(PaintopError) as ei:
+            ops.PoissonBlend().compute(inputs, {})
+        assert ei.value.error_class == ErrorClass.POLICY
+        assert ei.value.code == "E_GPU_UNAVAILABLE" and ei.value.exit_code == 4
+    finally:
+        hip.force_unavailable(False)
+
+
+@pytest.mark.parametrize("params,msg", [
+    ({"max_iterations": 0}, "1..=100000"),
+    ({"max_iterations": 100001}, "1..=100000"),
+    ({"max_iterations": -1}, "positive integer"),
+    ({"max_iterations": 3.0}, "positive integer"),
+    ({"max_iterations": "5"}, "positive integer"),
+    ({"tolerance": 2.0}, "(0, 1)"),  # poisson_blend/tests.rs:268
+    ({"tolerance": 0}, "(0, 1)"),
+    ({"tolerance": "x"}, "must be a number"),
+    ({"tolerance": True}, "must be a number"),
+])
+def test_blend_param_errors(params, msg):
+    with pytest.raises(PaintopError) as ei:
+        ops.resolve_blend_controls(params)
+    e = ei.value
+    assert e.error_class == ErrorClass.SCHEMA and e.code == "E_POISSON_BLEND_PARAM"
+    assert msg in e.message and e.exit_code == 2
+
+
+def test_blend_defaults_and_unknown_params_ignored():
+    assert ops.resolve_blend_controls({}) == (2000, 1e-6, 0.0)
+    assert ops.resolve_blend_controls({"omega": 1.2, "lambda": 3}) == (2000, 1e-6, 0.0)
+    assert ops.resolve_blend_controls({"max_iterations": 7, "tolerance": 0.5}) == (7, 0.5, 0.0)
+
+
+def test_screened_params():
+    assert ops.resolve_screened_controls({}) == (2000, 1e-6, 0.0)
+    assert ops.resolve_screened_controls({"lambda": 2}) == (2000, 1e-6, 2.0)
+    for bad in (-1.0, float("inf"), "a"):
+        with pytest.raises(PaintopError) as ei:  # screened tests :284
+            ops.resolve_screened_controls({"lambda": bad})
+        assert ei.value.code == "E_SCREENED_POISSON_PARAM"
+
+
+def test_shape_errors_before_any_device_work():
+    # mismatched extents are Semantic errors (poisson_blend/tests.rs:258)
+    inputs = {"source": image(8, 8, np.zeros(64)), "target": image(8, 8, np.zeros(64)),
+              "mask": mask(4, 4, np.zeros(16))}
+    with pytest.raises(PaintopError) as ei:
+        ops.PoissonBlend().compute(inputs, {})
+    assert ei.value.error_class == ErrorClass.SEMANTIC
+    assert ei.value.code == "E_POISSON_BLEND_SHAPE" and ei.value.exit_code == 3
+    inputs = {"source": image(8, 8, np.zeros(64)), "target": image(8, 8, np.zeros(64 * 3), "Rgb"),
+              "mask": mask(8, 8, np.zeros(64))}
+    with pytest.raises(PaintopError) as ei:
+        ops.PoissonBlend().compute(inputs, {})
+    assert "channel layout" in ei.value.message
+    with pytest.raises(PaintopError) as ei:
+        ops.ScreenedPoisson().compute({"guidance": image(8, 8, np.zeros(64))}, {})
+    assert ei.value.error_class == ErrorClass.REFERENCE
+    assert ei.value.code == "E_SCREENED_POISSON_INPUT"
+
+
+def test_manifests_declare_hip_beside_reference():
+    for op in (ops.POISSON_BLEND_OP_ID, ops.SCREENED_POISSON_OP_ID):
+        m = ops.load_manifest(op)
+        assert m["id"] == op
+        assert m["implementations"] == ["cpu.reference@1", "hip.poisson@1"]
+        assert m["determinism"] == "bounded" and m["roi"]["category"] == "full-domain"
+    names = [p["name"] for p in ops.PoissonBlend.manifest()["params"]]
+    assert names == ["max_iterations", "tolerance"]
+    assert "lambda" in [p["name"] for p in ops.ScreenedPoisson.manifest()["params"]]
+
+
+class _Stub:
+    def __init__(self, tag):
+        self.tag = tag
+
+    def compute(self, inputs, params):
+        return {"candidate": self.tag, "report": self.tag}
+
+
+def test_backend_selection_mirrors_dispatch():
+    op = ops.POISSON_BLEND_OP_ID
+    man = ops.load_manifest(op)
+    reg = executor.ImplRegistry()
+    reg.register(op, _Stub("ref"))
+    hip_b = executor.BackendId("hip", "poisson")
+    # declared but not registered: prefer falls back to the reference (recorded)
+    sel = executor.select_backend(op, man, reg, executor.BackendPolicy.prefer([hip_b]))
+    assert str(sel.impl_id) == "cpu.reference@1" and sel.fell_back
+    # require without a registered kernel: Policy error
+    with pytest.raises(PaintopError) as ei:
+        executor.select_backend(op, man, reg, executor.BackendPolicy.require(hip_b))
+    assert ei.value.code == "E_BACKEND_UNSUPPORTED" and ei.value.error_class == ErrorClass.POLICY
+    executor.register_hip_backend(reg)
+    sel = executor.select_backend(op, man, reg, executor.BackendPolicy.prefer([hip_b]))
+    assert str(sel.impl_id) == "hip.poisson@1" and not sel.fell_back
+    # the reference policy keeps the reference
+    sel = executor.select_backend(op, man, reg, executor.BackendPolicy.reference())
+    assert str(sel.impl_id) == "cpu.reference@1" and not sel.fell_back
+    # undeclared backend is never selected even when registered
+    undeclared = dict(man, implementations=["cpu.reference@1"])
+    sel = executor.select_backend(op, undeclared, reg, executor.BackendPolicy.prefer([hip_b]))
+    assert str(sel.impl_id) == "cpu.reference@1" and sel.fell_back
+    # duplicates and reference ids are rejected by register_backend
+    with pytest.raises(PaintopError):
+        executor.register_hip_backend(reg)
+    with pytest.raises(PaintopError):
+        reg.register_backend(op, executor.ImplId.parse("cpu.reference@1"), _Stub("x"))
+
+
+def test_dispatch_node_traces_selection_and_checks_outputs():
+    op = ops.POISSON_BLEND_OP_ID
+    man = ops.load_manifest(op)
+    reg = executor.ImplRegistry()
+    reg.register(op, _Stub("ref"))
+    reg.register_backend(op, executor.ImplId.parse("hip.poisson@1"), _Stub("gpu"))
+    trace = []
+    out = executor.dispatch_node("n1", op, man, reg,
+                                 executor.BackendPolicy.prefer([executor.BackendId("hip", "poisson")]),
+                                 {}, {}, trace)
+    assert out["candidate"] == "gpu"
+    ev = [t["event"] for t in trace]
+    assert ev == ["implementation_selected", "dispatch_started", "cache_lookup",
+                  "dispatch_completed"]
+    assert trace[0]["implementation"] == "hip.poisson@1"
+
+    class Missing:
+        def compute(self, i, p):
+            return {"candidate": 1}
+
+    reg2 = executor.ImplRegistry()
+    reg2.register(op, Missing())
+    with pytest.raises(PaintopError) as ei:
+        executor.dispatch_node("n2", op, man, reg2, executor.BackendPolicy.reference(), {}, {}, [])
+    assert ei.value.code == "E_OUTPUT_NOT_PRODUCED"
+
+
+def test_impl_id_grammar():
+    assert str(executor.ImplId.parse("hip.poisson@1")) == "hip.poisson@1"
+    for bad in ("Hip.poisson@1", "hip.poisson@0", "hip@1", "hip.poisson"):
+        with pytest.raises(PaintopError):
+            executor.ImplId.parse(bad)
+
+
+def test_star_polygon_area_is_about_30_percent():
+    from paintop_amd.inputs import star_polygon
+    n = 4096
+    v = np.array(star_polygon(n / 2, n / 2, np.sqrt(0.3 * n * n / 1.5), 0.5, 6, 0.1, 301))
+    x, y = v[:, 0], v[:, 1]
+    area = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+    assert 0.25 < area / (n * n) < 0.35
