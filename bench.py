@@ -49,18 +49,16 @@ def cpu_baselines(size, omega, gpu_sweeps, budget_s):
     16 threads) and the single-core restatement of cpu.reference (lexicographic
     SOR, omega 1.8). Both run whole sweeps of the full-size R channel; the
     per-sweep time is scaled to the sweep count the solve needs."""
-    import torch
-
     from oracle import oracle as orc
     from paintop_amd.inputs import seamless_clone_case
     from tests.helpers import classify, laplacian
 
     src, tgt, msk = seamless_clone_case(size, channels=4)
-    s = src[..., 0].cpu().numpy()
-    t = tgt[..., 0].cpu().numpy().astype(np.float64)
-    m = msk.cpu().numpy()
-    del src, tgt, msk
-    torch.cuda.empty_cache()
+    s = src.numpy()[..., 0].copy()
+    t = tgt.numpy()[..., 0].astype(np.float64)
+    m = msk.numpy()
+    for a in (src, tgt, msk):
+        a.free()
     cells = classify(m)
     rhs = np.where(cells == 1, laplacian(s), 0.0)
     threads = min(os.cpu_count() or 1, 16)
@@ -109,32 +107,31 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not implemented)"}))
         sys.exit(2)
-    import torch
-
     from paintop_amd import hip
+    from paintop_amd.device import DeviceArray, synchronize
     from paintop_amd.inputs import seamless_clone_case
 
     info = hip.probe()
     size = args.size
     src, tgt, msk = seamless_clone_case(size, channels=4)
-    out = torch.empty_like(tgt)
-    stream = torch.cuda.current_stream().cuda_stream
+    out = DeviceArray(tgt.shape, np.float32)
+    stream = 0
     desc = hip.make_desc(size, size, 4, 100000, args.tol, 0.0, hip.PH_OP_POISSON_BLEND,
                          omega=args.omega, half_sweeps=args.half_sweeps, flags=hip.PH_FLAG_TIMING)
 
     def step():
-        return hip.poisson_solve_device(desc, src.data_ptr(), tgt.data_ptr(), msk.data_ptr(),
-                                        out.data_ptr(), stream, want_history=False)
+        return hip.poisson_solve_device(desc, src.ptr, tgt.ptr, msk.ptr, out.ptr, stream,
+                                        want_history=False)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    synchronize()
     times, results = [], []
     for _ in range(args.steps):
-        torch.cuda.synchronize()
+        synchronize()
         t0 = time.perf_counter()
         r = step()
-        torch.cuda.synchronize()
+        synchronize()
         times.append((time.perf_counter() - t0) * 1e3)
         results.append(r)
     ms = float(np.median(times))
@@ -186,15 +183,17 @@ def main():
     }
     if not args.no_tight:
         # error of the 1e-6 result against the same solve driven to 1e-12 in FP64 refinement
-        out_tight = torch.empty_like(tgt)
+        out_tight = DeviceArray(tgt.shape, np.float32)
         d2 = hip.make_desc(size, size, 4, 100000, 1e-12, 0.0, hip.PH_OP_POISSON_BLEND,
                            omega=args.omega, half_sweeps=args.half_sweeps)
-        rt = hip.poisson_solve_device(d2, src.data_ptr(), tgt.data_ptr(), msk.data_ptr(),
-                                      out_tight.data_ptr(), stream, want_history=False)
-        err = (out - out_tight).abs().max().item()
+        rt = hip.poisson_solve_device(d2, src.ptr, tgt.ptr, msk.ptr, out_tight.ptr, stream,
+                                      want_history=False)
+        err = float(np.abs(out.numpy() - out_tight.numpy()).max())
         line["config"]["max_abs_err_vs_fp64_1e-12"] = err
         line["config"]["tight_sweeps"] = rt.iterations
-        del out_tight
+        out_tight.free()
+    for a in (src, tgt, msk, out):
+        a.free()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baselines(size, r.omega, r.iterations, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -176,6 +176,10 @@ ph_status ph_polygon_mask_device(float *d_out, uint32_t width, uint32_t height,
                                  const double *points_xy, uint32_t n_points, int32_t even_odd,
                                  void *stream, ph_error *err);
 
+/* d_out[i] += sigma * N(0, 1) for i < n (seeded SplitMix64 + Box-Muller). */
+ph_status ph_gauss_noise_device(float *d_out, uint64_t n, uint64_t seed, double sigma,
+                                void *stream, ph_error *err);
+
 /* Binary axis-aligned ellipse mask: 1 where ((x+.5-cx)/rx)^2 + ((y+.5-cy)/ry)^2 <= 1. */
 ph_status ph_ellipse_mask_device(float *d_out, uint32_t width, uint32_t height, double cx,
                                  double cy, double rx, double ry, void *stream, ph_error *err);

@@ -110,6 +110,18 @@ __global__ void k_ellipse(float *out, int w, int h, double cx, double cy, double
   out[(size_t)y * w + x] = (dx * dx + dy * dy <= 1.0) ? 1.0f : 0.0f;
 }
 
+// out[i] += sigma * N(0, 1): Box-Muller on two SplitMix64 draws keyed by (seed, i).
+__global__ void k_gauss_noise(float *out, size_t n, uint64_t seed, double sigma) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = mix64(seed + 0x9e3779b97f4a7c15ULL * (2 * (uint64_t)i + 1));
+  const uint64_t b = mix64(seed + 0x9e3779b97f4a7c15ULL * (2 * (uint64_t)i + 2));
+  const double u1 = ((double)(a >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+  const double z = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+  out[i] = out[i] + (float)(sigma * z);
+}
+
 }  // namespace ph
 
 using namespace ph;
@@ -143,6 +155,18 @@ extern "C" ph_status ph_fbm_device(float *d_out, uint32_t width, uint32_t height
   hipLaunchKernelGGL(k_fbm, dim3((width + 255) / 256, height), dim3(256), 0, (hipStream_t)stream,
                      d_out, (int)width, (int)height, (int)channel_stride, (int)channel_offset,
                      frequency / 64.0, seed, (int)octaves, lacunarity, gain, scale, bias);
+  return gen_finish(hipGetLastError(), err);
+}
+
+extern "C" ph_status ph_gauss_noise_device(float *d_out, uint64_t n, uint64_t seed, double sigma,
+                                           void *stream, ph_error *err) {
+  if (!d_out || !std::isfinite(sigma)) {
+    gen_err(err, PH_CLASS_SCHEMA, "E_NOISE_PARAM", "gauss noise: invalid parameters");
+    return PH_ERROR;
+  }
+  if (n == 0) return PH_OK;
+  hipLaunchKernelGGL(k_gauss_noise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, d_out, (size_t)n, seed, sigma);
   return gen_finish(hipGetLastError(), err);
 }
 

@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "ph_fbm_device",
     "ph_polygon_mask_device",
     "ph_ellipse_mask_device",
+    "ph_gauss_noise_device",
 )
 
 
@@ -145,6 +146,9 @@ def lib():
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, vp,
                                          ctypes.POINTER(PhError)]
     L.ph_ellipse_mask_device.restype = ctypes.c_int
+    L.ph_gauss_noise_device.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, vp,
+                                        ctypes.POINTER(PhError)]
+    L.ph_gauss_noise_device.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -287,4 +291,11 @@ def ellipse_mask_device(d_out: int, width, height, cx, cy, rx, ry, stream: int =
     err = PhError()
     st = lib().ph_ellipse_mask_device(d_out, width, height, cx, cy, rx, ry, stream or None,
                                       ctypes.byref(err))
+    _check(st, err)
+
+
+def gauss_noise_device(d_out: int, n: int, seed: int, sigma: float, stream: int = 0):
+    err = PhError()
+    st = lib().ph_gauss_noise_device(d_out, n, seed & 0xFFFFFFFFFFFFFFFF, sigma, stream or None,
+                                     ctypes.byref(err))
     _check(st, err)

@@ -108,53 +108,47 @@ def random_polygon(cx, cy, rx, ry, n, rng: SplitMix64):
 # --- device-side large cases -----------------------------------------------------
 
 
-def seamless_clone_case(size: int = 4096, device: str = "cuda", channels: int = 4,
-                        star_seed: int = 301):
-    """4096x4096 RGBA fbm blend generated in HBM; returns torch tensors
-    (source, target, mask) with source/target (H, W, C) and mask (H, W)."""
-    import torch
-
+def seamless_clone_case(size: int = 4096, channels: int = 4, star_seed: int = 301):
+    """4096x4096 RGBA fbm blend generated in HBM. Returns DeviceArrays
+    (source, target, mask): source/target (H, W, C), mask (H, W)."""
     from . import hip
+    from .device import DeviceArray, synchronize
 
-    src = torch.empty((size, size, channels), dtype=torch.float32, device=device)
-    tgt = torch.empty_like(src)
-    msk = torch.empty((size, size), dtype=torch.float32, device=device)
-    stream = torch.cuda.current_stream().cuda_stream
-    for c in range(min(channels, 3)):
-        hip.fbm_device(src.data_ptr(), size, size, 0.125, 101 + c, 6, 2.0, 0.5,
-                       channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5,
-                       stream=stream)
-        hip.fbm_device(tgt.data_ptr(), size, size, 0.125, 201 + c, 6, 2.0, 0.5,
-                       channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5,
-                       stream=stream)
-    if channels == 4:
-        src[:, :, 3] = 1.0
-        tgt[:, :, 3] = 1.0
+    src = DeviceArray((size, size, channels), np.float32)
+    tgt = DeviceArray((size, size, channels), np.float32)
+    msk = DeviceArray((size, size), np.float32)
+    for c in range(channels):
+        if c < 3:
+            hip.fbm_device(src.ptr, size, size, 0.125, 101 + c, 6, 2.0, 0.5,
+                           channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5)
+            hip.fbm_device(tgt.ptr, size, size, 0.125, 201 + c, 6, 2.0, 0.5,
+                           channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5)
+        else:  # alpha = 1 (fmaf(v, 0, 1))
+            for arr in (src, tgt):
+                hip.fbm_device(arr.ptr, size, size, 1.0, 0, 1, 2.0, 0.5,
+                               channel_stride=channels, channel_offset=c, scale=0.0, bias=1.0)
     r_outer = math.sqrt(0.30 * size * size / 1.5)
     verts = star_polygon(size / 2.0, size / 2.0, r_outer, 0.5, 6, 0.1, star_seed)
-    hip.polygon_mask_device(msk.data_ptr(), size, size, verts, stream=stream)
-    torch.cuda.synchronize()
+    hip.polygon_mask_device(msk.ptr, size, size, verts)
+    synchronize()
     return src, tgt, msk
 
 
-def screened_case(size: int = 8192, border: int = 64, device: str = "cuda"):
+def screened_case(size: int = 8192, border: int = 64):
     """8192x8192 Gray screened case: anchor = (fbm+1)/2 (seed 401), guidance =
-    (fbm2+1)/2 + N(0, 0.01) (seeds 402, 403), mask = 1 except a 64 px border."""
-    import torch
-
+    (fbm2+1)/2 + N(0, 0.01) (seeds 402, 403), mask = 1 except a 64 px border.
+    Returns DeviceArrays (guidance, anchor, mask)."""
     from . import hip
+    from .device import DeviceArray, synchronize
 
-    stream = torch.cuda.current_stream().cuda_stream
-    anchor = torch.empty((size, size), dtype=torch.float32, device=device)
-    guide = torch.empty_like(anchor)
-    hip.fbm_device(anchor.data_ptr(), size, size, 0.125, 401, 6, 2.0, 0.5, scale=0.5, bias=0.5,
-                   stream=stream)
-    hip.fbm_device(guide.data_ptr(), size, size, 0.125, 402, 6, 2.0, 0.5, scale=0.5, bias=0.5,
-                   stream=stream)
-    g = torch.Generator(device=device)
-    g.manual_seed(403)
-    guide += 0.01 * torch.randn(guide.shape, generator=g, device=device, dtype=torch.float32)
-    msk = torch.zeros_like(anchor)
-    msk[border:size - border, border:size - border] = 1.0
-    torch.cuda.synchronize()
+    anchor = DeviceArray((size, size), np.float32)
+    guide = DeviceArray((size, size), np.float32)
+    msk = DeviceArray((size, size), np.float32)
+    hip.fbm_device(anchor.ptr, size, size, 0.125, 401, 6, 2.0, 0.5, scale=0.5, bias=0.5)
+    hip.fbm_device(guide.ptr, size, size, 0.125, 402, 6, 2.0, 0.5, scale=0.5, bias=0.5)
+    hip.gauss_noise_device(guide.ptr, size * size, 403, 0.01)
+    b = float(border)
+    hip.polygon_mask_device(msk.ptr, size, size,
+                            [(b, b), (size - b, b), (size - b, size - b), (b, size - b)])
+    synchronize()
     return guide, anchor, msk

@@ -8,6 +8,7 @@ stated; stop reasons, iteration caps and history lengths as the reference's
 tests pin them.
 """
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -194,28 +195,48 @@ def check_parity(oracle_out, gpu_out, m, target, label):
 
 
 def test_gaussian_bump_case_parity(gpu, oracle):
-    """256x256 Gray, omega 1.9, 1e-5 -- the CPU-reference configuration."""
+    """256x256 Gray, omega 1.9 -- the CPU-reference configuration.
+
+    At the configuration's 1e-5 the two solvers stop at the same relative
+    residual along different orderings (lexicographic vs red-black), and the
+    solution error a 1e-5 residual allows is larger than 1e-4: both fields are
+    stated against the exact sparse direct solve, and the GPU's must be no
+    worse than the oracle's. At 1e-9 the fixed point is pinned and the
+    bounded-tier envelope (max-abs 1e-4, RMS 1e-5) must hold."""
     src, tgt, m = gaussian_bump_case(256)
-    ref, ref_rep = oracle.blend_channels(src, tgt, m, 0.0, 5000, 1e-5, omega=1.9)
-    out, sd, res = run_op(ops.PoissonBlend, src, tgt, m,
-                          {"tolerance": 1e-5, "max_iterations": 5000}, omega=1.9)
-    print(f"[parity] bump: oracle {ref_rep.iterations} sweeps, gpu {sd.iterations} sweeps "
-          f"(refinements {res.refinements}, K {res.half_sweeps})", flush=True)
-    assert sd.converged and ref_rep.converged and sd.final_residual <= 1e-5
-    assert len(sd.residual_history) == sd.iterations
-    assert oracle.classify_cells(m).tobytes() == classify(m).tobytes()
-    check_parity(ref, out, m, tgt, "bump")
-    # error of both against the exact solution of the same system
     cells = classify(m)
+    assert oracle.classify_cells(m).tobytes() == cells.tobytes()
     rhs = np.where(cells == 1, laplacian(src), 0.0)
     exact = direct_solve(cells, rhs, tgt.astype(np.float64), 0.0)
-    eg = np.abs(out[..., 0] - exact)[cells == 1].max()
-    eo = np.abs(ref[..., 0] - exact)[cells == 1].max()
-    print(f"[parity] bump: max error vs direct FP64 solve: gpu {eg:.3e}, oracle {eo:.3e}",
-          flush=True)
-    # the GPU's own FP64 stopping decision holds for the returned field (f32-rounded)
-    r0 = residual_norm(cells, rhs, tgt, 0.0, tgt)
-    assert residual_norm(cells, rhs, tgt, 0.0, out[..., 0]) / r0 < 1e-4
+    inside = cells == 1
+    for tol in (1e-5, 1e-9):
+        ref, ref_rep = oracle.blend_channels(src, tgt, m, 0.0, 100000, tol, omega=1.9)
+        out, sd, res = run_op(ops.PoissonBlend, src, tgt, m,
+                              {"tolerance": tol, "max_iterations": 100000}, omega=1.9)
+        eg = float(np.abs(out[..., 0] - exact)[inside].max())
+        eo = float(np.abs(ref[..., 0] - exact)[inside].max())
+        d = np.abs(out[..., 0].astype(np.float64) - ref[..., 0])[inside]
+        print(f"[parity] bump tol {tol:g}: oracle {ref_rep.iterations} sweeps, gpu "
+              f"{sd.iterations} sweeps (refinements {res.refinements}, K {res.half_sweeps}); "
+              f"|gpu-oracle| max {d.max():.3e} rms {np.sqrt(np.mean(d * d)):.3e}; max error vs "
+              f"direct FP64 solve: gpu {eg:.3e}, oracle {eo:.3e}", flush=True)
+        assert sd.converged and ref_rep.converged and sd.final_residual <= tol
+        assert len(sd.residual_history) == sd.iterations
+        assert out[..., 0][~inside].tobytes() == tgt[~inside].tobytes()
+        # the residual of the f32-rounded candidate: at loose tolerances it is the
+        # solver's; at tight ones the f32 rounding floor, which the reference's
+        # f32 output has too
+        r0 = residual_norm(cells, rhs, tgt, 0.0, tgt)
+        rg = residual_norm(cells, rhs, tgt, 0.0, out[..., 0]) / r0
+        ro = residual_norm(cells, rhs, tgt, 0.0, ref[..., 0]) / r0
+        print(f"[parity] bump tol {tol:g}: f32 candidate residual gpu {rg:.3e} oracle {ro:.3e}",
+              flush=True)
+        assert rg <= max(2.0 * tol, 2.0 * ro)
+        if tol == 1e-5:
+            assert eg <= 1.25 * eo + 1e-6
+        else:
+            check_parity(ref, out, m, tgt, f"bump tol {tol:g}")
+            assert eg < 1e-5
 
 
 @pytest.mark.parametrize("lam", [0.0, 0.01, 1.0])
@@ -249,9 +270,9 @@ def test_star_mask_rgba_parity(gpu, oracle, lam):
         winding += np.where(hit, np.where(b0, 1, -1), 0)
     inside = winding != 0
     m[inside] = 1.0
-    ref, rrep = oracle.blend_channels(src, tgt, m, lam, 100000, 1e-6, omega=1.9)
+    ref, rrep = oracle.blend_channels(src, tgt, m, lam, 100000, 1e-9, omega=1.9)
     op = ops.ScreenedPoisson if lam > 0 else ops.PoissonBlend
-    params = {"tolerance": 1e-6, "max_iterations": 100000}
+    params = {"tolerance": 1e-9, "max_iterations": 100000}
     if lam > 0:
         params["lambda"] = lam
     out, sd, res = run_op(op, src, tgt, m, params, layout="Rgba")
@@ -302,11 +323,56 @@ def test_sweeps_match_fp64_red_black_reference(gpu):
 
 
 def test_device_fbm_is_bit_identical_to_oracle(gpu, oracle):
-    import torch
+    from paintop_amd.device import DeviceArray, synchronize
 
     for (w, h, freq, seed, octs) in [(64, 48, 0.125, 101, 6), (37, 29, 3.0, 12345, 4),
                                       (50, 50, 1.0, 7, 32)]:
-        d = torch.empty((h, w), dtype=torch.float32, device="cuda")
-        gpu.fbm_device(d.data_ptr(), w, h, freq, seed, octs)
-        torch.cuda.synchronize()
-        assert d.cpu().numpy().tobytes() == oracle.fbm(w, h, freq, seed, octs).tobytes()
+        d = DeviceArray((h, w), np.float32)
+        gpu.fbm_device(d.ptr, w, h, freq, seed, octs)
+        synchronize()
+        assert d.numpy().tobytes() == oracle.fbm(w, h, freq, seed, octs).tobytes()
+
+
+def test_device_polygon_and_ellipse_masks(gpu):
+    from paintop_amd.device import DeviceArray, synchronize
+
+    w, h = 70, 50
+    verts = star_polygon(35, 25, 22, 0.5, 6, 0.1, 5)
+    d = DeviceArray((h, w), np.float32)
+    gpu.polygon_mask_device(d.ptr, w, h, verts)
+    synchronize()
+    got = d.numpy()
+    # the same half-open ray cast (mask_polygon.rs:161-195) on the host
+    want = np.zeros((h, w), np.float32)
+    n = len(verts)
+    for y in range(h):
+        for x in range(w):
+            px, py, wind = x + 0.5, y + 0.5, 0
+            for i in range(n):
+                (x0, y0), (x1, y1) = verts[i], verts[(i + 1) % n]
+                if (y0 <= py) == (y1 <= py):
+                    continue
+                t = (py - y0) / (y1 - y0)
+                cross = float(Fraction(t) * Fraction(x1 - x0) + Fraction(x0))  # exact fma
+                if cross > px:
+                    wind += 1 if y0 <= py else -1
+            want[y, x] = 1.0 if wind != 0 else 0.0
+    assert np.array_equal(got, want)
+    gpu.ellipse_mask_device(d.ptr, w, h, 35.0, 25.0, 20.0, 12.0)
+    synchronize()
+    from paintop_amd.inputs import ellipse_mask
+    assert np.array_equal(d.numpy(), ellipse_mask(w, h, 35.0, 25.0, 20.0, 12.0))
+
+
+def test_device_resident_solve_matches_host_entry(gpu):
+    """ph_poisson_solve_device on device buffers gives the host entry's bits."""
+    from paintop_amd.device import DeviceArray
+
+    src, tgt, m = gaussian_bump_case(128)
+    desc = gpu.make_desc(128, 128, 1, 5000, 1e-6)
+    out_h, r_h = gpu.poisson_solve(desc, src, tgt, m)
+    ds, dt, dm = (DeviceArray.from_numpy(a) for a in (src, tgt, m))
+    dout = DeviceArray((128, 128), np.float32)
+    r_d = gpu.poisson_solve_device(desc, ds.ptr, dt.ptr, dm.ptr, dout.ptr)
+    assert dout.numpy().tobytes() == out_h.reshape(128, 128).tobytes()
+    assert r_d.iterations == r_h.iterations and r_d.residual_history == r_h.residual_history
