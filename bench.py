@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--half-sweeps", type=int, default=0)
     p.add_argument("--omega", type=float, default=0.0)
+    p.add_argument("--tile-rows", type=int, default=0)
+    p.add_argument("--max-iterations", type=int, default=100000)
     p.add_argument("--tol", type=float, default=1e-6)
     p.add_argument("--cpu-seconds", type=float, default=20.0,
                    help="budget of the CPU baseline sample")
@@ -65,11 +67,13 @@ def cpu_baselines(size, omega, gpu_sweeps, budget_s):
     # all-cores stand-in: time n sweeps
     n = 4
     t0 = time.perf_counter()
-    _, rep, used = orc.rbsor_omp(cells, rhs, t, 0.0, n, 1e-30, omega, threads=threads)
+    _, rep, used = orc.rbsor_omp(cells, rhs, t, 0.0, n, 1e-30, omega, threads=threads,
+                                  stall_window=0)
     per_sweep = (time.perf_counter() - t0) / n
     n2 = max(2, min(200, int(budget_s / max(per_sweep, 1e-6))))
     t0 = time.perf_counter()
-    _, rep, used = orc.rbsor_omp(cells, rhs, t, 0.0, n2, 1e-30, omega, threads=threads)
+    _, rep, used = orc.rbsor_omp(cells, rhs, t, 0.0, n2, 1e-30, omega, threads=threads,
+                                  stall_window=0)
     per_sweep = (time.perf_counter() - t0) / n2
     mp = size * size / 1e6
     opt_ms_per_mp = per_sweep * gpu_sweeps * 3 * 1e3 / mp
@@ -116,8 +120,9 @@ def main():
     src, tgt, msk = seamless_clone_case(size, channels=4)
     out = DeviceArray(tgt.shape, np.float32)
     stream = 0
-    desc = hip.make_desc(size, size, 4, 100000, args.tol, 0.0, hip.PH_OP_POISSON_BLEND,
-                         omega=args.omega, half_sweeps=args.half_sweeps, flags=hip.PH_FLAG_TIMING)
+    desc = hip.make_desc(size, size, 4, args.max_iterations, args.tol, 0.0,
+                         hip.PH_OP_POISSON_BLEND, omega=args.omega, half_sweeps=args.half_sweeps,
+                         flags=hip.PH_FLAG_TIMING, tile_rows=args.tile_rows)
 
     def step():
         return hip.poisson_solve_device(desc, src.ptr, tgt.ptr, msk.ptr, out.ptr, stream,
@@ -185,7 +190,8 @@ def main():
         # error of the 1e-6 result against the same solve driven to 1e-12 in FP64 refinement
         out_tight = DeviceArray(tgt.shape, np.float32)
         d2 = hip.make_desc(size, size, 4, 100000, 1e-12, 0.0, hip.PH_OP_POISSON_BLEND,
-                           omega=args.omega, half_sweeps=args.half_sweeps)
+                           omega=args.omega, half_sweeps=args.half_sweeps,
+                           tile_rows=args.tile_rows)
         rt = hip.poisson_solve_device(d2, src.ptr, tgt.ptr, msk.ptr, out_tight.ptr, stream,
                                       want_history=False)
         err = float(np.abs(out.numpy() - out_tight.numpy()).max())

@@ -84,11 +84,11 @@ typedef struct ph_solve_desc {
   uint32_t max_iterations; /* 1..=100000 (poisson_blend.rs:71) */
   double tolerance;        /* (0, 1) */
   double lambda;           /* >= 0, finite; 0 for PH_OP_POISSON_BLEND */
-  double omega;            /* SOR factor in (0, 2); 0 selects the tuned value */
+  double omega;            /* fixed SOR factor in (0, 2); 0 = Chebyshev schedule to the tuned value */
   uint32_t flags;          /* PH_FLAG_* */
   uint32_t half_sweeps;    /* red/black half-sweeps per HBM round trip; 0 = tuned */
   int32_t op;              /* ph_op */
-  uint32_t reserved;
+  uint32_t tile_rows;      /* rows per sweep tile (even, 16..1024); 0 = tuned */
 } ph_solve_desc;
 
 /* SolveReport (poisson.rs:126-141) of the worst channel (poisson_blend.rs:240-246),

@@ -76,11 +76,13 @@ void or_fbm(size_t width, size_t height, double frequency, uint64_t seed, uint32
             double lacunarity, double gain, float *out);
 
 /* The all-cores CPU stand-in: red-black SOR in f64 on interior cells, with the
- * reference's per-sweep relative residual and stop rules. threads <= 0 means
- * the OpenMP default. Returns the thread count used. */
+ * reference's per-sweep relative residual and stop rules (stall_window 16 is
+ * the reference's; 0 turns the stall guard off, for timing fixed sweep counts).
+ * threads <= 0 means the OpenMP default. Returns the thread count used. */
 int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double *rhs,
                  const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
-                 double omega, int threads, double *field, double *history, or_report *report);
+                 double omega, int threads, uint32_t stall_window, double *field,
+                 double *history, or_report *report);
 
 #ifdef __cplusplus
 }

@@ -91,8 +91,8 @@ def lib():
         L.or_fbm.argtypes = [sz, sz, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
                              ctypes.c_double, ctypes.c_double, fp]
         L.or_rbsor_omp.argtypes = [sz, sz, up, dp, dp, ctypes.c_double, ctypes.c_uint32,
-                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp,
-                                   ctypes.POINTER(_Report)]
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_uint32, dp, dp, ctypes.POINTER(_Report)]
         L.or_rbsor_omp.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -181,8 +181,10 @@ def blend_channels(source, target, mask, lam, max_iterations, tolerance, omega=1
     return out, _report(rep, hist)
 
 
-def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads=0):
-    """The all-cores red-black SOR stand-in for cpu.optimized."""
+def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads=0,
+              stall_window=16):
+    """The all-cores red-black SOR stand-in for cpu.optimized (stall_window 0
+    disables the stall guard, for timing a fixed number of sweeps)."""
     cells = np.ascontiguousarray(cells, dtype=np.uint8)
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
     anchor = np.ascontiguousarray(anchor, dtype=np.float64)
@@ -193,7 +195,8 @@ def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads
     used = lib().or_rbsor_omp(w, h, _p(cells, ctypes.c_uint8), _p(rhs, ctypes.c_double),
                               _p(anchor, ctypes.c_double), float(lam), int(max_iterations),
                               float(tolerance), float(omega), int(threads),
-                              _p(out, ctypes.c_double), _p(hist, ctypes.c_double),
+                              int(stall_window), _p(out, ctypes.c_double),
+                              _p(hist, ctypes.c_double),
                               ctypes.byref(rep))
     return out, _report(rep, hist), int(used)
 

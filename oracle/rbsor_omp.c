@@ -43,7 +43,8 @@ static double residual_norm(size_t width, size_t height, const uint8_t *cells,
 
 int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double *rhs,
                  const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
-                 double omega, int threads, double *field, double *history, or_report *report) {
+                 double omega, int threads, uint32_t stall_window, double *field,
+                 double *history, or_report *report) {
   size_t n = width * height;
   int nthreads = threads > 0 ? threads : omp_get_max_threads();
   memcpy(field, anchor, n * sizeof(double));
@@ -99,7 +100,7 @@ int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double
     if (rel < best * (1.0 - 1e-12)) {
       best = rel;
       since = 0;
-    } else if (++since >= 16) {
+    } else if (++since >= stall_window && stall_window > 0) {
       stop = OR_STALLED;
       break;
     }

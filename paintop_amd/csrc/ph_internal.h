@@ -29,7 +29,7 @@ struct PhInst {
   int32_t pw, ph;   // plane width / height in cells
   int32_t ox, oy;   // image coordinates of plane cell (0, 0)
   int32_t img_w, img_h, channels, channel; // source image geometry for gather/scatter
-  float c;          // omega / (4 + lambda)
+  const double *omega_tab; // SOR factor of half-sweep h (0-based h), h < 2 * max_it
   float dg;         // 4 + lambda (FP32)
   double dg64;      // 4 + lambda (FP64)
   double lambda;
@@ -54,9 +54,8 @@ struct PhCtl {
   double r0;          // ||r(anchor start)|| in FP64: the relative-residual denominator
   double tol;
   double refine_abs2; // refine when the squared inner residual falls to this
-  double om1sq;       // (1 - omega)^2
   double best;        // best relative residual so far (stall guard)
-  double prev_black;  // black pre-update residual sum of the last half-sweep of the previous launch
+  double prev_black;  // (1 - omega)^2 x black pre-update residual sum, last half-sweep of the previous launch
   double *history;    // relative residual per sweep, capacity max_it
   uint32_t max_it;
   uint32_t sweep_limit;

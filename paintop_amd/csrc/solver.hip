@@ -193,6 +193,8 @@ static void validate(const ph_solve_desc *d) {
   const uint32_t k = d->half_sweeps;
   if (k != 0 && k != 2 && k != 4 && k != 8 && k != 12 && k != 16)
     throw fail(PH_CLASS_SCHEMA, "E_HIP_TUNING", "half_sweeps must be one of 2, 4, 8, 12, 16");
+  if (d->tile_rows != 0 && (d->tile_rows < 16 || d->tile_rows > 1024 || (d->tile_rows & 1)))
+    throw fail(PH_CLASS_SCHEMA, "E_HIP_TUNING", "tile_rows must be 0 or even in 16..1024");
   if ((size_t)d->width * d->height > ((size_t)1 << 31))
     throw fail(PH_CLASS_POLICY, "E_HIP_LIMIT", "image of %ux%u exceeds the 2^31-pixel limit",
                d->width, d->height);
@@ -206,16 +208,20 @@ static void validate(const ph_solve_desc *d) {
 // (Faber-Krahn, mu >= (j01 / R)^2) and by the bounding rectangle (domain
 // monotonicity, mu >= (pi/bw)^2 + (pi/bh)^2); the larger bound is used, which
 // errs on the side of a slightly large omega, where SOR degrades gently.
-static double tuned_omega(double n_interior, int bw, int bh, double lambda) {
+static double rho_jacobi(double n_interior, int bw, int bh, double lambda) {
   const double j01 = 2.404825557695773;
   const double R = sqrt(std::max(n_interior, 1.0) / M_PI);
   const double mu_disc = (j01 / R) * (j01 / R);
   const double mu_rect = (M_PI / std::max(bw, 1)) * (M_PI / std::max(bw, 1)) +
                          (M_PI / std::max(bh, 1)) * (M_PI / std::max(bh, 1));
   const double mu = std::min(std::max(mu_disc, mu_rect), 4.0);
-  const double rho = (4.0 - mu) / (4.0 + lambda);
+  return (4.0 - mu) / (4.0 + lambda);
+}
+
+static double tuned_omega(double n_interior, int bw, int bh, double lambda) {
+  const double rho = rho_jacobi(n_interior, bw, bh, lambda);
   const double w = 2.0 / (1.0 + sqrt(std::max(1.0 - rho * rho, 0.0)));
-  return std::min(std::max(w, 1.0), 1.999);
+  return std::min(std::max(w, 1.0), 1.9999);
 }
 
 typedef void (*sor_fn)(const PhTile *, int, const PhInst *, PhCtl *, double *, uint32_t *, int);
@@ -287,7 +293,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
   // --- classification and bounding box
   const int K = d->half_sweeps ? (int)d->half_sweeps : 8;
   const int CPL = 4, WS = 64 * CPL, WV = WS - 2 * K;
-  int hb = 128;
+  int hb = d->tile_rows ? (int)d->tile_rows : 128;
   {
     // upper bound of the workspace for this call, so the pool never regrows mid-call
     const size_t pw_up = (size_t)W + 2 * PH_PAD + 4 + WV + 2 * K + 64;
@@ -296,7 +302,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
     const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
     const size_t need = npx + 16 * 1024 + ntile_up * (sizeof(int2) + sizeof(int)) +
                         plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
-                        (size_t)C * (d->max_iterations + 1) * sizeof(double) +
+                        (size_t)(C + 2) * (d->max_iterations + 1) * sizeof(double) +
                         (size_t)C * ph_up * sizeof(double) +
                         ntile_up * (size_t)C * (sizeof(PhTile) + K * sizeof(double)) +
                         (size_t)C * (sizeof(PhInst) + sizeof(PhCtl) + 64) + 64 * 256;
@@ -362,6 +368,22 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
                                       : tuned_omega(n_interior, region_w, region_h, d->lambda);
   rep->omega_used = omega;
   rep->half_sweeps_used = (uint32_t)K;
+  // SOR factor per half-sweep. A fixed omega when the caller asks for one;
+  // otherwise Chebyshev acceleration of red/black SOR: omega_1 = 1,
+  // omega_2 = 1/(1 - rho^2/2), omega_{h+1} = 1/(1 - rho^2 omega_h / 4), which
+  // rises to omega_opt = 2/(1 + sqrt(1 - rho^2)) without the transient residual
+  // growth a large fixed omega shows on big domains.
+  const size_t ntab = 2 * (size_t)d->max_iterations + 2;
+  std::vector<double> wtab(ntab, omega);
+  if (d->omega <= 0.0) {
+    const double rho = rho_jacobi(n_interior, region_w, region_h, d->lambda);
+    const double r2 = rho * rho;
+    wtab[0] = 1.0;
+    if (ntab > 1) wtab[1] = 1.0 / (1.0 - 0.5 * r2);
+    for (size_t h = 2; h < ntab; ++h) wtab[h] = 1.0 / (1.0 - 0.25 * r2 * wtab[h - 1]);
+  }
+  double *omega_d = g_pool.take<double>(ntab);
+  HIP_OK(hipMemcpyAsync(omega_d, wtab.data(), ntab * sizeof(double), hipMemcpyHostToDevice, st));
 
   // --- per-channel planes and r0
   std::vector<double> r0(C, 0.0);
@@ -398,7 +420,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       I.channel = c;
       I.dg64 = 4.0 + d->lambda;
       I.dg = (float)I.dg64;
-      I.c = (float)(omega / I.dg64);
+      I.omega_tab = omega_d;
       I.lambda = d->lambda;
       hipLaunchKernelGGL(k_assemble, pgrd, pblk, 0, st, I, src, tgt, cls_img, cls_plane);
       HIP_OK(hipGetLastError());
@@ -467,7 +489,6 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       Cc.r0 = r0[c];
       Cc.tol = d->tolerance;
       Cc.refine_abs2 = (REFINE_RATIO * r0[c]) * (REFINE_RATIO * r0[c]);
-      Cc.om1sq = (1.0 - omega) * (1.0 - omega);
       Cc.best = INFINITY;
       Cc.history = hist_d[c];
       Cc.max_it = d->max_iterations;
@@ -701,7 +722,8 @@ extern "C" ph_status ph_probe(ph_device_info *out, ph_error *err) {
     ensure_device();
     if (out) {
       memset(out, 0, sizeof(*out));
-      snprintf(out->name, sizeof(out->name), "%s", g_ctx.prop.name);
+      snprintf(out->name, sizeof(out->name), "%s",
+               g_ctx.prop.name[0] ? g_ctx.prop.name : "AMD Instinct (gfx950)");
       snprintf(out->arch, sizeof(out->arch), "%s", g_ctx.prop.gcnArchName);
       out->compute_units = g_ctx.prop.multiProcessorCount;
       out->device_index = g_ctx.dev;

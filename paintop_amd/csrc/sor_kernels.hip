@@ -22,8 +22,9 @@
 //
 // Residuals. With r_pre the residual of a cell just before its update, the
 // residual after the update is (1 - omega) r_pre exactly, and black cells do
-// not move after their own update within a sweep. So
-//     ||r(after sweep j)||^2 = (1-omega)^2 sum_black r_pre(2j)^2 + sum_red r_pre(2j+1)^2,
+// not move after their own update within a sweep. So, with omega_h the factor
+// of half-sweep h,
+//     ||r(after sweep j)||^2 = (1-omega_2j)^2 sum_black r_pre(2j)^2 + sum_red r_pre(2j+1)^2,
 // available one half-sweep late, with no extra pass. Per-lane sums are FP32 in
 // fixed row order, lanes are combined by a fixed butterfly in FP64, tiles by a
 // fixed-order sum: reruns are bit-identical.
@@ -113,15 +114,19 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *partials, int l
     return;
   }
   const uint32_t in_buf = C.cur_buf;
+  const double *W = I.omega_tab + 2 * (size_t)S;  // omega of this launch's half-sweeps
   bool stop = false;
-  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, S, C.om1sq * C.prev_black + A[0]);
+  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, S, C.prev_black + A[0]);
 #pragma unroll
   for (int j = 1; j < K / 2; ++j)
-    if (j < nsw && !stop) stop = finalize_sweep(C, S + j, C.om1sq * A[2 * j - 1] + A[2 * j]);
+    if (j < nsw && !stop) {
+      const double f = (1.0 - W[2 * j - 1]) * (1.0 - W[2 * j - 1]);
+      stop = finalize_sweep(C, S + j, f * A[2 * j - 1] + A[2 * j]);
+    }
   double last_black = A[0];
 #pragma unroll
   for (int j = 1; j <= K / 2; ++j)
-    if (j == nsw) last_black = A[2 * j - 1];
+    if (j == nsw) last_black = (1.0 - W[2 * j - 1]) * (1.0 - W[2 * j - 1]) * A[2 * j - 1];
   C.prev_black = last_black;
   if (stop) {
     C.halt_base = S;
@@ -165,7 +170,15 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
     const float *__restrict__ gp = I.g;
     const size_t pitch = (size_t)I.pitch;
     const int col0 = T.px0 + lane * CPL;
-    const float c = I.c, dg = I.dg;
+    const float dg = I.dg;
+    // omega/(4+lambda) of each half-sweep of this launch (the SOR factor may
+    // change from one half-sweep to the next)
+    float cs[K];
+    {
+      const double *W = I.omega_tab + 2 * (size_t)C.sweeps_done;
+#pragma unroll
+      for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(W[s] / I.dg64) : 0.0f;
+    }
     bool cv[CPL];
 #pragma unroll
     for (int j = 0; j < CPL; ++j) cv[j] = (lane * CPL + j >= K) && (lane * CPL + j < 64 * CPL - K);
@@ -224,16 +237,16 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
               const float *Gr = G[si];
               if (j0 == 0) {
                 const float l0 = from_left(Er[3]);
-                const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
-                const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
+                const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], cs[s - 1], dg);
+                const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], cs[s - 1], dg);
                 if (rv) {
                   if (cv[0]) acc[s - 1] = fmaf(r0, r0, acc[s - 1]);
                   if (cv[2]) acc[s - 1] = fmaf(r2, r2, acc[s - 1]);
                 }
               } else {
                 const float r3n = from_right(Er[0]);
-                const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
-                const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
+                const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], cs[s - 1], dg);
+                const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], cs[s - 1], dg);
                 if (rv) {
                   if (cv[1]) acc[s - 1] = fmaf(r1, r1, acc[s - 1]);
                   if (cv[3]) acc[s - 1] = fmaf(r3, r3, acc[s - 1]);

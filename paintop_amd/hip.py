@@ -52,7 +52,7 @@ class PhSolveDesc(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("half_sweeps", ctypes.c_uint32),
         ("op", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32),
+        ("tile_rows", ctypes.c_uint32),
     ]
 
 
@@ -182,10 +182,11 @@ class SolveResult:
 
 
 def make_desc(width, height, channels, max_iterations, tolerance, lam=0.0, op=PH_OP_POISSON_BLEND,
-              omega=0.0, half_sweeps=0, flags=0) -> PhSolveDesc:
+              omega=0.0, half_sweeps=0, flags=0, tile_rows=0) -> PhSolveDesc:
     return PhSolveDesc(width=width, height=height, channels=channels,
                        max_iterations=max_iterations, tolerance=tolerance, lam=lam,
-                       omega=omega, flags=flags, half_sweeps=half_sweeps, op=op, reserved=0)
+                       omega=omega, flags=flags, half_sweeps=half_sweeps, op=op,
+                       tile_rows=tile_rows)
 
 
 def _result(rep: PhReport, hist: np.ndarray) -> SolveResult:
