@@ -78,11 +78,12 @@ void or_fbm(size_t width, size_t height, double frequency, uint64_t seed, uint32
 /* The all-cores CPU stand-in: red-black SOR in f64 on interior cells, with the
  * reference's per-sweep relative residual and stop rules (stall_window 16 is
  * the reference's; 0 turns the stall guard off, for timing fixed sweep counts).
- * threads <= 0 means the OpenMP default. Returns the thread count used. */
+ * omega_tab (NULL: the fixed omega) gives the factor of half-sweep h, red
+ * first. threads <= 0 means the OpenMP default. Returns the thread count used. */
 int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double *rhs,
                  const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
-                 double omega, int threads, uint32_t stall_window, double *field,
-                 double *history, or_report *report);
+                 double omega, const double *omega_tab, int threads, uint32_t stall_window,
+                 double *field, double *history, or_report *report);
 
 #ifdef __cplusplus
 }

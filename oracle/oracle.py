@@ -91,7 +91,7 @@ def lib():
         L.or_fbm.argtypes = [sz, sz, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32,
                              ctypes.c_double, ctypes.c_double, fp]
         L.or_rbsor_omp.argtypes = [sz, sz, up, dp, dp, ctypes.c_double, ctypes.c_uint32,
-                                   ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, dp, ctypes.c_int,
                                    ctypes.c_uint32, dp, dp, ctypes.POINTER(_Report)]
         L.or_rbsor_omp.restype = ctypes.c_int
         _lib = L
@@ -182,7 +182,7 @@ def blend_channels(source, target, mask, lam, max_iterations, tolerance, omega=1
 
 
 def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads=0,
-              stall_window=16):
+              stall_window=16, omega_tab=None):
     """The all-cores red-black SOR stand-in for cpu.optimized (stall_window 0
     disables the stall guard, for timing a fixed number of sweeps)."""
     cells = np.ascontiguousarray(cells, dtype=np.uint8)
@@ -192,9 +192,14 @@ def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads
     out = np.empty((h, w), dtype=np.float64)
     hist = np.zeros(max(int(max_iterations), 1), dtype=np.float64)
     rep = _Report()
+    tab = None
+    if omega_tab is not None:
+        tab = np.ascontiguousarray(omega_tab, dtype=np.float64)
+        assert tab.size >= 2 * int(max_iterations)
     used = lib().or_rbsor_omp(w, h, _p(cells, ctypes.c_uint8), _p(rhs, ctypes.c_double),
                               _p(anchor, ctypes.c_double), float(lam), int(max_iterations),
-                              float(tolerance), float(omega), int(threads),
+                              float(tolerance), float(omega),
+                              None if tab is None else _p(tab, ctypes.c_double), int(threads),
                               int(stall_window), _p(out, ctypes.c_double),
                               _p(hist, ctypes.c_double),
                               ctypes.byref(rep))

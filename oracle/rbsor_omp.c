@@ -5,7 +5,8 @@
  * keeps the reference's stopping rules (poisson.rs:168-240: relative residual
  * against the anchor-start residual, per-sweep history, 16-sweep stall window,
  * iteration cap) and its update formula (poisson.rs:256-262), only the visiting
- * order is red-black so the rows can be split across threads. The residual sum
+ * order is red-black so the rows can be split across threads, and the SOR
+ * factor may follow a per-half-sweep table (the GPU path's schedule). The residual sum
  * is taken per row and the row partials are added in row order, so a rerun is
  * bit-identical for any thread count.
  * Test infrastructure: used only by the tests and the cpu_baseline leg.
@@ -43,8 +44,8 @@ static double residual_norm(size_t width, size_t height, const uint8_t *cells,
 
 int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double *rhs,
                  const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
-                 double omega, int threads, uint32_t stall_window, double *field,
-                 double *history, or_report *report) {
+                 double omega, const double *omega_tab, int threads, uint32_t stall_window,
+                 double *field, double *history, or_report *report) {
   size_t n = width * height;
   int nthreads = threads > 0 ? threads : omp_get_max_threads();
   memcpy(field, anchor, n * sizeof(double));
@@ -75,6 +76,7 @@ int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double
   uint32_t iterations = 0, since = 0;
   for (uint32_t sweep = 0; sweep < max_iterations; ++sweep) {
     for (int colour = 0; colour < 2; ++colour) {
+      const double w = omega_tab ? omega_tab[2 * (size_t)sweep + (size_t)colour] : omega;
       long h1 = (long)height - 1;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
       for (long row = 1; row < h1; ++row) {
@@ -83,7 +85,7 @@ int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double
           size_t idx = (size_t)row * width + col;
           if (!cells[idx]) continue;
           double target = (nsum_at(field, idx, width) - bprime[idx]) / denom;
-          field[idx] = fma(omega, target - field[idx], field[idx]);
+          field[idx] = fma(w, target - field[idx], field[idx]);
         }
       }
     }

@@ -71,6 +71,25 @@ __global__ void k_tile_occupancy(const uint8_t *__restrict__ cls, int w, int h, 
   if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
 }
 
+// Interior-cell count of each f x f block of the region starting at image
+// cell (x0, y0): the coarse mask the SOR-factor tuner works on.
+__global__ void k_coarse_count(const uint8_t *__restrict__ cls, int w, int h, int x0, int y0,
+                               int f, int cw, int ch, int *__restrict__ counts) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cw * ch) return;
+  const int cx = c % cw, cy = c / cw;
+  int n = 0;
+  for (int dy = 0; dy < f; ++dy) {
+    const int iy = y0 + cy * f + dy;
+    if (iy < 0 || iy >= h) continue;
+    for (int dx = 0; dx < f; ++dx) {
+      const int ix = x0 + cx * f + dx;
+      if (ix >= 0 && ix < w) n += cls[(size_t)iy * w + ix];
+    }
+  }
+  counts[c] = n;
+}
+
 // Fill an instance's planes from the interleaved image channel.
 __global__ void k_assemble(PhInst I, const float *__restrict__ src, const float *__restrict__ tgt,
                            const uint8_t *__restrict__ cls_img, uint8_t *__restrict__ cls) {

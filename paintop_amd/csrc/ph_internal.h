@@ -15,7 +15,7 @@
 
 #define PH_PAD 16          // >= the largest half-sweep count per launch
 #define PH_MAX_K 16
-#define PH_STALL_WINDOW 16u       // poisson.rs:81
+#define PH_STALL_WINDOW 16u       // poisson.rs:81 (the window for a fixed omega)
 #define PH_STALL_IMPROVEMENT 1e-12 // poisson.rs:85
 
 // Per-instance descriptor (read-only for the sweep kernels).
@@ -71,6 +71,8 @@ struct PhCtl {
   uint32_t mode;
   uint32_t refine_pending;
   uint32_t launches;
+  uint32_t stall_window; // sweeps without a new best that end the solve as stalled
+  uint32_t pad_;
 };
 
 struct PhTile {

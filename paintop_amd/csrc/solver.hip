@@ -42,6 +42,7 @@ __global__ void k_sor(const PhTile *, int, const PhInst *, PhCtl *, double *, ui
 __global__ void k_classify(const float *, int, int, uint8_t *, int *);
 __global__ void k_tile_occupancy(const uint8_t *, int, int, int, int, int, int, int, const int2 *,
                                  int *);
+__global__ void k_coarse_count(const uint8_t *, int, int, int, int, int, int, int, int *);
 __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
 __global__ void k_apply_correction(PhInst, int);
 __global__ void k_residual(PhInst, double *, uint32_t *, double *);
@@ -203,25 +204,126 @@ static void validate(const ph_solve_desc *d) {
 // ---------------------------------------------------------------------------
 // tuning
 
-// SOR factor for the domain: the smallest eigenvalue mu of the 5-point
-// negative Laplacian on the interior is bounded below by the equal-area disc
-// (Faber-Krahn, mu >= (j01 / R)^2) and by the bounding rectangle (domain
-// monotonicity, mu >= (pi/bw)^2 + (pi/bh)^2); the larger bound is used, which
-// errs on the side of a slightly large omega, where SOR degrades gently.
-static double rho_jacobi(double n_interior, int bw, int bh, double lambda) {
-  const double j01 = 2.404825557695773;
-  const double R = sqrt(std::max(n_interior, 1.0) / M_PI);
-  const double mu_disc = (j01 / R) * (j01 / R);
-  const double mu_rect = (M_PI / std::max(bw, 1)) * (M_PI / std::max(bw, 1)) +
-                         (M_PI / std::max(bh, 1)) * (M_PI / std::max(bh, 1));
-  const double mu = std::min(std::max(mu_disc, mu_rect), 4.0);
-  return (4.0 - mu) / (4.0 + lambda);
+// SOR factor tuner. omega_opt = 2 / (1 + sqrt(1 - rho^2)) with the Jacobi
+// spectral radius rho = (4 - mu) / (4 + lambda), where mu is the smallest
+// eigenvalue of the 5-point negative Laplacian on the interior cells. mu is
+// measured on the mask coarsened by f (a coarse cell is interior when at least
+// half of its f x f cells are), by inverse iteration with conjugate gradients,
+// and scaled by 1/f^2. On a 4096^2 star domain a <= 128^2 coarse grid lands
+// within a few percent of the fine eigenvalue, on the low side (a slightly
+// large omega, where SOR degrades gently). Results are cached per coarse mask.
+struct TuneKey {
+  int cw, ch, f;
+  uint64_t hash;
+  bool operator==(const TuneKey &o) const {
+    return cw == o.cw && ch == o.ch && f == o.f && hash == o.hash;
+  }
+};
+static std::vector<std::pair<TuneKey, double>> g_tune_cache;
+
+static double coarse_min_eigenvalue(const std::vector<uint8_t> &in, int cw, int ch) {
+  const int n = cw * ch;
+  std::vector<int> idx(n, -1);
+  int m = 0;
+  for (int i = 0; i < n; ++i)
+    if (in[i]) idx[i] = m++;
+  if (m == 0) return 4.0;
+  std::vector<int> nb((size_t)m * 4, -1);
+  for (int y = 0; y < ch; ++y)
+    for (int x = 0; x < cw; ++x) {
+      const int k = idx[y * cw + x];
+      if (k < 0) continue;
+      nb[4 * k + 0] = x > 0 ? idx[y * cw + x - 1] : -1;
+      nb[4 * k + 1] = x + 1 < cw ? idx[y * cw + x + 1] : -1;
+      nb[4 * k + 2] = y > 0 ? idx[(y - 1) * cw + x] : -1;
+      nb[4 * k + 3] = y + 1 < ch ? idx[(y + 1) * cw + x] : -1;
+    }
+  auto apply = [&](const std::vector<double> &v, std::vector<double> &out) {
+    for (int k = 0; k < m; ++k) {
+      double s = 4.0 * v[k];
+      for (int j = 0; j < 4; ++j)
+        if (nb[4 * k + j] >= 0) s -= v[nb[4 * k + j]];
+      out[k] = s;
+    }
+  };
+  auto dot = [&](const std::vector<double> &a, const std::vector<double> &b) {
+    double s = 0;
+    for (int k = 0; k < m; ++k) s += a[k] * b[k];
+    return s;
+  };
+  std::vector<double> v(m, 1.0), x(m, 0.0), r(m), p(m), q(m);
+  double mu = 4.0;
+  for (int it = 0; it < 8; ++it) {
+    // solve A x = v by CG (warm start from the previous iterate's direction)
+    apply(x, q);
+    for (int k = 0; k < m; ++k) r[k] = v[k] - q[k];
+    p = r;
+    double rr = dot(r, r);
+    const double stop = 1e-16 * dot(v, v);
+    for (int cg = 0; cg < 4 * m + 100 && rr > stop; ++cg) {
+      apply(p, q);
+      const double alpha = rr / dot(p, q);
+      for (int k = 0; k < m; ++k) {
+        x[k] += alpha * p[k];
+        r[k] -= alpha * q[k];
+      }
+      const double rr2 = dot(r, r);
+      for (int k = 0; k < m; ++k) p[k] = r[k] + (rr2 / rr) * p[k];
+      rr = rr2;
+    }
+    const double nx = sqrt(dot(x, x));
+    for (int k = 0; k < m; ++k) v[k] = x[k] / nx;
+    apply(v, q);
+    mu = dot(v, q);  // Rayleigh quotient of the normalised iterate
+    x = v;
+    for (int k = 0; k < m; ++k) x[k] /= mu;  // next solve starts at A^-1 v ~ v / mu
+  }
+  return mu;
 }
 
-static double tuned_omega(double n_interior, int bw, int bh, double lambda) {
-  const double rho = rho_jacobi(n_interior, bw, bh, lambda);
+// mu (fine grid) of the interior cells in region [rx0, rx0+rw) x [ry0, ry0+rh).
+static double tuned_mu(const uint8_t *cls_img, int W, int H, int rx0, int ry0, int rw, int rh,
+                       hipStream_t st) {
+  const int f = std::max(1, (std::max(rw, rh) + 127) / 128);
+  const int cw = (rw + f - 1) / f, ch = (rh + f - 1) / f;
+  int *counts_d = g_pool.take<int>((size_t)cw * ch);
+  hipLaunchKernelGGL(k_coarse_count, dim3((cw * ch + 255) / 256), dim3(256), 0, st, cls_img, W, H,
+                     rx0, ry0, f, cw, ch, counts_d);
+  HIP_OK(hipGetLastError());
+  std::vector<int> counts((size_t)cw * ch);
+  HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
+                        hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  std::vector<uint8_t> in(counts.size());
+  uint64_t hsh = 1469598103934665603ull;  // FNV-1a of the coarse mask
+  for (size_t i = 0; i < counts.size(); ++i) {
+    in[i] = 2 * counts[i] >= f * f ? 1 : 0;
+    hsh = (hsh ^ in[i]) * 1099511628211ull;
+  }
+  const TuneKey key{cw, ch, f, hsh};
+  for (const auto &e : g_tune_cache)
+    if (e.first == key) return e.second;
+  const double mu = coarse_min_eigenvalue(in, cw, ch) / ((double)f * f);
+  if (g_tune_cache.size() > 256) g_tune_cache.erase(g_tune_cache.begin());
+  g_tune_cache.push_back({key, mu});
+  return mu;
+}
+
+static double omega_for(double mu, double lambda) {
+  const double rho = (4.0 - std::min(mu, 4.0)) / (4.0 + lambda);
   const double w = 2.0 / (1.0 + sqrt(std::max(1.0 - rho * rho, 0.0)));
   return std::min(std::max(w, 1.0), 1.9999);
+}
+
+// Window of the stall guard (poisson.rs:215-228). The reference's 16 sweeps
+// fit its fixed omega = 1.8; the over-relaxation transient, during which the
+// residual of red/black SOR may rise before it falls, lasts ~1/(2 - omega)
+// sweeps, so a tuned omega gets a window of 3/(2 - omega) (16 for any
+// omega <= 1.81). A fixed omega from the caller keeps the reference's 16.
+static uint32_t stall_window_for(double omega, bool tuned) {
+  if (!tuned) return PH_STALL_WINDOW;
+  const double w = ceil(3.0 / std::max(2.0 - omega, 1e-6));
+  return (uint32_t)std::max<double>(PH_STALL_WINDOW, std::min(w, 1e6));
 }
 
 typedef void (*sor_fn)(const PhTile *, int, const PhInst *, PhCtl *, double *, uint32_t *, int);
@@ -293,7 +395,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
   // --- classification and bounding box
   const int K = d->half_sweeps ? (int)d->half_sweeps : 8;
   const int CPL = 4, WS = 64 * CPL, WV = WS - 2 * K;
-  int hb = d->tile_rows ? (int)d->tile_rows : 128;
+  int hb = d->tile_rows ? (int)d->tile_rows : 64;
   {
     // upper bound of the workspace for this call, so the pool never regrows mid-call
     const size_t pw_up = (size_t)W + 2 * PH_PAD + 4 + WV + 2 * K + 64;
@@ -305,7 +407,8 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
                         (size_t)(C + 2) * (d->max_iterations + 1) * sizeof(double) +
                         (size_t)C * ph_up * sizeof(double) +
                         ntile_up * (size_t)C * (sizeof(PhTile) + K * sizeof(double)) +
-                        (size_t)C * (sizeof(PhInst) + sizeof(PhCtl) + 64) + 64 * 256;
+                        (size_t)C * (sizeof(PhInst) + sizeof(PhCtl) + 64) + 64 * 256 +
+                        130 * 130 * sizeof(int);
     g_pool.reset(need);
   }
   uint8_t *cls_img = g_pool.take<uint8_t>(npx);
@@ -364,24 +467,21 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       }
   }
 
-  const double omega = d->omega > 0.0 ? d->omega
-                                      : tuned_omega(n_interior, region_w, region_h, d->lambda);
+  const bool tuned = d->omega <= 0.0;
+  double omega = d->omega;
+  if (tuned) {
+    const double mu = any_interior ? tuned_mu(cls_img, W, H, ox + PH_PAD, oy + PH_PAD, region_w,
+                                              region_h, st)
+                                   : 4.0;
+    omega = omega_for(mu, d->lambda);
+  }
   rep->omega_used = omega;
   rep->half_sweeps_used = (uint32_t)K;
-  // SOR factor per half-sweep. A fixed omega when the caller asks for one;
-  // otherwise Chebyshev acceleration of red/black SOR: omega_1 = 1,
-  // omega_2 = 1/(1 - rho^2/2), omega_{h+1} = 1/(1 - rho^2 omega_h / 4), which
-  // rises to omega_opt = 2/(1 + sqrt(1 - rho^2)) without the transient residual
-  // growth a large fixed omega shows on big domains.
+  const uint32_t stall_window = stall_window_for(omega, tuned);
+  // SOR factor of every half-sweep (one value for now; the table lets a
+  // schedule vary it per half-sweep)
   const size_t ntab = 2 * (size_t)d->max_iterations + 2;
   std::vector<double> wtab(ntab, omega);
-  if (d->omega <= 0.0) {
-    const double rho = rho_jacobi(n_interior, region_w, region_h, d->lambda);
-    const double r2 = rho * rho;
-    wtab[0] = 1.0;
-    if (ntab > 1) wtab[1] = 1.0 / (1.0 - 0.5 * r2);
-    for (size_t h = 2; h < ntab; ++h) wtab[h] = 1.0 / (1.0 - 0.25 * r2 * wtab[h - 1]);
-  }
   double *omega_d = g_pool.take<double>(ntab);
   HIP_OK(hipMemcpyAsync(omega_d, wtab.data(), ntab * sizeof(double), hipMemcpyHostToDevice, st));
 
@@ -495,6 +595,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       Cc.sweep_limit = d->max_iterations;
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
+      Cc.stall_window = stall_window;
     }
     HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
 
@@ -535,7 +636,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
         return false;
       }
       Cc.since += 1;
-      return Cc.since >= PH_STALL_WINDOW;
+      return Cc.since >= Cc.stall_window;
     };
     auto resume = [&](int a, double gnorm) {
       PhCtl &Cc = ctl[a];

@@ -81,7 +81,7 @@ __device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
     C.since = 0;
   } else {
     C.since += 1;
-    if (C.since >= PH_STALL_WINDOW) {
+    if (C.since >= C.stall_window) {
       C.reason = PH_REASON_STALLED;
       C.halt_sweep = m;
       return true;
