@@ -72,6 +72,8 @@ struct PhCtl {
   uint32_t refine_pending;
   uint32_t launches;
   uint32_t stall_window; // sweeps without a new best that end the solve as stalled
+  uint32_t refine_every; // refine at least this often (sweeps): bounds |e| and the FP32 floor
+  uint32_t last_refine;  // sweep count of the last refinement
   uint32_t pad_;
 };
 

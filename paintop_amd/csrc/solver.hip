@@ -37,8 +37,10 @@
 
 namespace ph {
 
+template <int K, int PF>
+__global__ void k_sor(const PhTile *, int, const PhInst *, const PhCtl *, double *, int);
 template <int K>
-__global__ void k_sor(const PhTile *, int, const PhInst *, PhCtl *, double *, uint32_t *, int);
+__global__ void k_sor_control(const PhInst *, PhCtl *, const double *, int);
 __global__ void k_classify(const float *, int, int, uint8_t *, int *);
 __global__ void k_tile_occupancy(const uint8_t *, int, int, int, int, int, int, int, const int2 *,
                                  int *);
@@ -326,20 +328,46 @@ static uint32_t stall_window_for(double omega, bool tuned) {
   return (uint32_t)std::max<double>(PH_STALL_WINDOW, std::min(w, 1e6));
 }
 
-typedef void (*sor_fn)(const PhTile *, int, const PhInst *, PhCtl *, double *, uint32_t *, int);
-
-static void launch_sor(int K, int ntiles, hipStream_t st, const PhTile *tiles,
-                       const PhInst *insts, PhCtl *ctl, double *partials, uint32_t *arrivals,
-                       int hb) {
-  dim3 grid(ntiles), block(64);
+// One sweep launch: the tiles, then the per-instance control.
+static void launch_sor(int K, int PF, int ntiles, int ninst, hipStream_t st, const PhTile *tiles,
+                       const PhInst *insts, PhCtl *ctl, double *partials, int hb) {
+  const dim3 grid(ntiles), block(64), cgrid(ninst);
+#define PH_SOR(KK, PP)                                                                      \
+  hipLaunchKernelGGL((k_sor<KK, PP>), grid, block, 0, st, tiles, ntiles, insts, ctl, partials, hb)
+  switch (K * 8 + PF) {
+    case 2 * 8 + 1: PH_SOR(2, 1); break;
+    case 4 * 8 + 1: PH_SOR(4, 1); break;
+    case 8 * 8 + 1: PH_SOR(8, 1); break;
+    case 12 * 8 + 1: PH_SOR(12, 1); break;
+    case 16 * 8 + 1: PH_SOR(16, 1); break;
+    case 2 * 8 + 3: PH_SOR(2, 3); break;
+    case 4 * 8 + 3: PH_SOR(4, 3); break;
+    case 8 * 8 + 3: PH_SOR(8, 3); break;
+    case 12 * 8 + 3: PH_SOR(12, 3); break;
+    case 16 * 8 + 3: PH_SOR(16, 3); break;
+    case 8 * 8 + 5: PH_SOR(8, 5); break;
+    case 16 * 8 + 5: PH_SOR(16, 5); break;
+    default: throw fail(PH_CLASS_SCHEMA, "E_HIP_TUNING", "no sweep kernel for K=%d prefetch=%d", K, PF);
+  }
+#undef PH_SOR
+  HIP_OK(hipGetLastError());
   switch (K) {
-    case 2: hipLaunchKernelGGL(k_sor<2>, grid, block, 0, st, tiles, ntiles, insts, ctl, partials, arrivals, hb); break;
-    case 4: hipLaunchKernelGGL(k_sor<4>, grid, block, 0, st, tiles, ntiles, insts, ctl, partials, arrivals, hb); break;
-    case 8: hipLaunchKernelGGL(k_sor<8>, grid, block, 0, st, tiles, ntiles, insts, ctl, partials, arrivals, hb); break;
-    case 12: hipLaunchKernelGGL(k_sor<12>, grid, block, 0, st, tiles, ntiles, insts, ctl, partials, arrivals, hb); break;
-    default: hipLaunchKernelGGL(k_sor<16>, grid, block, 0, st, tiles, ntiles, insts, ctl, partials, arrivals, hb); break;
+    case 2: hipLaunchKernelGGL(k_sor_control<2>, cgrid, block, 0, st, insts, ctl, partials, ninst); break;
+    case 4: hipLaunchKernelGGL(k_sor_control<4>, cgrid, block, 0, st, insts, ctl, partials, ninst); break;
+    case 8: hipLaunchKernelGGL(k_sor_control<8>, cgrid, block, 0, st, insts, ctl, partials, ninst); break;
+    case 12: hipLaunchKernelGGL(k_sor_control<12>, cgrid, block, 0, st, insts, ctl, partials, ninst); break;
+    default: hipLaunchKernelGGL(k_sor_control<16>, cgrid, block, 0, st, insts, ctl, partials, ninst); break;
   }
   HIP_OK(hipGetLastError());
+}
+
+// Rows a sweep tile loads ahead of use (1, 3 or 5; PAINTOP_HIP_PREFETCH overrides).
+static int prefetch_rows(int K) {
+  const char *e = getenv("PAINTOP_HIP_PREFETCH");
+  int pf = e ? atoi(e) : 3;
+  if (pf != 1 && pf != 3 && pf != 5) pf = 3;
+  if (pf == 5 && K != 8 && K != 16) pf = 3;
+  return pf;
 }
 
 // ---------------------------------------------------------------------------
@@ -356,7 +384,12 @@ struct ChanResult {
   uint64_t interior = 0;
 };
 
+// Refinement triggers: the inner residual fell 1e3-fold, or REFINE_EVERY
+// sweeps passed. The second keeps the correction e small, so the FP32 rounding
+// floor of the correction equation (~ulp(|e|) / |g| relative) stays far below
+// the tolerances asked for.
 static const double REFINE_RATIO = 1e-3;
+static const uint32_t REFINE_EVERY = 256;
 
 static void solve_device(const ph_solve_desc *d, const float *src, const float *tgt,
                          const float *mask, float *out, hipStream_t st, ph_report *rep) {
@@ -577,10 +610,8 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
     PhInst *inst_d = g_pool.take<PhInst>(nact);
     PhCtl *ctl_d = g_pool.take<PhCtl>(nact);
     double *partials_d = g_pool.take<double>((size_t)ntiles * K);
-    uint32_t *arr_d = g_pool.take<uint32_t>(nact);
     HIP_OK(hipMemcpyAsync(tiles_d, tl.data(), ntiles * sizeof(PhTile), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(inst_d, ai.data(), nact * sizeof(PhInst), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemsetAsync(arr_d, 0, nact * sizeof(uint32_t), st));
     std::vector<PhCtl> ctl(nact);
     for (int a = 0; a < nact; ++a) {
       PhCtl &Cc = ctl[a];
@@ -596,6 +627,8 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
       Cc.stall_window = stall_window;
+      Cc.refine_every = REFINE_EVERY;
+      Cc.last_refine = 0;
     }
     HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
 
@@ -640,6 +673,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
     };
     auto resume = [&](int a, double gnorm) {
       PhCtl &Cc = ctl[a];
+      Cc.last_refine = Cc.sweeps_done;
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
       Cc.sweep_limit = Cc.max_it;
@@ -661,6 +695,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       resume(a, gnorm);
     };
 
+    const int PF = prefetch_rows(K);
     int batch = 4;
     int remaining = nact;
     double guard = 0;
@@ -674,7 +709,7 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
     while (remaining > 0) {
       if (timing) HIP_OK(hipEventRecord(eb0, st));
       for (int i = 0; i < batch; ++i) {
-        launch_sor(K, ntiles, st, tiles_d, inst_d, ctl_d, partials_d, arr_d, hb);
+        launch_sor(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb);
         rep->launches += 1;
       }
       if (timing) HIP_OK(hipEventRecord(eb1, st));
@@ -723,7 +758,20 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
         if (res[act[a]].done) --remaining;
       }
       HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
-      batch = any_halt ? 4 : std::min(batch * 2, 64);
+      // next batch: up to the next scheduled halt (refinement or sweep limit) of
+      // the running instances, so no launch finds every instance halted
+      {
+        uint32_t need = 0;
+        for (int a = 0; a < nact; ++a) {
+          const PhCtl &Cc = ctl[a];
+          if (res[act[a]].done || Cc.halt != PH_RUN) continue;
+          uint32_t next = std::min(Cc.sweep_limit, Cc.last_refine + Cc.refine_every);
+          if (Cc.mode == PH_MODE_REDO) next = Cc.sweep_limit;
+          const uint32_t todo = next > Cc.sweeps_done ? next - Cc.sweeps_done : 1;
+          need = std::max(need, (todo + (uint32_t)(K / 2) - 1) / (uint32_t)(K / 2));
+        }
+        batch = (int)std::max<uint32_t>(1, std::min<uint32_t>(need, 64));
+      }
       // every batch must move some instance forward (sweeps or halt state)
       uint64_t progress = 0;
       for (int a = 0; a < nact; ++a)

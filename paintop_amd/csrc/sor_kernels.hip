@@ -92,7 +92,7 @@ __device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
 }
 
 // Control for one instance after all of its tiles finished a launch. Runs on
-// one whole wave (the last tile to arrive).
+// one whole wave.
 template <int K>
 __device__ void control(PhCtl &C, const PhInst &I, const double *partials, int lane) {
   const uint32_t S = C.sweeps_done;
@@ -136,6 +136,7 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *partials, int l
   }
   C.sweeps_done = S + nsw;
   C.cur_buf = in_buf ^ 1u;
+  if (C.sweeps_done - C.last_refine >= C.refine_every) C.refine_pending = 1;
   if (C.sweeps_done >= C.sweep_limit) C.halt = PH_HALT_LIMIT;
   else if (C.refine_pending) C.halt = PH_HALT_REFINE;
 }
@@ -149,155 +150,168 @@ __device__ __forceinline__ float relax(float &e, float g, float l, float r, floa
   return rr;
 }
 
-template <int K>
+// PF: rows loaded ahead of use (odd, so the ring size stays even and the
+// red/black colour of a ring slot is fixed at compile time).
+template <int K, int PF>
 __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, int ntiles,
-                                            const PhInst *__restrict__ insts, PhCtl *ctl,
-                                            double *__restrict__ partials,
-                                            uint32_t *__restrict__ arrivals, int hb) {
+                                            const PhInst *__restrict__ insts,
+                                            const PhCtl *__restrict__ ctl,
+                                            double *__restrict__ partials, int hb) {
   constexpr int CPL = 4;
-  constexpr int U = K + 2;  // live rows: t-K .. t+1
+  constexpr int U = K + PF + 1;  // live rows t-K .. t+1, plus rows in flight up to t+PF
+  static_assert((U & 1) == 0, "ring size must be even");
   const int lane = threadIdx.x;
   const int b = xcd_tile(blockIdx.x, ntiles);
   const PhTile T = tiles[b];
   const PhInst &I = insts[T.inst];
-  PhCtl &C = ctl[T.inst];
+  const PhCtl &C = ctl[T.inst];
+  if (C.halt != PH_RUN) return;
 
-  if (C.halt == PH_RUN) {
-    const int k_run = min(K, (int)(2 * (C.sweep_limit - C.sweeps_done)));
-    const uint32_t cur = C.cur_buf;
-    const float *__restrict__ ein = I.e[cur];
-    float *__restrict__ eout = I.e[cur ^ 1u];
-    const float *__restrict__ gp = I.g;
-    const size_t pitch = (size_t)I.pitch;
-    const int col0 = T.px0 + lane * CPL;
-    const float dg = I.dg;
-    // omega/(4+lambda) of each half-sweep of this launch (the SOR factor may
-    // change from one half-sweep to the next)
-    float cs[K];
-    {
-      const double *W = I.omega_tab + 2 * (size_t)C.sweeps_done;
+  const int k_run = min(K, (int)(2 * (C.sweep_limit - C.sweeps_done)));
+  const uint32_t cur = C.cur_buf;
+  const float *__restrict__ ein = I.e[cur];
+  float *__restrict__ eout = I.e[cur ^ 1u];
+  const float *__restrict__ gp = I.g;
+  const size_t pitch = (size_t)I.pitch;
+  const int col0 = T.px0 + lane * CPL;
+  const float dg = I.dg;
+  // omega/(4+lambda) of each half-sweep of this launch (the SOR factor may
+  // change from one half-sweep to the next)
+  float cs[K];
+  {
+    const double *W = I.omega_tab + 2 * (size_t)C.sweeps_done;
 #pragma unroll
-      for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(W[s] / I.dg64) : 0.0f;
-    }
-    bool cv[CPL];
+    for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(W[s] / I.dg64) : 0.0f;
+  }
+  bool cv[CPL];
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) cv[j] = (lane * CPL + j >= K) && (lane * CPL + j < 64 * CPL - K);
-    bool lane_full = cv[0] && cv[CPL - 1];
+  for (int j = 0; j < CPL; ++j) cv[j] = (lane * CPL + j >= K) && (lane * CPL + j < 64 * CPL - K);
+  const bool lane_full = cv[0] && cv[CPL - 1];
 
-    float E[U][CPL], G[U][CPL];
-    float acc[K];
+  float E[U][CPL], G[U][CPL];
+  float acc[K];
 #pragma unroll
-    for (int s = 0; s < K; ++s) acc[s] = 0.0f;
+  for (int s = 0; s < K; ++s) acc[s] = 0.0f;
 #pragma unroll
-    for (int r = 0; r < U; ++r)
+  for (int r = 0; r < U; ++r)
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) E[r][j] = G[r][j] = 0.0f;
+    for (int j = 0; j < CPL; ++j) E[r][j] = G[r][j] = 0.0f;
 
-    const int nrows = hb + 2 * K;
-    const int nsteps = nrows - 1;
-    const float *erow = ein + (size_t)T.py0 * pitch + col0;
-    const float *grow = gp + (size_t)T.py0 * pitch + col0;
-    float *orow = eout + (size_t)T.py0 * pitch + col0;
-    {
-      const float4 ev = *reinterpret_cast<const float4 *>(erow);
-      const float4 gv = *reinterpret_cast<const float4 *>(grow);
-      E[0][0] = ev.x; E[0][1] = ev.y; E[0][2] = ev.z; E[0][3] = ev.w;
-      G[0][0] = gv.x; G[0][1] = gv.y; G[0][2] = gv.z; G[0][3] = gv.w;
-    }
+  const int nrows = hb + 2 * K;
+  const int nsteps = nrows - 1;
+  const float *erow = ein + (size_t)T.py0 * pitch + col0;
+  const float *grow = gp + (size_t)T.py0 * pitch + col0;
+  float *orow = eout + (size_t)T.py0 * pitch + col0;
+#pragma unroll
+  for (int r = 0; r < PF; ++r) {  // rows 0 .. PF-1 (nrows > PF always)
+    const float4 ev = *reinterpret_cast<const float4 *>(erow + (size_t)r * pitch);
+    const float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t)r * pitch);
+    E[r][0] = ev.x; E[r][1] = ev.y; E[r][2] = ev.z; E[r][3] = ev.w;
+    G[r][0] = gv.x; G[r][1] = gv.y; G[r][2] = gv.z; G[r][3] = gv.w;
+  }
 
-    for (int t0 = 0; t0 < nsteps; t0 += U) {
+  for (int t0 = 0; t0 < nsteps; t0 += U) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int t = t0 + u;
-        if (t < nsteps) {
-          {  // bring in row t+1 (t+1 <= nrows-1 always holds here)
-            constexpr int dummy = 0;
-            (void)dummy;
-            const int sl = (u + 1) % U;
-            const size_t off = (size_t)(t + 1) * pitch;
-            const float4 ev = *reinterpret_cast<const float4 *>(erow + off);
-            const float4 gv = *reinterpret_cast<const float4 *>(grow + off);
-            E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
-            G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
-          }
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u;
+      if (t < nsteps) {
+        if (t + PF < nrows) {  // row t+PF goes into the slot of row t+PF-U (< t-K, dead)
+          const int sl = (u + PF) % U;
+          const size_t off = (size_t)(t + PF) * pitch;
+          const float4 ev = *reinterpret_cast<const float4 *>(erow + off);
+          const float4 gv = *reinterpret_cast<const float4 *>(grow + off);
+          E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
+          G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
+        }
 #pragma unroll
-          for (int s = 1; s <= K; ++s) {
-            // stage s on row i = t - s + 1; rows above i < s never reach the valid core
-            if (s <= k_run && t >= 2 * s - 1) {
-              const int si = ((u - s + 1) % U + U) % U;
-              const int su = ((u - s) % U + U) % U;
-              const int sd = ((u - s + 2) % U + U) % U;
-              const int colour = (s - 1) & 1;
-              const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
-              const int i = t - s + 1;
-              const bool rv = (i >= K) && (i < K + hb);
-              float *Er = E[si];
-              const float *Eu = E[su];
-              const float *Ed = E[sd];
-              const float *Gr = G[si];
-              if (j0 == 0) {
-                const float l0 = from_left(Er[3]);
-                const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], cs[s - 1], dg);
-                const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], cs[s - 1], dg);
-                if (rv) {
-                  if (cv[0]) acc[s - 1] = fmaf(r0, r0, acc[s - 1]);
-                  if (cv[2]) acc[s - 1] = fmaf(r2, r2, acc[s - 1]);
-                }
-              } else {
-                const float r3n = from_right(Er[0]);
-                const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], cs[s - 1], dg);
-                const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], cs[s - 1], dg);
-                if (rv) {
-                  if (cv[1]) acc[s - 1] = fmaf(r1, r1, acc[s - 1]);
-                  if (cv[3]) acc[s - 1] = fmaf(r3, r3, acc[s - 1]);
-                }
+        for (int s = 1; s <= K; ++s) {
+          // stage s on row i = t - s + 1; rows above i < s never reach the valid core
+          if (s <= k_run && t >= 2 * s - 1) {
+            const int si = ((u - s + 1) % U + U) % U;
+            const int su = ((u - s) % U + U) % U;
+            const int sd = ((u - s + 2) % U + U) % U;
+            const int colour = (s - 1) & 1;
+            const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
+            const int i = t - s + 1;
+            const bool rv = (i >= K) && (i < K + hb);
+            float *Er = E[si];
+            const float *Eu = E[su];
+            const float *Ed = E[sd];
+            const float *Gr = G[si];
+            if (j0 == 0) {
+              const float l0 = from_left(Er[3]);
+              const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], cs[s - 1], dg);
+              const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], cs[s - 1], dg);
+              if (rv) {
+                if (cv[0]) acc[s - 1] = fmaf(r0, r0, acc[s - 1]);
+                if (cv[2]) acc[s - 1] = fmaf(r2, r2, acc[s - 1]);
+              }
+            } else {
+              const float r3n = from_right(Er[0]);
+              const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], cs[s - 1], dg);
+              const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], cs[s - 1], dg);
+              if (rv) {
+                if (cv[1]) acc[s - 1] = fmaf(r1, r1, acc[s - 1]);
+                if (cv[3]) acc[s - 1] = fmaf(r3, r3, acc[s - 1]);
               }
             }
           }
-          // row t-K+1 has had all of its half-sweeps
-          const int w = t - K + 1;
-          if (w >= K && w < K + hb) {
-            const int sw = ((u - K + 1) % U + U) % U;
-            float *dst = orow + (size_t)w * pitch;
-            if (lane_full) {
-              *reinterpret_cast<float4 *>(dst) = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
-            } else {
+        }
+        // row t-K+1 has had all of its half-sweeps
+        const int w = t - K + 1;
+        if (w >= K && w < K + hb) {
+          const int sw = ((u - K + 1) % U + U) % U;
+          float *dst = orow + (size_t)w * pitch;
+          if (lane_full) {
+            *reinterpret_cast<float4 *>(dst) = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
+          } else {
 #pragma unroll
-              for (int j = 0; j < CPL; ++j)
-                if (cv[j]) dst[j] = E[sw][j];
-            }
+            for (int j = 0; j < CPL; ++j)
+              if (cv[j]) dst[j] = E[sw][j];
           }
         }
       }
     }
-#pragma unroll
-    for (int s = 0; s < K; ++s) {
-      const double tot = wave_sum((double)acc[s]);
-      if (lane == 0) partials[(size_t)b * K + s] = tot;
-    }
   }
-
-  // The last tile of the instance to arrive runs the instance's control.
-  __threadfence();
-  uint32_t ticket = 0;
-  if (lane == 0) ticket = atomicAdd(&arrivals[T.inst], 1u);
-  ticket = __shfl(ticket, 0, 64);
-  if (ticket == (uint32_t)(I.tile_end - I.tile_begin - 1)) {
-    __threadfence();
-    if (C.halt == PH_RUN) control<K>(C, I, partials, lane);
-    if (lane == 0) arrivals[T.inst] = 0u;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const double tot = wave_sum((double)acc[s]);
+    if (lane == 0) partials[(size_t)b * K + s] = tot;
   }
 }
 
-template __global__ void k_sor<2>(const PhTile *, int, const PhInst *, PhCtl *, double *,
-                                  uint32_t *, int);
-template __global__ void k_sor<4>(const PhTile *, int, const PhInst *, PhCtl *, double *,
-                                  uint32_t *, int);
-template __global__ void k_sor<8>(const PhTile *, int, const PhInst *, PhCtl *, double *,
-                                  uint32_t *, int);
-template __global__ void k_sor<12>(const PhTile *, int, const PhInst *, PhCtl *, double *,
-                                   uint32_t *, int);
-template __global__ void k_sor<16>(const PhTile *, int, const PhInst *, PhCtl *, double *,
-                                   uint32_t *, int);
+// One wave per instance, after the sweep launch: the instance's residual
+// sums and stop rules (the kernel boundary orders it after every tile).
+template <int K>
+__global__ __launch_bounds__(64) void k_sor_control(const PhInst *__restrict__ insts,
+                                                    PhCtl *ctl, const double *__restrict__ partials,
+                                                    int ninst) {
+  const int a = blockIdx.x;
+  if (a >= ninst) return;
+  PhCtl &C = ctl[a];
+  if (C.halt != PH_RUN) return;
+  control<K>(C, insts[a], partials, threadIdx.x);
+}
+
+#define PH_INSTANTIATE(K, PF)                                                                \
+  template __global__ void k_sor<K, PF>(const PhTile *, int, const PhInst *, const PhCtl *,  \
+                                        double *, int);
+PH_INSTANTIATE(2, 1)
+PH_INSTANTIATE(4, 1)
+PH_INSTANTIATE(8, 1)
+PH_INSTANTIATE(12, 1)
+PH_INSTANTIATE(16, 1)
+PH_INSTANTIATE(2, 3)
+PH_INSTANTIATE(4, 3)
+PH_INSTANTIATE(8, 3)
+PH_INSTANTIATE(12, 3)
+PH_INSTANTIATE(16, 3)
+PH_INSTANTIATE(8, 5)
+PH_INSTANTIATE(16, 5)
+template __global__ void k_sor_control<2>(const PhInst *, PhCtl *, const double *, int);
+template __global__ void k_sor_control<4>(const PhInst *, PhCtl *, const double *, int);
+template __global__ void k_sor_control<8>(const PhInst *, PhCtl *, const double *, int);
+template __global__ void k_sor_control<12>(const PhInst *, PhCtl *, const double *, int);
+template __global__ void k_sor_control<16>(const PhInst *, PhCtl *, const double *, int);
 
 }  // namespace ph
