@@ -150,6 +150,12 @@ __device__ __forceinline__ float relax(float &e, float g, float l, float r, floa
   return rr;
 }
 
+// Global-address-space views, so loads and stores compile to global_* (not
+// flat_*, which also wait on the LDS counter).
+typedef __attribute__((address_space(1))) const float4 gcf4;
+typedef __attribute__((address_space(1))) float4 gf4;
+typedef __attribute__((address_space(1))) float gf1;
+
 // PF: rows loaded ahead of use (odd, so the ring size stays even and the
 // red/black colour of a ring slot is fixed at compile time).
 template <int K, int PF>
@@ -169,24 +175,27 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
 
   const int k_run = min(K, (int)(2 * (C.sweep_limit - C.sweeps_done)));
   const uint32_t cur = C.cur_buf;
-  const float *__restrict__ ein = I.e[cur];
-  float *__restrict__ eout = I.e[cur ^ 1u];
-  const float *__restrict__ gp = I.g;
   const size_t pitch = (size_t)I.pitch;
-  const int col0 = T.px0 + lane * CPL;
+  const size_t base = (size_t)T.py0 * pitch + T.px0 + lane * CPL;
+  gcf4 *erow = (gcf4 *)(I.e[cur] + base);
+  gcf4 *grow = (gcf4 *)(I.g + base);
+  float *ob = I.e[cur ^ 1u] + base;
   const float dg = I.dg;
-  // omega/(4+lambda) of each half-sweep of this launch (the SOR factor may
-  // change from one half-sweep to the next)
+  // omega/(4+lambda) of each half-sweep of this launch; 0 past k_run, which
+  // leaves e unchanged (the control reads only the first k_run residual sums)
   float cs[K];
   {
     const double *W = I.omega_tab + 2 * (size_t)C.sweeps_done;
 #pragma unroll
     for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(W[s] / I.dg64) : 0.0f;
   }
-  bool cv[CPL];
+  // columns of the valid core, as 0/1 weights of the residual sums
+  float cw[CPL];
 #pragma unroll
-  for (int j = 0; j < CPL; ++j) cv[j] = (lane * CPL + j >= K) && (lane * CPL + j < 64 * CPL - K);
-  const bool lane_full = cv[0] && cv[CPL - 1];
+  for (int j = 0; j < CPL; ++j)
+    cw[j] = (lane * CPL + j >= K) && (lane * CPL + j < 64 * CPL - K) ? 1.0f : 0.0f;
+  const bool lane_full = cw[0] != 0.0f && cw[CPL - 1] != 0.0f;
+  const bool lane_any = cw[0] != 0.0f || cw[CPL - 1] != 0.0f;
 
   float E[U][CPL], G[U][CPL];
   float acc[K];
@@ -199,13 +208,10 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
 
   const int nrows = hb + 2 * K;
   const int nsteps = nrows - 1;
-  const float *erow = ein + (size_t)T.py0 * pitch + col0;
-  const float *grow = gp + (size_t)T.py0 * pitch + col0;
-  float *orow = eout + (size_t)T.py0 * pitch + col0;
 #pragma unroll
   for (int r = 0; r < PF; ++r) {  // rows 0 .. PF-1 (nrows > PF always)
-    const float4 ev = *reinterpret_cast<const float4 *>(erow + (size_t)r * pitch);
-    const float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t)r * pitch);
+    const float4 ev = erow[(size_t)r * (pitch / 4)];
+    const float4 gv = grow[(size_t)r * (pitch / 4)];
     E[r][0] = ev.x; E[r][1] = ev.y; E[r][2] = ev.z; E[r][3] = ev.w;
     G[r][0] = gv.x; G[r][1] = gv.y; G[r][2] = gv.z; G[r][3] = gv.w;
   }
@@ -217,57 +223,56 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
       if (t < nsteps) {
         if (t + PF < nrows) {  // row t+PF goes into the slot of row t+PF-U (< t-K, dead)
           const int sl = (u + PF) % U;
-          const size_t off = (size_t)(t + PF) * pitch;
-          const float4 ev = *reinterpret_cast<const float4 *>(erow + off);
-          const float4 gv = *reinterpret_cast<const float4 *>(grow + off);
+          const size_t off = (size_t)(t + PF) * (pitch / 4);
+          const float4 ev = erow[off];
+          const float4 gv = grow[off];
           E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
           G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
         }
+        // Stage s on row i = t - s + 1. Rows above the tile (i < 0, zero slots)
+        // and rows i < s never reach the valid core; updating them anyway keeps
+        // the loop free of branches.
 #pragma unroll
         for (int s = 1; s <= K; ++s) {
-          // stage s on row i = t - s + 1; rows above i < s never reach the valid core
-          if (s <= k_run && t >= 2 * s - 1) {
-            const int si = ((u - s + 1) % U + U) % U;
-            const int su = ((u - s) % U + U) % U;
-            const int sd = ((u - s + 2) % U + U) % U;
-            const int colour = (s - 1) & 1;
-            const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
-            const int i = t - s + 1;
-            const bool rv = (i >= K) && (i < K + hb);
-            float *Er = E[si];
-            const float *Eu = E[su];
-            const float *Ed = E[sd];
-            const float *Gr = G[si];
-            if (j0 == 0) {
-              const float l0 = from_left(Er[3]);
-              const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], cs[s - 1], dg);
-              const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], cs[s - 1], dg);
-              if (rv) {
-                if (cv[0]) acc[s - 1] = fmaf(r0, r0, acc[s - 1]);
-                if (cv[2]) acc[s - 1] = fmaf(r2, r2, acc[s - 1]);
-              }
-            } else {
-              const float r3n = from_right(Er[0]);
-              const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], cs[s - 1], dg);
-              const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], cs[s - 1], dg);
-              if (rv) {
-                if (cv[1]) acc[s - 1] = fmaf(r1, r1, acc[s - 1]);
-                if (cv[3]) acc[s - 1] = fmaf(r3, r3, acc[s - 1]);
-              }
-            }
+          const int si = ((u - s + 1) % U + U) % U;
+          const int su = ((u - s) % U + U) % U;
+          const int sd = ((u - s + 2) % U + U) % U;
+          const int colour = (s - 1) & 1;
+          const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
+          const int i = t - s + 1;
+          const float rw = (i >= K && i < K + hb) ? 1.0f : 0.0f;  // valid-core row
+          float *Er = E[si];
+          const float *Eu = E[su];
+          const float *Ed = E[sd];
+          const float *Gr = G[si];
+          const float c = cs[s - 1];
+          if (j0 == 0) {
+            const float l0 = from_left(Er[3]);
+            const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
+            const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
+            acc[s - 1] = fmaf(r0 * (rw * cw[0]), r0, acc[s - 1]);
+            acc[s - 1] = fmaf(r2 * (rw * cw[2]), r2, acc[s - 1]);
+          } else {
+            const float r3n = from_right(Er[0]);
+            const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
+            const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
+            acc[s - 1] = fmaf(r1 * (rw * cw[1]), r1, acc[s - 1]);
+            acc[s - 1] = fmaf(r3 * (rw * cw[3]), r3, acc[s - 1]);
           }
         }
         // row t-K+1 has had all of its half-sweeps
         const int w = t - K + 1;
         if (w >= K && w < K + hb) {
           const int sw = ((u - K + 1) % U + U) % U;
-          float *dst = orow + (size_t)w * pitch;
-          if (lane_full) {
-            *reinterpret_cast<float4 *>(dst) = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
-          } else {
+          float *dst = ob + (size_t)w * pitch;
+          if (K % 4 == 0) {  // every lane is wholly inside or outside the core
+            if (lane_full) *(gf4 *)dst = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
+          } else if (lane_full) {
+            *(gf4 *)dst = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
+          } else if (lane_any) {
 #pragma unroll
             for (int j = 0; j < CPL; ++j)
-              if (cv[j]) dst[j] = E[sw][j];
+              if (cw[j] != 0.0f) ((gf1 *)dst)[j] = E[sw][j];
           }
         }
       }
