@@ -152,8 +152,9 @@ __device__ __forceinline__ float relax(float &e, float g, float l, float r, floa
 
 // Global-address-space views, so loads and stores compile to global_* (not
 // flat_*, which also wait on the LDS counter).
-typedef __attribute__((address_space(1))) const float4 gcf4;
-typedef __attribute__((address_space(1))) float4 gf4;
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const v4f gcf4;
+typedef __attribute__((address_space(1))) v4f gf4;
 typedef __attribute__((address_space(1))) float gf1;
 
 // PF: rows loaded ahead of use (odd, so the ring size stays even and the
@@ -210,8 +211,8 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
   const int nsteps = nrows - 1;
 #pragma unroll
   for (int r = 0; r < PF; ++r) {  // rows 0 .. PF-1 (nrows > PF always)
-    const float4 ev = erow[(size_t)r * (pitch / 4)];
-    const float4 gv = grow[(size_t)r * (pitch / 4)];
+    const v4f ev = erow[(size_t)r * (pitch / 4)];
+    const v4f gv = grow[(size_t)r * (pitch / 4)];
     E[r][0] = ev.x; E[r][1] = ev.y; E[r][2] = ev.z; E[r][3] = ev.w;
     G[r][0] = gv.x; G[r][1] = gv.y; G[r][2] = gv.z; G[r][3] = gv.w;
   }
@@ -224,8 +225,8 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
         if (t + PF < nrows) {  // row t+PF goes into the slot of row t+PF-U (< t-K, dead)
           const int sl = (u + PF) % U;
           const size_t off = (size_t)(t + PF) * (pitch / 4);
-          const float4 ev = erow[off];
-          const float4 gv = grow[off];
+          const v4f ev = erow[off];
+          const v4f gv = grow[off];
           E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
           G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
         }
@@ -266,9 +267,9 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
           const int sw = ((u - K + 1) % U + U) % U;
           float *dst = ob + (size_t)w * pitch;
           if (K % 4 == 0) {  // every lane is wholly inside or outside the core
-            if (lane_full) *(gf4 *)dst = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
+            if (lane_full) *(gf4 *)dst = v4f{E[sw][0], E[sw][1], E[sw][2], E[sw][3]};
           } else if (lane_full) {
-            *(gf4 *)dst = make_float4(E[sw][0], E[sw][1], E[sw][2], E[sw][3]);
+            *(gf4 *)dst = v4f{E[sw][0], E[sw][1], E[sw][2], E[sw][3]};
           } else if (lane_any) {
 #pragma unroll
             for (int j = 0; j < CPL; ++j)
