@@ -426,9 +426,9 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
   HIP_OK(hipMemcpyAsync(out, tgt, npx * C * sizeof(float), hipMemcpyDeviceToDevice, st));
 
   // --- classification and bounding box
-  const int K = d->half_sweeps ? (int)d->half_sweeps : 8;
+  const int K = d->half_sweeps ? (int)d->half_sweeps : 16;
   const int CPL = 4, WS = 64 * CPL, WV = WS - 2 * K;
-  int hb = d->tile_rows ? (int)d->tile_rows : 64;
+  int hb = d->tile_rows ? (int)d->tile_rows : 48;
   {
     // upper bound of the workspace for this call, so the pool never regrows mid-call
     const size_t pw_up = (size_t)W + 2 * PH_PAD + 4 + WV + 2 * K + 64;
