@@ -91,21 +91,13 @@ __device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
   return false;
 }
 
-// Control for one instance after all of its tiles finished a launch. Runs on
-// one whole wave.
+// Control for one instance after all of its tiles finished a launch. A holds
+// the instance's residual sums of the launch's K half-sweeps; runs on one lane.
 template <int K>
-__device__ void control(PhCtl &C, const PhInst &I, const double *partials, int lane) {
+__device__ void control(PhCtl &C, const PhInst &I, const double *A) {
   const uint32_t S = C.sweeps_done;
   const int k_run = min(K, (int)(2 * (C.sweep_limit - S)));
   const int nsw = k_run >> 1;
-  double A[K];
-#pragma unroll
-  for (int s = 0; s < K; ++s) {
-    double v = 0.0;
-    for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += partials[(size_t)t * K + s];
-    A[s] = wave_sum(v);
-  }
-  if (lane != 0) return;
   C.launches += 1;
   if (C.mode == PH_MODE_REDO) {
     C.sweeps_done = S + nsw;
@@ -117,17 +109,11 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *partials, int l
   const double *W = I.omega_tab + 2 * (size_t)S;  // omega of this launch's half-sweeps
   bool stop = false;
   if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, S, C.prev_black + A[0]);
-#pragma unroll
-  for (int j = 1; j < K / 2; ++j)
-    if (j < nsw && !stop) {
-      const double f = (1.0 - W[2 * j - 1]) * (1.0 - W[2 * j - 1]);
-      stop = finalize_sweep(C, S + j, f * A[2 * j - 1] + A[2 * j]);
-    }
-  double last_black = A[0];
-#pragma unroll
-  for (int j = 1; j <= K / 2; ++j)
-    if (j == nsw) last_black = (1.0 - W[2 * j - 1]) * (1.0 - W[2 * j - 1]) * A[2 * j - 1];
-  C.prev_black = last_black;
+  for (int j = 1; j < nsw && !stop; ++j) {
+    const double f = (1.0 - W[2 * j - 1]) * (1.0 - W[2 * j - 1]);
+    stop = finalize_sweep(C, S + j, f * A[2 * j - 1] + A[2 * j]);
+  }
+  C.prev_black = (1.0 - W[2 * nsw - 1]) * (1.0 - W[2 * nsw - 1]) * A[2 * nsw - 1];
   if (stop) {
     C.halt_base = S;
     C.halt_buf = in_buf;
@@ -286,17 +272,27 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
   }
 }
 
-// One wave per instance, after the sweep launch: the instance's residual
-// sums and stop rules (the kernel boundary orders it after every tile).
+// After the sweep launch (the kernel boundary orders it after every tile):
+// one block of K waves per instance; wave s adds the tiles' sums of
+// half-sweep s in a fixed order, then one lane runs the stop rules.
 template <int K>
-__global__ __launch_bounds__(64) void k_sor_control(const PhInst *__restrict__ insts,
-                                                    PhCtl *ctl, const double *__restrict__ partials,
-                                                    int ninst) {
+__global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict__ insts,
+                                                        PhCtl *ctl,
+                                                        const double *__restrict__ partials,
+                                                        int ninst) {
+  __shared__ double A[K];
   const int a = blockIdx.x;
   if (a >= ninst) return;
   PhCtl &C = ctl[a];
   if (C.halt != PH_RUN) return;
-  control<K>(C, insts[a], partials, threadIdx.x);
+  const PhInst &I = insts[a];
+  const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += partials[(size_t)t * K + s];
+  v = wave_sum(v);
+  if (lane == 0) A[s] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) control<K>(C, I, A);
 }
 
 #define PH_INSTANTIATE(K, PF)                                                                \
