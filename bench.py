@@ -142,6 +142,15 @@ def main():
     ms = float(np.median(times))
     mp = size * size / 1e6
     r = results[len(results) // 2]
+    # HBM traffic per sweep launch, measured by the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # passes recorded in profiles/ (null when no measurement for this kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_sor_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("kernel", "").startswith(f"k_sor<{results[0].half_sweeps},"):
+            traffic = tr["hbm_bytes_per_launch"]
     # roofline of the sweep kernel: algorithmic bytes over its event-timed duration
     sweep_ms = float(np.median([x.sweep_ms for x in results]))
     alg_bytes = BYTES_PER_UNKNOWN_SWEEP * r.unknown_sweeps
@@ -180,7 +189,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_us": round(per_launch_us, 2),
             "note": "achieved = 12 B x unknown-sweeps / sweep-kernel time; with K half-sweeps "
                     "per launch the compulsory HBM traffic is 12 B per unknown per launch",
