@@ -158,6 +158,32 @@ ph_status ph_poisson_solve_device(const ph_solve_desc *desc, const float *d_sour
                                   float *d_out_candidate, void *stream, ph_report *report,
                                   ph_error *err);
 
+/* One image of a batched solve: device buffers in the ResourceValue layout. */
+typedef struct ph_batch_item {
+  uint32_t width;
+  uint32_t height;
+  uint32_t channels;
+  uint32_t reserved;
+  const float *source; /* or guidance */
+  const float *target; /* or anchor */
+  const float *mask;
+  float *out_candidate;
+} ph_batch_item;
+
+/* Many independent op invocations solved together: every channel of every
+ * image is one instance, all instances share the sweep launches, and each
+ * instance stops on its own (so each report equals the one its image would
+ * get alone). desc supplies the controls (op, max_iterations, tolerance,
+ * lambda, omega, tuning); its width/height/channels are ignored. reports[i]
+ * (with its own history buffer) receives image i's worst-channel report.
+ * No reference counterpart: paintop dispatches nodes one at a time
+ * (paintop-core/src/executor/mod.rs:175-193); this is the batching entry
+ * point an executor-level batcher, or a caller with many masked regions,
+ * would use. */
+ph_status ph_poisson_solve_batch_device(const ph_solve_desc *desc, const ph_batch_item *items,
+                                        uint32_t n_items, void *stream, ph_report *reports,
+                                        ph_error *err);
+
 /* ---- device-side input generators (inputs stay in HBM) ------------------ */
 
 /* field.fbm@1 (crates/paintop-cpu/src/noise.rs:660-694, value formula

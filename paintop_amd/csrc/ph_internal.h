@@ -29,7 +29,10 @@ struct PhInst {
   int32_t pw, ph;   // plane width / height in cells
   int32_t ox, oy;   // image coordinates of plane cell (0, 0)
   int32_t img_w, img_h, channels, channel; // source image geometry for gather/scatter
-  const double *omega_tab; // SOR factor of half-sweep h (0-based h), h < 2 * max_it
+  double omega;        // SOR factor of half-sweeps h < omega_late_from (0-based h)
+  double omega_late;   // SOR factor of half-sweeps h >= omega_late_from
+  uint32_t omega_late_from;
+  uint32_t pad0_;
   float dg;         // 4 + lambda (FP32)
   double dg64;      // 4 + lambda (FP64)
   double lambda;
@@ -76,6 +79,11 @@ struct PhCtl {
   uint32_t last_refine;  // sweep count of the last refinement
   uint32_t pad_;
 };
+
+// SOR factor of half-sweep h (0-based, counted from the first sweep).
+__host__ __device__ inline double ph_omega_at(const PhInst &I, uint64_t h) {
+  return h < I.omega_late_from ? I.omega : I.omega_late;
+}
 
 struct PhTile {
   int32_t inst;

@@ -172,7 +172,19 @@ static Pool g_pool;
 
 static const uint32_t MAX_ITERATIONS_LIMIT = 100000;  // poisson_blend.rs:71
 
-static void validate(const ph_solve_desc *d) {
+static void validate_shape(const ph_solve_desc *d, uint32_t width, uint32_t height,
+                           uint32_t channels) {
+  const bool screened = d->op == PH_OP_SCREENED_POISSON;
+  const char *opn = screened ? "repair.screened_poisson" : "repair.poisson_blend";
+  if (channels == 0 || channels > 64)
+    throw fail(PH_CLASS_SEMANTIC, "E_POISSON_BLEND_SHAPE", "%s: channel count %u out of range",
+               opn, channels);
+  if ((size_t)width * height > ((size_t)1 << 31))
+    throw fail(PH_CLASS_POLICY, "E_HIP_LIMIT", "image of %ux%u exceeds the 2^31-pixel limit",
+               width, height);
+}
+
+static void validate(const ph_solve_desc *d, bool check_shape = true) {
   if (!d) throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null descriptor");
   const bool screened = d->op == PH_OP_SCREENED_POISSON;
   if (d->op != PH_OP_POISSON_BLEND && !screened)
@@ -190,17 +202,13 @@ static void validate(const ph_solve_desc *d) {
     throw fail(PH_CLASS_SCHEMA, pcode, "%s: lambda must be 0 for the pure Poisson blend", opn);
   if (d->omega != 0.0 && !(d->omega > 0.0 && d->omega < 2.0))
     throw fail(PH_CLASS_SCHEMA, pcode, "%s: omega must be in (0, 2)", opn);
-  if (d->channels == 0 || d->channels > 64)
-    throw fail(PH_CLASS_SEMANTIC, screened ? "E_SCREENED_POISSON_SHAPE" : "E_POISSON_BLEND_SHAPE",
-               "%s: channel count %u out of range", opn, d->channels);
+
   const uint32_t k = d->half_sweeps;
   if (k != 0 && k != 2 && k != 4 && k != 8 && k != 12 && k != 16)
     throw fail(PH_CLASS_SCHEMA, "E_HIP_TUNING", "half_sweeps must be one of 2, 4, 8, 12, 16");
   if (d->tile_rows != 0 && (d->tile_rows < 16 || d->tile_rows > 1024 || (d->tile_rows & 1)))
     throw fail(PH_CLASS_SCHEMA, "E_HIP_TUNING", "tile_rows must be 0 or even in 16..1024");
-  if ((size_t)d->width * d->height > ((size_t)1 << 31))
-    throw fail(PH_CLASS_POLICY, "E_HIP_LIMIT", "image of %ux%u exceeds the 2^31-pixel limit",
-               d->width, d->height);
+  if (check_shape) validate_shape(d, d->width, d->height, d->channels);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,34 +291,6 @@ static double coarse_min_eigenvalue(const std::vector<uint8_t> &in, int cw, int 
   return mu;
 }
 
-// mu (fine grid) of the interior cells in region [rx0, rx0+rw) x [ry0, ry0+rh).
-static double tuned_mu(const uint8_t *cls_img, int W, int H, int rx0, int ry0, int rw, int rh,
-                       hipStream_t st) {
-  const int f = std::max(1, (std::max(rw, rh) + 127) / 128);
-  const int cw = (rw + f - 1) / f, ch = (rh + f - 1) / f;
-  int *counts_d = g_pool.take<int>((size_t)cw * ch);
-  hipLaunchKernelGGL(k_coarse_count, dim3((cw * ch + 255) / 256), dim3(256), 0, st, cls_img, W, H,
-                     rx0, ry0, f, cw, ch, counts_d);
-  HIP_OK(hipGetLastError());
-  std::vector<int> counts((size_t)cw * ch);
-  HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
-                        hipMemcpyDeviceToHost, st));
-  HIP_OK(hipStreamSynchronize(st));
-  std::vector<uint8_t> in(counts.size());
-  uint64_t hsh = 1469598103934665603ull;  // FNV-1a of the coarse mask
-  for (size_t i = 0; i < counts.size(); ++i) {
-    in[i] = 2 * counts[i] >= f * f ? 1 : 0;
-    hsh = (hsh ^ in[i]) * 1099511628211ull;
-  }
-  const TuneKey key{cw, ch, f, hsh};
-  for (const auto &e : g_tune_cache)
-    if (e.first == key) return e.second;
-  const double mu = coarse_min_eigenvalue(in, cw, ch) / ((double)f * f);
-  if (g_tune_cache.size() > 256) g_tune_cache.erase(g_tune_cache.begin());
-  g_tune_cache.push_back({key, mu});
-  return mu;
-}
-
 static double omega_for(double mu, double lambda) {
   const double rho = (4.0 - std::min(mu, 4.0)) / (4.0 + lambda);
   const double w = 2.0 / (1.0 + sqrt(std::max(1.0 - rho * rho, 0.0)));
@@ -381,7 +361,6 @@ struct ChanResult {
   bool active = false;
   bool done = true;
   bool nonfinite = false;
-  uint64_t interior = 0;
 };
 
 // Refinement triggers: the inner residual fell 1e3-fold, or REFINE_EVERY
@@ -391,270 +370,371 @@ struct ChanResult {
 // absorbing the late, small updates.
 static const double REFINE_RATIO = 1e-3;
 static const uint32_t REFINE_EVERY = 1024;
+// Below this many interior cells the SOR factor comes from the geometric
+// bounds instead of the coarse-mask eigenvalue (small domains converge in a
+// few hundred sweeps; the host-side eigen solve would cost more than it saves).
+static const double PRECISE_TUNE_MIN_CELLS = 32768.0;
 
-static void solve_device(const ph_solve_desc *d, const float *src, const float *tgt,
-                         const float *mask, float *out, hipStream_t st, ph_report *rep) {
-  const int W = (int)d->width, H = (int)d->height, C = (int)d->channels;
+// One image of a solve (one op call).
+struct ItemIn {
+  int W, H, C;
+  const float *src, *tgt, *mask;
+  float *out;
+  ph_report *rep;
+};
+
+struct ItemWork {
+  ItemIn in;
+  uint8_t *cls_img = nullptr;
+  bool any = false;
+  int rx0 = 0, ry0 = 0, rw = 0, rh = 0, ox = 0, oy = 0, pw = 0, phh = 0, pitch = 0;
+  std::vector<int2> origins;
+  size_t origin_off = 0;
+  std::vector<int2> live;  // tile origins whose core holds interior cells
+  double n_interior = 0;
+  int cf = 1, cw = 0, ch = 0;
+  size_t coarse_off = 0;
+  bool precise = false;
+  double omega = 1.0;
+  uint32_t stall_window = PH_STALL_WINDOW;
+  uint8_t *cls_plane = nullptr;
+  int inst0 = -1;
+};
+
+struct InstWork {
+  int item = 0, channel = 0;
+  PhInst I{};
+  double r0 = 0.0;
+  ChanResult R;
+  double *hist = nullptr;
+  double *rowsum = nullptr;
+  dim3 pgrd;
+};
+
+// Geometric bounds on mu (the smallest eigenvalue of the 5-point operator on
+// the interior): the equal-area disc (Faber-Krahn, mu >= (j01/R)^2) and the
+// bounding rectangle (domain monotonicity); the larger bound is used.
+static double mu_bound(double n_interior, int bw, int bh) {
+  const double j01 = 2.404825557695773;
+  const double R = sqrt(std::max(n_interior, 1.0) / M_PI);
+  const double mu_disc = (j01 / R) * (j01 / R);
+  const double mu_rect = (M_PI / std::max(bw, 1)) * (M_PI / std::max(bw, 1)) +
+                         (M_PI / std::max(bh, 1)) * (M_PI / std::max(bh, 1));
+  return std::min(std::max(mu_disc, mu_rect), 4.0);
+}
+
+// mu from a coarse mask of interior counts (cached per coarse mask).
+static double mu_from_counts(const int *counts, int f, int cw, int ch) {
+  std::vector<uint8_t> in((size_t)cw * ch);
+  uint64_t hsh = 1469598103934665603ull;  // FNV-1a of the coarse mask
+  for (size_t i = 0; i < in.size(); ++i) {
+    in[i] = 2 * counts[i] >= f * f ? 1 : 0;
+    hsh = (hsh ^ in[i]) * 1099511628211ull;
+  }
+  const TuneKey key{cw, ch, f, hsh};
+  for (const auto &e : g_tune_cache)
+    if (e.first == key) return e.second;
+  const double mu = coarse_min_eigenvalue(in, cw, ch) / ((double)f * f);
+  if (g_tune_cache.size() > 4096) g_tune_cache.erase(g_tune_cache.begin());
+  g_tune_cache.push_back({key, mu});
+  return mu;
+}
+
+// Upper bound of one image's workspace, so the pool never regrows mid-call.
+static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int hb) {
+  const int WV = 256 - 2 * K;
   const size_t npx = (size_t)W * H;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const size_t pw_up = (size_t)W + 2 * PH_PAD + 4 + WV + 2 * K + 64;
+  const size_t ph_up = (size_t)H + 2 * PH_PAD + 4 + hb + 2 * K;
+  const size_t plane_up = pw_up * ph_up;
+  const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
+  return npx + 8 * 1024 + ntile_up * (sizeof(int2) + sizeof(int)) +
+         plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
+         (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
+         (size_t)C * ph_up * sizeof(double) +
+         ntile_up * (size_t)C * (sizeof(PhTile) + K * sizeof(double)) +
+         (size_t)C * (sizeof(PhInst) + sizeof(PhCtl) + 256) + 40 * 256 +
+         130 * 130 * sizeof(int);
+}
+
+static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipStream_t st) {
+  const int nitem = (int)items.size();
+  const int K = d->half_sweeps ? (int)d->half_sweeps : 16;
+  const int WV = 256 - 2 * K;
+  const int hb = d->tile_rows ? (int)d->tile_rows : 48;
   const bool timing = (d->flags & PH_FLAG_TIMING) != 0;
+  const bool tuned = d->omega <= 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (timing) {
     HIP_OK(hipEventCreate(&ev0));
     HIP_OK(hipEventCreate(&ev1));
     HIP_OK(hipEventRecord(ev0, st));
   }
-  std::vector<ChanResult> res(C);
-  rep->launches = 0;
-  rep->refinements = 0;
-  rep->unknowns = 0;
-  rep->unknown_sweeps = 0;
-  rep->warm_start_used = 0;
-  rep->warm_start_rel = 0.0;
-  rep->sweep_ms = 0.0;
-
-  if (npx == 0) {
-    rep->iterations = 0;
-    rep->stop_reason = PH_STOP_CONVERGED;
-    rep->converged = 1;
-    rep->history_len = 0;
-    rep->tolerance = d->tolerance;
-    rep->final_residual = 0.0;
-    rep->worst_channel = 0;
-    rep->device_ms = 0.0;
-    return;
-  }
-  // the candidate starts as the target: every non-interior sample keeps its bits
-  HIP_OK(hipMemcpyAsync(out, tgt, npx * C * sizeof(float), hipMemcpyDeviceToDevice, st));
-
-  // --- classification and bounding box
-  const int K = d->half_sweeps ? (int)d->half_sweeps : 16;
-  const int CPL = 4, WS = 64 * CPL, WV = WS - 2 * K;
-  int hb = d->tile_rows ? (int)d->tile_rows : 48;
   {
-    // upper bound of the workspace for this call, so the pool never regrows mid-call
-    const size_t pw_up = (size_t)W + 2 * PH_PAD + 4 + WV + 2 * K + 64;
-    const size_t ph_up = (size_t)H + 2 * PH_PAD + 4 + hb + 2 * K;
-    const size_t plane_up = pw_up * ph_up;
-    const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
-    const size_t need = npx + 16 * 1024 + ntile_up * (sizeof(int2) + sizeof(int)) +
-                        plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
-                        (size_t)(C + 2) * (d->max_iterations + 1) * sizeof(double) +
-                        (size_t)C * ph_up * sizeof(double) +
-                        ntile_up * (size_t)C * (sizeof(PhTile) + K * sizeof(double)) +
-                        (size_t)C * (sizeof(PhInst) + sizeof(PhCtl) + 64) + 64 * 256 +
-                        130 * 130 * sizeof(int);
+    size_t need = 64 * 1024 + (size_t)nitem * 64;
+    for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
     g_pool.reset(need);
   }
-  uint8_t *cls_img = g_pool.take<uint8_t>(npx);
-  int *bbox = g_pool.take<int>(4);
-  int bbox_h[4] = {0x7fffffff, 0x7fffffff, -1, -1};
-  HIP_OK(hipMemcpyAsync(bbox, bbox_h, sizeof(bbox_h), hipMemcpyHostToDevice, st));
-  {
-    dim3 blk(256), grd((W + 255) / 256, H);
-    hipLaunchKernelGGL(k_classify, grd, blk, 0, st, mask, W, H, cls_img, bbox);
+  std::vector<ItemWork> iw(nitem);
+  for (int i = 0; i < nitem; ++i) {
+    iw[i].in = items[i];
+    ph_report *rep = items[i].rep;
+    rep->launches = 0;
+    rep->refinements = 0;
+    rep->unknowns = 0;
+    rep->unknown_sweeps = 0;
+    rep->warm_start_used = 0;
+    rep->warm_start_rel = 0.0;
+    rep->sweep_ms = 0.0;
+    rep->device_ms = 0.0;
+    rep->half_sweeps_used = (uint32_t)K;
+    rep->omega_used = d->omega;
+  }
+
+  // --- 1. classification and bounding boxes (one readback for all images)
+  int *bbox_d = g_pool.take<int>(4 * (size_t)nitem);
+  std::vector<int> bbox_h(4 * (size_t)nitem);
+  for (int i = 0; i < nitem; ++i) {
+    bbox_h[4 * i + 0] = bbox_h[4 * i + 1] = 0x7fffffff;
+    bbox_h[4 * i + 2] = bbox_h[4 * i + 3] = -1;
+  }
+  HIP_OK(hipMemcpyAsync(bbox_d, bbox_h.data(), bbox_h.size() * sizeof(int), hipMemcpyHostToDevice,
+                        st));
+  for (int i = 0; i < nitem; ++i) {
+    const ItemIn &it = iw[i].in;
+    const size_t npx = (size_t)it.W * it.H;
+    if (npx == 0) continue;
+    // the candidate starts as the target: every non-interior sample keeps its bits
+    HIP_OK(hipMemcpyAsync(it.out, it.tgt, npx * it.C * sizeof(float), hipMemcpyDeviceToDevice,
+                          st));
+    iw[i].cls_img = g_pool.take<uint8_t>(npx);
+    hipLaunchKernelGGL(k_classify, dim3((it.W + 255) / 256, it.H), dim3(256), 0, st, it.mask,
+                       it.W, it.H, iw[i].cls_img, bbox_d + 4 * i);
     HIP_OK(hipGetLastError());
   }
-  HIP_OK(hipMemcpyAsync(bbox_h, bbox, sizeof(bbox_h), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(bbox_h.data(), bbox_d, bbox_h.size() * sizeof(int), hipMemcpyDeviceToHost,
+                        st));
   HIP_OK(hipStreamSynchronize(st));
 
-  const bool any_interior = bbox_h[2] >= 0;
-  std::vector<PhInst> inst_h;
-  std::vector<int> inst_chan;
-  std::vector<PhTile> tiles_h;
-  std::vector<int2> chan_origins;
-  double n_interior = 0;
-  int region_w = 0, region_h = 0;
-  int ox = 0, oy = 0, pw = 0, phh = 0, pitch = 0;
-
-  if (any_interior) {
-    int rx0 = bbox_h[0] - 1, ry0 = bbox_h[1] - 1;
-    const int rx1 = bbox_h[2] + 2, ry1 = bbox_h[3] + 2;
-    if ((rx0 + ry0) & 1) rx0 -= 1;  // red/black colour of plane cells = image colour
-    region_w = rx1 - rx0;
-    region_h = ry1 - ry0;
-    ox = rx0 - PH_PAD;
-    oy = ry0 - PH_PAD;
-    const int nstrips = (PH_PAD + region_w - K + WV - 1) / WV;
-    const int nblocks = (PH_PAD + region_h - K + hb - 1) / hb;
-    pw = nstrips * WV + 2 * K;
-    phh = nblocks * hb + 2 * K;
-    pitch = (pw + 63) & ~63;
-    // tile occupancy
-    std::vector<int2> origins;
+  // --- 2. planes, tile occupancy, coarse masks for the tuner (one readback)
+  std::vector<int2> all_origins;
+  size_t coarse_total = 0;
+  for (int i = 0; i < nitem; ++i) {
+    ItemWork &w = iw[i];
+    if (bbox_h[4 * i + 2] < 0) continue;
+    w.any = true;
+    w.rx0 = bbox_h[4 * i] - 1;
+    w.ry0 = bbox_h[4 * i + 1] - 1;
+    const int rx1 = bbox_h[4 * i + 2] + 2, ry1 = bbox_h[4 * i + 3] + 2;
+    if ((w.rx0 + w.ry0) & 1) w.rx0 -= 1;  // red/black colour of plane cells = image colour
+    w.rw = rx1 - w.rx0;
+    w.rh = ry1 - w.ry0;
+    w.ox = w.rx0 - PH_PAD;
+    w.oy = w.ry0 - PH_PAD;
+    const int nstrips = (PH_PAD + w.rw - K + WV - 1) / WV;
+    const int nblocks = (PH_PAD + w.rh - K + hb - 1) / hb;
+    w.pw = nstrips * WV + 2 * K;
+    w.phh = nblocks * hb + 2 * K;
+    w.pitch = (w.pw + 63) & ~63;
+    w.origin_off = all_origins.size();
     for (int m = 0; m < nblocks; ++m)
-      for (int n = 0; n < nstrips; ++n) origins.push_back(make_int2(n * WV, m * hb));
-    int2 *origins_d = g_pool.take<int2>(origins.size());
-    int *counts_d = g_pool.take<int>(origins.size());
-    HIP_OK(hipMemcpyAsync(origins_d, origins.data(), origins.size() * sizeof(int2),
+      for (int n = 0; n < nstrips; ++n) all_origins.push_back(make_int2(n * WV, m * hb));
+    w.origins.assign(all_origins.begin() + w.origin_off, all_origins.end());
+    if (tuned) {
+      w.cf = std::max(1, (std::max(w.rw, w.rh) + 127) / 128);
+      w.cw = (w.rw + w.cf - 1) / w.cf;
+      w.ch = (w.rh + w.cf - 1) / w.cf;
+      w.coarse_off = coarse_total;
+      coarse_total += (size_t)w.cw * w.ch;
+    }
+  }
+  std::vector<int> counts(all_origins.size()), coarse(coarse_total);
+  if (!all_origins.empty()) {
+    int2 *origins_d = g_pool.take<int2>(all_origins.size());
+    int *counts_d = g_pool.take<int>(all_origins.size());
+    int *coarse_d = coarse_total ? g_pool.take<int>(coarse_total) : nullptr;
+    HIP_OK(hipMemcpyAsync(origins_d, all_origins.data(), all_origins.size() * sizeof(int2),
                           hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_tile_occupancy, dim3(origins.size()), dim3(256), 0, st, cls_img, W, H,
-                       ox, oy, K, WV, hb, origins_d, counts_d);
-    HIP_OK(hipGetLastError());
-    std::vector<int> counts(origins.size());
+    for (int i = 0; i < nitem; ++i) {
+      ItemWork &w = iw[i];
+      if (!w.any) continue;
+      hipLaunchKernelGGL(k_tile_occupancy, dim3(w.origins.size()), dim3(256), 0, st, w.cls_img,
+                         w.in.W, w.in.H, w.ox, w.oy, K, WV, hb, origins_d + w.origin_off,
+                         counts_d + w.origin_off);
+      HIP_OK(hipGetLastError());
+      if (tuned) {
+        hipLaunchKernelGGL(k_coarse_count, dim3((w.cw * w.ch + 255) / 256), dim3(256), 0, st,
+                           w.cls_img, w.in.W, w.in.H, w.rx0, w.ry0, w.cf, w.cw, w.ch,
+                           coarse_d + w.coarse_off);
+        HIP_OK(hipGetLastError());
+      }
+    }
     HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
                           hipMemcpyDeviceToHost, st));
+    if (coarse_total)
+      HIP_OK(hipMemcpyAsync(coarse.data(), coarse_d, coarse.size() * sizeof(int),
+                            hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
-    for (size_t i = 0; i < origins.size(); ++i)
-      if (counts[i] > 0) {
-        chan_origins.push_back(origins[i]);
-        n_interior += counts[i];
+  }
+
+  // --- 3. live tiles, SOR factor and stall window per image
+  for (int i = 0; i < nitem; ++i) {
+    ItemWork &w = iw[i];
+    if (!w.any) continue;
+    for (size_t k = 0; k < w.origins.size(); ++k)
+      if (counts[w.origin_off + k] > 0) {
+        w.live.push_back(w.origins[k]);
+        w.n_interior += counts[w.origin_off + k];
       }
+    w.omega = d->omega;
+    if (tuned) {
+      const double mu = w.n_interior >= PRECISE_TUNE_MIN_CELLS
+                            ? mu_from_counts(coarse.data() + w.coarse_off, w.cf, w.cw, w.ch)
+                            : mu_bound(w.n_interior, w.rw, w.rh);
+      w.omega = omega_for(mu, d->lambda);
+    }
+    w.stall_window = stall_window_for(w.omega, tuned);
+    w.in.rep->omega_used = w.omega;
   }
 
-  const bool tuned = d->omega <= 0.0;
-  double omega = d->omega;
-  if (tuned) {
-    const double mu = any_interior ? tuned_mu(cls_img, W, H, ox + PH_PAD, oy + PH_PAD, region_w,
-                                              region_h, st)
-                                   : 4.0;
-    omega = omega_for(mu, d->lambda);
-  }
-  rep->omega_used = omega;
-  rep->half_sweeps_used = (uint32_t)K;
-  const uint32_t stall_window = stall_window_for(omega, tuned);
-  // SOR factor of every half-sweep (one value for now; the table lets a
-  // schedule vary it per half-sweep)
-  const size_t ntab = 2 * (size_t)d->max_iterations + 2;
-  std::vector<double> wtab(ntab, omega);
-  double *omega_d = g_pool.take<double>(ntab);
-  HIP_OK(hipMemcpyAsync(omega_d, wtab.data(), ntab * sizeof(double), hipMemcpyHostToDevice, st));
-
-  // --- per-channel planes and r0
-  std::vector<double> r0(C, 0.0);
-  uint8_t *cls_plane = nullptr;
-  double *res_d = nullptr;
-  uint32_t *arrival_d = nullptr;
-  double *rowsum_d = nullptr;
-  std::vector<double *> hist_d(C, nullptr);
-  dim3 pblk(256), pgrd(1, 1);
-  if (any_interior) {
-    const size_t plane = (size_t)pitch * phh;
-    cls_plane = g_pool.take<uint8_t>(plane);
-    res_d = g_pool.take<double>(2 * (size_t)C);
-    arrival_d = g_pool.take<uint32_t>(C + 1);
-    rowsum_d = g_pool.take<double>((size_t)phh * C);
-    HIP_OK(hipMemsetAsync(arrival_d, 0, (C + 1) * sizeof(uint32_t), st));
-    pgrd = dim3((pitch + 255) / 256, phh);
-    for (int c = 0; c < C; ++c) {
-      PhInst I{};
+  // --- 4. per-channel planes, assembly and r0 (one readback)
+  std::vector<InstWork> inst;
+  for (int i = 0; i < nitem; ++i) {
+    ItemWork &w = iw[i];
+    if (!w.any) continue;
+    const size_t plane = (size_t)w.pitch * w.phh;
+    w.cls_plane = g_pool.take<uint8_t>(plane);
+    w.inst0 = (int)inst.size();
+    for (int c = 0; c < w.in.C; ++c) {
+      InstWork X;
+      X.item = i;
+      X.channel = c;
+      PhInst &I = X.I;
       I.e[0] = g_pool.take<float>(plane);
       I.e[1] = g_pool.take<float>(plane);
       I.g = g_pool.take<float>(plane);
       I.x = g_pool.take<double>(plane);
       I.bp = g_pool.take<double>(plane);
-      I.cls = cls_plane;
-      I.pitch = pitch;
-      I.pw = pw;
-      I.ph = phh;
-      I.ox = ox;
-      I.oy = oy;
-      I.img_w = W;
-      I.img_h = H;
-      I.channels = C;
+      I.cls = w.cls_plane;
+      I.pitch = w.pitch;
+      I.pw = w.pw;
+      I.ph = w.phh;
+      I.ox = w.ox;
+      I.oy = w.oy;
+      I.img_w = w.in.W;
+      I.img_h = w.in.H;
+      I.channels = w.in.C;
       I.channel = c;
       I.dg64 = 4.0 + d->lambda;
       I.dg = (float)I.dg64;
-      I.omega_tab = omega_d;
+      I.omega = w.omega;
+      I.omega_late = w.omega;
+      I.omega_late_from = 0xffffffffu;
       I.lambda = d->lambda;
-      hipLaunchKernelGGL(k_assemble, pgrd, pblk, 0, st, I, src, tgt, cls_img, cls_plane);
+      X.pgrd = dim3((w.pitch + 255) / 256, w.phh);
+      X.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
+      X.rowsum = g_pool.take<double>(w.phh);
+      hipLaunchKernelGGL(k_assemble, X.pgrd, dim3(256), 0, st, I, w.in.src, w.in.tgt, w.cls_img,
+                         w.cls_plane);
       HIP_OK(hipGetLastError());
-      hist_d[c] = g_pool.take<double>(std::max(d->max_iterations, 1u));
-      inst_h.push_back(I);
-      inst_chan.push_back(c);
+      inst.push_back(X);
     }
-    for (int c = 0; c < C; ++c) {
-      hipLaunchKernelGGL(k_residual, dim3(phh), dim3(256), 0, st, inst_h[c],
-                         rowsum_d + (size_t)c * phh, arrival_d + c, res_d + 2 * c);
-      HIP_OK(hipGetLastError());
-    }
-    std::vector<double> rr(2 * C);
-    HIP_OK(hipMemcpyAsync(rr.data(), res_d, rr.size() * sizeof(double), hipMemcpyDeviceToHost,
+  }
+  const int ninst = (int)inst.size();
+  double *resv_d = g_pool.take<double>(2 * (size_t)std::max(ninst, 1));
+  uint32_t *arr_d = g_pool.take<uint32_t>((size_t)std::max(ninst, 1));
+  HIP_OK(hipMemsetAsync(arr_d, 0, std::max(ninst, 1) * sizeof(uint32_t), st));
+  std::vector<double> resv(2 * (size_t)std::max(ninst, 1));
+  auto launch_residual = [&](int k) {
+    const InstWork &X = inst[k];
+    hipLaunchKernelGGL(k_residual, dim3(X.I.ph), dim3(256), 0, st, X.I, X.rowsum, arr_d + k,
+                       resv_d + 2 * k);
+    HIP_OK(hipGetLastError());
+  };
+  for (int k = 0; k < ninst; ++k) launch_residual(k);
+  if (ninst) {
+    HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
-    for (int c = 0; c < C; ++c) r0[c] = rr[2 * c];
   }
 
-  // --- instances that need sweeps
-  std::vector<int> act;  // channel ids
-  for (int c = 0; c < C; ++c) {
-    res[c] = ChanResult{};
-    res[c].interior = (uint64_t)n_interior;
-    if (!any_interior || r0[c] == 0.0) continue;  // poisson.rs:177-189
-    if (!std::isfinite(r0[c])) {
+  // --- 5. instances that need sweeps
+  std::vector<int> act;
+  for (int k = 0; k < ninst; ++k) {
+    InstWork &X = inst[k];
+    X.r0 = resv[2 * k];
+    X.R = ChanResult{};
+    if (X.r0 == 0.0) continue;  // poisson.rs:177-189
+    if (!std::isfinite(X.r0)) {
       // Non-finite inputs: the reference's residuals are NaN, no sweep ever
       // improves on the best, and the stall guard ends it after 16 sweeps.
-      res[c].nonfinite = true;
-      res[c].iterations = std::min(PH_STALL_WINDOW, d->max_iterations);
-      res[c].stop = res[c].iterations == PH_STALL_WINDOW ? PH_STOP_STALLED : PH_STOP_MAX_ITERATIONS;
-      res[c].converged = false;
-      res[c].final_residual = NAN;
+      X.R.nonfinite = true;
+      X.R.iterations = std::min(PH_STALL_WINDOW, d->max_iterations);
+      X.R.stop = X.R.iterations == PH_STALL_WINDOW ? PH_STOP_STALLED : PH_STOP_MAX_ITERATIONS;
+      X.R.converged = false;
+      X.R.final_residual = NAN;
       continue;
     }
-    res[c].active = true;
-    res[c].done = false;
-    act.push_back(c);
+    X.R.active = true;
+    X.R.done = false;
+    act.push_back(k);
+    iw[X.item].in.rep->unknowns += (uint64_t)iw[X.item].n_interior;
   }
-  rep->unknowns = (uint64_t)n_interior * act.size();
 
-  if (!act.empty()) {
-    const int nact = (int)act.size();
+  const int nact = (int)act.size();
+  if (nact > 0) {
+    // the tile list of the running instances (rebuilt as instances finish)
     std::vector<PhInst> ai(nact);
-    std::vector<PhTile> tl;
-    for (int a = 0; a < nact; ++a) {
-      ai[a] = inst_h[act[a]];
-      ai[a].tile_begin = (int)tl.size();
-      for (const int2 &o : chan_origins) tl.push_back(PhTile{a, o.x, o.y, 0});
-      ai[a].tile_end = (int)tl.size();
-    }
-    const int ntiles = (int)tl.size();
-    PhTile *tiles_d = g_pool.take<PhTile>(ntiles);
+    for (int a = 0; a < nact; ++a) ai[a] = inst[act[a]].I;
+    size_t ntile_cap = 0;
+    for (int a = 0; a < nact; ++a) ntile_cap += iw[inst[act[a]].item].live.size();
+    PhTile *tiles_d = g_pool.take<PhTile>(ntile_cap);
     PhInst *inst_d = g_pool.take<PhInst>(nact);
     PhCtl *ctl_d = g_pool.take<PhCtl>(nact);
-    double *partials_d = g_pool.take<double>((size_t)ntiles * K);
-    HIP_OK(hipMemcpyAsync(tiles_d, tl.data(), ntiles * sizeof(PhTile), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(inst_d, ai.data(), nact * sizeof(PhInst), hipMemcpyHostToDevice, st));
+    double *partials_d = g_pool.take<double>(ntile_cap * K);
+    std::vector<PhTile> tl;
+    int ntiles = 0;
+    auto build_tiles = [&](const std::vector<char> &running) {
+      tl.clear();
+      for (int a = 0; a < nact; ++a) {
+        ai[a].tile_begin = (int)tl.size();
+        if (running[a])
+          for (const int2 &o : iw[inst[act[a]].item].live) tl.push_back(PhTile{a, o.x, o.y, 0});
+        ai[a].tile_end = (int)tl.size();
+      }
+      ntiles = (int)tl.size();
+      if (ntiles)
+        HIP_OK(hipMemcpyAsync(tiles_d, tl.data(), ntiles * sizeof(PhTile), hipMemcpyHostToDevice,
+                              st));
+      HIP_OK(hipMemcpyAsync(inst_d, ai.data(), nact * sizeof(PhInst), hipMemcpyHostToDevice, st));
+      HIP_OK(hipStreamSynchronize(st));  // tl and ai are host staging
+    };
+    std::vector<char> running(nact, 1);
+    build_tiles(running);
+
     std::vector<PhCtl> ctl(nact);
     for (int a = 0; a < nact; ++a) {
+      const InstWork &X = inst[act[a]];
+      const ItemWork &w = iw[X.item];
       PhCtl &Cc = ctl[a];
       memset(&Cc, 0, sizeof(Cc));
-      const int c = act[a];
-      Cc.r0 = r0[c];
+      Cc.r0 = X.r0;
       Cc.tol = d->tolerance;
-      Cc.refine_abs2 = (REFINE_RATIO * r0[c]) * (REFINE_RATIO * r0[c]);
+      Cc.refine_abs2 = (REFINE_RATIO * X.r0) * (REFINE_RATIO * X.r0);
       Cc.best = INFINITY;
-      Cc.history = hist_d[c];
+      Cc.history = X.hist;
       Cc.max_it = d->max_iterations;
       Cc.sweep_limit = d->max_iterations;
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
-      Cc.stall_window = stall_window;
+      Cc.stall_window = w.stall_window;
       Cc.refine_every = REFINE_EVERY;
       Cc.last_refine = 0;
     }
     HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
 
-    std::vector<double> rr(2);
-    auto refine = [&](int a) -> double {
-      PhInst &I = ai[a];
-      hipLaunchKernelGGL(k_apply_correction, pgrd, pblk, 0, st, I, (int)ctl[a].cur_buf);
-      HIP_OK(hipGetLastError());
-      const int c = act[a];
-      hipLaunchKernelGGL(k_residual, dim3(phh), dim3(256), 0, st, I, rowsum_d + (size_t)c * phh,
-                         arrival_d + c, res_d + 2 * c);
-      HIP_OK(hipGetLastError());
-      HIP_OK(hipMemcpyAsync(rr.data(), res_d + 2 * c, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                            st));
-      HIP_OK(hipStreamSynchronize(st));
-      rep->refinements += 1;
-      return rr[0];
-    };
-    auto put_hist = [&](int a, uint32_t m, double rel) {
-      HIP_OK(hipMemcpyAsync(ctl[a].history + (m - 1), &rel, sizeof(double),
-                            hipMemcpyHostToDevice, st));
-      HIP_OK(hipStreamSynchronize(st));
-    };
     auto finish = [&](int a, int stop, uint32_t m, double rel) {
-      ChanResult &R = res[act[a]];
+      ChanResult &R = inst[act[a]].R;
       R.done = true;
       R.iterations = m;
       R.stop = stop;
@@ -681,20 +761,6 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       Cc.refine_pending = 0;
       Cc.refine_abs2 = (REFINE_RATIO * gnorm) * (REFINE_RATIO * gnorm);
     };
-    // an exact FP64 evaluation of sweep m (state in e[cur_buf]) and the reference's rules
-    auto settle = [&](int a, uint32_t m, int device_reason) {
-      PhCtl &Cc = ctl[a];
-      const double gnorm = refine(a);
-      const double rel = gnorm / Cc.r0;
-      put_hist(a, m, rel);
-      Cc.sweeps_final = m;
-      Cc.sweeps_done = m;
-      if (rel <= Cc.tol) return finish(a, PH_STOP_CONVERGED, m, rel);
-      if (device_reason == PH_REASON_STALLED) return finish(a, PH_STOP_STALLED, m, rel);
-      if (stall_step(Cc, rel)) return finish(a, PH_STOP_STALLED, m, rel);
-      if (m >= Cc.max_it) return finish(a, PH_STOP_MAX_ITERATIONS, m, rel);
-      resume(a, gnorm);
-    };
 
     const int PF = prefetch_rows(K);
     int batch = 4;
@@ -707,11 +773,20 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       HIP_OK(hipEventCreate(&eb0));
       HIP_OK(hipEventCreate(&eb1));
     }
+    struct Pending {
+      int a;
+      uint32_t m;
+      int reason;
+    };
+    std::vector<Pending> pend;
+    std::vector<double> hist_stage;
+    uint32_t total_launches = 0, total_refine = 0;
+    double sweep_ms = 0.0;
     while (remaining > 0) {
       if (timing) HIP_OK(hipEventRecord(eb0, st));
       for (int i = 0; i < batch; ++i) {
-        launch_sor(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb);
-        rep->launches += 1;
+        if (ntiles > 0) launch_sor(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb);
+        total_launches += 1;
       }
       if (timing) HIP_OK(hipEventRecord(eb1, st));
       HIP_OK(hipMemcpyAsync(ctl.data(), ctl_d, nact * sizeof(PhCtl), hipMemcpyDeviceToHost, st));
@@ -719,18 +794,19 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       if (timing) {
         float bms = 0.f;
         HIP_OK(hipEventElapsedTime(&bms, eb0, eb1));
-        rep->sweep_ms += bms;
+        sweep_ms += bms;
       }
-      bool any_halt = false;
+      // pass 1: every instance that needs an FP64 evaluation gets its
+      // refinement kernels; one readback serves all of them
+      pend.clear();
       for (int a = 0; a < nact; ++a) {
         PhCtl &Cc = ctl[a];
-        if (res[act[a]].done) continue;
+        if (inst[act[a]].R.done) continue;
         switch (Cc.halt) {
           case PH_RUN:
             break;
           case PH_HALT_STOP:
-            any_halt = true;
-            if (Cc.halt_sweep > Cc.halt_base) {
+            if (Cc.halt_sweep > Cc.halt_base) {  // re-run up to the stop sweep
               Cc.mode = PH_MODE_REDO;
               Cc.sweeps_done = Cc.halt_base;
               Cc.sweep_limit = Cc.halt_sweep;
@@ -738,34 +814,71 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
               Cc.halt = PH_RUN;
             } else {
               Cc.cur_buf = Cc.halt_buf;
-              settle(a, Cc.halt_sweep, (int)Cc.reason);
+              pend.push_back({a, Cc.halt_sweep, (int)Cc.reason});
             }
             break;
           case PH_HALT_REDO_DONE:
-            any_halt = true;
-            settle(a, Cc.halt_sweep, (int)Cc.reason);
+            pend.push_back({a, Cc.halt_sweep, (int)Cc.reason});
             break;
           case PH_HALT_LIMIT:
-            any_halt = true;
-            settle(a, Cc.sweeps_done, -1);
-            break;
           case PH_HALT_REFINE:
-            any_halt = true;
-            settle(a, Cc.sweeps_done, -1);
+            pend.push_back({a, Cc.sweeps_done, -1});
             break;
           default:
             throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "unexpected halt state %u", Cc.halt);
         }
-        if (res[act[a]].done) --remaining;
+      }
+      if (!pend.empty()) {
+        for (const Pending &P : pend) {
+          const int k = act[P.a];
+          hipLaunchKernelGGL(k_apply_correction, inst[k].pgrd, dim3(256), 0, st, inst[k].I,
+                             (int)ctl[P.a].cur_buf);
+          HIP_OK(hipGetLastError());
+          launch_residual(k);
+        }
+        HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        total_refine += (uint32_t)pend.size();
+        // pass 2: the reference's rules on the exact FP64 residual of sweep m
+        hist_stage.assign(pend.size(), 0.0);
+        for (size_t q = 0; q < pend.size(); ++q) {
+          const Pending &P = pend[q];
+          PhCtl &Cc = ctl[P.a];
+          const double gnorm = resv[2 * act[P.a]];
+          const double rel = gnorm / Cc.r0;
+          hist_stage[q] = rel;
+          HIP_OK(hipMemcpyAsync(Cc.history + (P.m - 1), &hist_stage[q], sizeof(double),
+                                hipMemcpyHostToDevice, st));
+          Cc.sweeps_final = P.m;
+          Cc.sweeps_done = P.m;
+          if (rel <= Cc.tol) finish(P.a, PH_STOP_CONVERGED, P.m, rel);
+          else if (P.reason == PH_REASON_STALLED) finish(P.a, PH_STOP_STALLED, P.m, rel);
+          else if (stall_step(Cc, rel)) finish(P.a, PH_STOP_STALLED, P.m, rel);
+          else if (P.m >= Cc.max_it) finish(P.a, PH_STOP_MAX_ITERATIONS, P.m, rel);
+          else resume(P.a, gnorm);
+          if (inst[act[P.a]].R.done) --remaining;
+        }
       }
       HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
+      if (remaining == 0) break;
+      // drop the tiles of finished instances once they are most of the list
+      {
+        size_t dead = 0;
+        for (int a = 0; a < nact; ++a)
+          if (running[a] && inst[act[a]].R.done) dead += ai[a].tile_end - ai[a].tile_begin;
+        if (dead > 0 && 2 * dead >= (size_t)ntiles) {
+          for (int a = 0; a < nact; ++a) running[a] = !inst[act[a]].R.done;
+          build_tiles(running);
+        }
+      }
       // next batch: up to the next scheduled halt (refinement or sweep limit) of
       // the running instances, so no launch finds every instance halted
       {
         uint32_t need = 0;
         for (int a = 0; a < nact; ++a) {
           const PhCtl &Cc = ctl[a];
-          if (res[act[a]].done || Cc.halt != PH_RUN) continue;
+          if (inst[act[a]].R.done || Cc.halt != PH_RUN) continue;
           uint32_t next = std::min(Cc.sweep_limit, Cc.last_refine + Cc.refine_every);
           if (Cc.mode == PH_MODE_REDO) next = Cc.sweep_limit;
           const uint32_t todo = next > Cc.sweeps_done ? next - Cc.sweeps_done : 1;
@@ -776,7 +889,8 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       // every batch must move some instance forward (sweeps or halt state)
       uint64_t progress = 0;
       for (int a = 0; a < nact; ++a)
-        progress += (uint64_t)ctl[a].sweeps_done * 8u + ctl[a].halt + ctl[a].launches * 131u;
+        progress += (uint64_t)ctl[a].sweeps_done * 8u + ctl[a].halt + ctl[a].launches * 131u +
+                    (inst[act[a]].R.done ? 7919u : 0u);
       if (progress == last_progress && ++stuck >= 3)
         throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "sweep control made no progress");
       if (progress != last_progress) stuck = 0;
@@ -785,70 +899,96 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
       if (guard > 4.0 * MAX_ITERATIONS_LIMIT)
         throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "solve loop did not terminate");
     }
+    HIP_OK(hipStreamSynchronize(st));  // hist_stage is host staging
     if (timing) {
       hipEventDestroy(eb0);
       hipEventDestroy(eb1);
     }
     for (int a = 0; a < nact; ++a) {
-      hipLaunchKernelGGL(k_scatter, pgrd, pblk, 0, st, ai[a], out);
+      const InstWork &X = inst[act[a]];
+      hipLaunchKernelGGL(k_scatter, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.out);
       HIP_OK(hipGetLastError());
-      rep->unknown_sweeps += (uint64_t)n_interior * res[act[a]].iterations;
+      iw[X.item].in.rep->unknown_sweeps += (uint64_t)iw[X.item].n_interior * X.R.iterations;
+    }
+    for (int i = 0; i < nitem; ++i) {
+      iw[i].in.rep->launches = total_launches;
+      iw[i].in.rep->refinements = total_refine;
+      iw[i].in.rep->sweep_ms = sweep_ms;
     }
   }
   // non-finite channels: interior samples become NaN (the reference's
   // residuals and field are NaN there)
-  for (int c = 0; c < C; ++c)
-    if (res[c].nonfinite) {
-      PhInst I = inst_h[c];
+  for (int k = 0; k < ninst; ++k)
+    if (inst[k].R.nonfinite) {
+      PhInst I = inst[k].I;
       I.x = I.bp;  // reuse: fill with NaN then scatter
-      std::vector<double> nanv((size_t)pitch * phh, NAN);
+      std::vector<double> nanv((size_t)I.pitch * I.ph, NAN);
       HIP_OK(hipMemcpyAsync(I.x, nanv.data(), nanv.size() * sizeof(double), hipMemcpyHostToDevice,
                             st));
-      hipLaunchKernelGGL(k_scatter, pgrd, pblk, 0, st, I, out);
+      hipLaunchKernelGGL(k_scatter, inst[k].pgrd, dim3(256), 0, st, I, iw[inst[k].item].in.out);
       HIP_OK(hipGetLastError());
       HIP_OK(hipStreamSynchronize(st));
     }
 
-  // --- worst channel (poisson_blend.rs:240-246: more iterations, then larger
-  // final residual; ties keep the earlier channel)
-  int worst = 0;
-  for (int c = 1; c < C; ++c) {
-    const ChanResult &a = res[worst], &b = res[c];
-    const bool worse = b.iterations != a.iterations ? b.iterations > a.iterations
-                                                    : b.final_residual > a.final_residual;
-    if (worse) worst = c;
-  }
-  const ChanResult &Wr = res[worst];
-  rep->worst_channel = (uint32_t)worst;
-  rep->iterations = Wr.iterations;
-  rep->stop_reason = Wr.stop;
-  rep->converged = Wr.converged ? 1 : 0;
-  rep->tolerance = d->tolerance;
-  rep->final_residual = Wr.final_residual;
-  rep->history_len = Wr.iterations;
-  if (rep->residual_history && rep->history_capacity >= Wr.iterations && Wr.iterations > 0) {
-    if (Wr.nonfinite) {
-      for (uint32_t i = 0; i < Wr.iterations; ++i) rep->residual_history[i] = NAN;
-    } else {
-      HIP_OK(hipMemcpyAsync(rep->residual_history, hist_d[worst], Wr.iterations * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
+  // --- worst channel of each image (poisson_blend.rs:240-246: more
+  // iterations, then larger final residual; ties keep the earlier channel)
+  for (int i = 0; i < nitem; ++i) {
+    ItemWork &w = iw[i];
+    ph_report *rep = w.in.rep;
+    ChanResult zero;
+    int worst = 0;
+    const ChanResult *Wr = &zero;
+    double *whist = nullptr;
+    if (w.any) {
+      for (int c = 0; c < w.in.C; ++c) {
+        const InstWork &X = inst[w.inst0 + c];
+        const bool worse = c == 0 ? true
+                           : X.R.iterations != Wr->iterations
+                               ? X.R.iterations > Wr->iterations
+                               : X.R.final_residual > Wr->final_residual;
+        if (worse) {
+          worst = c;
+          Wr = &X.R;
+          whist = X.hist;
+        }
+      }
     }
-  } else if (Wr.iterations > 0 && rep->residual_history) {
-    throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "history capacity %u < %u iterations",
-               rep->history_capacity, Wr.iterations);
+    rep->worst_channel = (uint32_t)worst;
+    rep->iterations = Wr->iterations;
+    rep->stop_reason = Wr->stop;
+    rep->converged = Wr->converged ? 1 : 0;
+    rep->tolerance = d->tolerance;
+    rep->final_residual = Wr->final_residual;
+    rep->history_len = Wr->iterations;
+    if (rep->residual_history && rep->history_capacity >= Wr->iterations && Wr->iterations > 0) {
+      if (Wr->nonfinite) {
+        for (uint32_t q = 0; q < Wr->iterations; ++q) rep->residual_history[q] = NAN;
+      } else {
+        HIP_OK(hipMemcpyAsync(rep->residual_history, whist, Wr->iterations * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+      }
+    } else if (Wr->iterations > 0 && rep->residual_history) {
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "history capacity %u < %u iterations",
+                 rep->history_capacity, Wr->iterations);
+    }
   }
   if (timing) {
     HIP_OK(hipEventRecord(ev1, st));
     HIP_OK(hipEventSynchronize(ev1));
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
-    rep->device_ms = ms;
+    for (int i = 0; i < nitem; ++i) iw[i].in.rep->device_ms = ms;
     hipEventDestroy(ev0);
     hipEventDestroy(ev1);
-  } else {
-    rep->device_ms = 0.0;
   }
   HIP_OK(hipStreamSynchronize(st));
+}
+
+static void solve_device(const ph_solve_desc *d, const float *src, const float *tgt,
+                         const float *mask, float *out, hipStream_t st, ph_report *rep) {
+  std::vector<ItemIn> items{
+      ItemIn{(int)d->width, (int)d->height, (int)d->channels, src, tgt, mask, out, rep}};
+  solve_items(d, items, st);
 }
 
 }  // namespace ph
@@ -892,12 +1032,12 @@ extern "C" ph_status ph_probe(ph_device_info *out, ph_error *err) {
 }
 
 static ph_status run_guarded(const ph_solve_desc *desc, ph_report *rep, ph_error *err,
-                             const std::function<void()> &body) {
+                             const std::function<void()> &body, bool check_shape = true) {
   std::lock_guard<std::mutex> lk(g_mu);
   clear_error(err);
   try {
     if (!rep) throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null report");
-    validate(desc);
+    validate(desc, check_shape);
     ensure_device();
     body();
     return PH_OK;
@@ -949,4 +1089,28 @@ extern "C" ph_status ph_poisson_solve(const ph_solve_desc *desc, const float *so
       HIP_OK(hipStreamSynchronize(st));
     }
   });
+}
+
+extern "C" ph_status ph_poisson_solve_batch_device(const ph_solve_desc *desc,
+                                                   const ph_batch_item *items, uint32_t n_items,
+                                                   void *stream, ph_report *reports,
+                                                   ph_error *err) {
+  return run_guarded(
+      desc, reports, err,
+      [&]() {
+        if (n_items > 0 && !items) throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null items");
+        std::vector<ItemIn> v;
+        v.reserve(n_items);
+        for (uint32_t i = 0; i < n_items; ++i) {
+          const ph_batch_item &b = items[i];
+          validate_shape(desc, b.width, b.height, b.channels);
+          if ((size_t)b.width * b.height > 0 &&
+              (!b.source || !b.target || !b.mask || !b.out_candidate))
+            throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "item %u: null buffer", i);
+          v.push_back(ItemIn{(int)b.width, (int)b.height, (int)b.channels, b.source, b.target,
+                             b.mask, b.out_candidate, &reports[i]});
+        }
+        if (!v.empty()) solve_items(desc, v, (hipStream_t)stream);
+      },
+      false);
 }

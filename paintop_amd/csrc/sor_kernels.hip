@@ -106,14 +106,15 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *A) {
     return;
   }
   const uint32_t in_buf = C.cur_buf;
-  const double *W = I.omega_tab + 2 * (size_t)S;  // omega of this launch's half-sweeps
+  const uint64_t h0 = 2 * (uint64_t)S;  // first half-sweep of this launch
   bool stop = false;
   if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, S, C.prev_black + A[0]);
   for (int j = 1; j < nsw && !stop; ++j) {
-    const double f = (1.0 - W[2 * j - 1]) * (1.0 - W[2 * j - 1]);
-    stop = finalize_sweep(C, S + j, f * A[2 * j - 1] + A[2 * j]);
+    const double wb = ph_omega_at(I, h0 + 2 * j - 1);  // black half-sweep of sweep S + j
+    stop = finalize_sweep(C, S + j, (1.0 - wb) * (1.0 - wb) * A[2 * j - 1] + A[2 * j]);
   }
-  C.prev_black = (1.0 - W[2 * nsw - 1]) * (1.0 - W[2 * nsw - 1]) * A[2 * nsw - 1];
+  const double wl = ph_omega_at(I, h0 + 2 * nsw - 1);
+  C.prev_black = (1.0 - wl) * (1.0 - wl) * A[2 * nsw - 1];
   if (stop) {
     C.halt_base = S;
     C.halt_buf = in_buf;
@@ -172,9 +173,9 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
   // leaves e unchanged (the control reads only the first k_run residual sums)
   float cs[K];
   {
-    const double *W = I.omega_tab + 2 * (size_t)C.sweeps_done;
+    const uint64_t h0 = 2 * (uint64_t)C.sweeps_done;
 #pragma unroll
-    for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(W[s] / I.dg64) : 0.0f;
+    for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(ph_omega_at(I, h0 + s) / I.dg64) : 0.0f;
   }
   // columns of the valid core, as 0/1 weights of the residual sums
   float cw[CPL];

@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "ph_polygon_mask_device",
     "ph_ellipse_mask_device",
     "ph_gauss_noise_device",
+    "ph_poisson_solve_batch_device",
 )
 
 
@@ -77,6 +78,19 @@ class PhReport(ctypes.Structure):
         ("warm_start_used", ctypes.c_uint32),
         ("warm_start_rel", ctypes.c_double),
         ("sweep_ms", ctypes.c_double),
+    ]
+
+
+class PhBatchItem(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("channels", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("source", ctypes.c_void_p),
+        ("target", ctypes.c_void_p),
+        ("mask", ctypes.c_void_p),
+        ("out_candidate", ctypes.c_void_p),
     ]
 
 
@@ -149,6 +163,10 @@ def lib():
     L.ph_gauss_noise_device.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, vp,
                                         ctypes.POINTER(PhError)]
     L.ph_gauss_noise_device.restype = ctypes.c_int
+    L.ph_poisson_solve_batch_device.argtypes = [ctypes.POINTER(PhSolveDesc),
+                                                ctypes.POINTER(PhBatchItem), ctypes.c_uint32, vp,
+                                                ctypes.POINTER(PhReport), ctypes.POINTER(PhError)]
+    L.ph_poisson_solve_batch_device.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -300,3 +318,27 @@ def gauss_noise_device(d_out: int, n: int, seed: int, sigma: float, stream: int 
     st = lib().ph_gauss_noise_device(d_out, n, seed & 0xFFFFFFFFFFFFFFFF, sigma, stream or None,
                                      ctypes.byref(err))
     _check(st, err)
+
+
+def poisson_solve_batch_device(desc: PhSolveDesc, items, stream: int = 0,
+                               want_history: bool = True):
+    """Batched solve. items: sequence of (width, height, channels, d_source,
+    d_target, d_mask, d_out). Returns one SolveResult per item."""
+    n = len(items)
+    arr = (PhBatchItem * max(n, 1))()
+    for i, (w, h, c, s_, t_, m_, o_) in enumerate(items):
+        arr[i] = PhBatchItem(w, h, c, 0, s_, t_, m_, o_)
+    reps = (PhReport * max(n, 1))()
+    cap = int(desc.max_iterations)
+    hists = []
+    for i in range(n):
+        hist = np.zeros(max(cap, 1), dtype=np.float64)
+        hists.append(hist)
+        if want_history:
+            reps[i].residual_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            reps[i].history_capacity = cap
+    err = PhError()
+    st = lib().ph_poisson_solve_batch_device(ctypes.byref(desc), arr, n, stream or None, reps,
+                                             ctypes.byref(err))
+    _check(st, err)
+    return [_result(reps[i], hists[i]) for i in range(n)]

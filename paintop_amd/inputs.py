@@ -152,3 +152,43 @@ def screened_case(size: int = 8192, border: int = 64):
                             [(b, b), (size - b, b), (size - b, size - b), (b, size - b)])
     synchronize()
     return guide, anchor, msk
+
+
+def batch_case(n: int = 512, seed0: int = 500):
+    """512 independent single-channel blends of uneven size: side s = exp U(ln 32,
+    ln 1024), aspect U(0.5, 2), W = round(s sqrt(a)), H = round(s / sqrt(a)); the
+    mask is a centred ellipse or a 6-10 vertex polygon filling U(0.2, 0.6) of the
+    box; source and target are fbm (frequency 0.5, 6 octaves) mapped to [0, 1].
+    Instance i draws from SplitMix64(seed0 + i). Returns a list of
+    (width, height, source, target, mask) DeviceArrays."""
+    from . import hip
+    from .device import DeviceArray, synchronize
+
+    out = []
+    for i in range(n):
+        rng = SplitMix64(seed0 + i)
+        s = math.exp(rng.uniform(math.log(32.0), math.log(1024.0)))
+        a = rng.uniform(0.5, 2.0)
+        w = max(3, int(round(s * math.sqrt(a))))
+        h = max(3, int(round(s / math.sqrt(a))))
+        fill = rng.uniform(0.2, 0.6)
+        src = DeviceArray((h, w), np.float32)
+        tgt = DeviceArray((h, w), np.float32)
+        msk = DeviceArray((h, w), np.float32)
+        hip.fbm_device(src.ptr, w, h, 0.5, 10_000 + 2 * i, 6, 2.0, 0.5, scale=0.5, bias=0.5)
+        hip.fbm_device(tgt.ptr, w, h, 0.5, 10_001 + 2 * i, 6, 2.0, 0.5, scale=0.5, bias=0.5)
+        if rng.uniform() < 0.5:
+            k = math.sqrt(4.0 * fill / math.pi)
+            hip.ellipse_mask_device(msk.ptr, w, h, w / 2.0, h / 2.0, k * w / 2.0, k * h / 2.0)
+        else:
+            nv = 6 + int(rng.uniform() * 5) % 5
+            verts = random_polygon(w / 2.0, h / 2.0, w / 2.0, h / 2.0, nv, rng)
+            x = np.array([v[0] for v in verts]) - w / 2.0
+            y = np.array([v[1] for v in verts]) - h / 2.0
+            area = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+            sc = math.sqrt(fill * w * h / max(area, 1e-9))
+            verts = [(w / 2.0 + sc * xx, h / 2.0 + sc * yy) for xx, yy in zip(x, y)]
+            hip.polygon_mask_device(msk.ptr, w, h, verts)
+        out.append((w, h, src, tgt, msk))
+    synchronize()
+    return out

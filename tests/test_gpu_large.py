@@ -1,0 +1,105 @@
+"""Full-size GPU cases, checked through size-independent properties: the
+outside-mask samples are byte-identical to the target/anchor, the FP64
+stopping decision holds, reruns are bit-identical, and the result agrees with
+the same system driven to 1e-12 (the error against a tight FP64 solve is
+stated). A mid-size case is compared with the oracle's red-black SOR.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from paintop_amd import hip
+from tests.helpers import classify, laplacian, residual_norm
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(desc, src, tgt, msk):
+    from paintop_amd.device import DeviceArray
+
+    out = DeviceArray(tgt.shape, np.float32)
+    r = hip.poisson_solve_device(desc, src.ptr, tgt.ptr, msk.ptr, out.ptr)
+    a = out.numpy()
+    out.free()
+    return a, r
+
+
+def test_seamless_clone_4096_rgba(gpu):
+    from paintop_amd.inputs import seamless_clone_case
+
+    n = 4096
+    src, tgt, msk = seamless_clone_case(n, channels=4)
+    desc = hip.make_desc(n, n, 4, 100000, 1e-6)
+    out, r = _solve(desc, src, tgt, msk)
+    print(f"[large] seamless 4096 RGBA: {r.iterations} sweeps, omega {r.omega:.6f}, "
+          f"refinements {r.refinements}, final {r.final_residual:.3e}", flush=True)
+    assert r.converged and r.final_residual <= 1e-6
+    assert len(r.residual_history) == r.iterations
+    t = tgt.numpy()
+    m = msk.numpy()
+    inside = classify(m).astype(bool)
+    for c in range(4):
+        assert out[..., c][~inside].tobytes() == t[..., c][~inside].tobytes()
+    assert out[..., 3].tobytes() == t[..., 3].tobytes()  # constant alpha: r0 == 0
+    out2, r2 = _solve(desc, src, tgt, msk)
+    assert out2.tobytes() == out.tobytes() and r2.residual_history == r.residual_history
+    tight, rt = _solve(hip.make_desc(n, n, 4, 100000, 1e-12), src, tgt, msk)
+    err = float(np.abs(out[..., :3] - tight[..., :3]).max())
+    print(f"[large] seamless 4096: max |u(1e-6) - u(1e-12)| = {err:.3e} "
+          f"({rt.iterations} sweeps to 1e-12)", flush=True)
+    assert rt.converged and err < 1e-5
+
+
+def test_screened_8192(gpu):
+    from paintop_amd.inputs import screened_case
+
+    n = 8192
+    g, a, m = screened_case(n)
+    desc = hip.make_desc(n, n, 1, 100000, 1e-7, 0.01, hip.PH_OP_SCREENED_POISSON)
+    out, r = _solve(desc, g, a, m)
+    print(f"[large] screened 8192: {r.iterations} sweeps, omega {r.omega:.6f}, "
+          f"refinements {r.refinements}, final {r.final_residual:.3e}", flush=True)
+    assert r.converged and r.final_residual <= 1e-7
+    an = a.numpy()
+    mn = m.numpy()
+    inside = classify(mn).astype(bool)
+    assert out[~inside].tobytes() == an[~inside].tobytes()
+    tight, rt = _solve(hip.make_desc(n, n, 1, 100000, 1e-12, 0.01, hip.PH_OP_SCREENED_POISSON),
+                       g, a, m)
+    err = float(np.abs(out - tight).max())
+    print(f"[large] screened 8192: max |u(1e-7) - u(1e-12)| = {err:.3e} "
+          f"({rt.iterations} sweeps to 1e-12)", flush=True)
+    assert rt.converged and err < 1e-5
+
+
+def test_star_512_against_oracle_red_black(gpu, oracle):
+    """512^2 fbm blend with the scaled star mask: GPU (FP32 sweeps + FP64
+    refinement, tuned omega) vs the oracle's FP64 red-black SOR, both to 1e-10."""
+    from paintop_amd.inputs import star_polygon
+
+    n = 512
+    sc = n / 4096
+    s = oracle.fbm(n, n, 0.125 / sc, 101, 6) * 0.5 + 0.5
+    t = oracle.fbm(n, n, 0.125 / sc, 201, 6) * 0.5 + 0.5
+    verts = [(x * sc, y * sc) for (x, y) in
+             star_polygon(2048, 2048, math.sqrt(0.3 * 4096 * 4096 / 1.5), 0.5, 6, 0.1, 301)]
+    from paintop_amd.device import DeviceArray
+    dm = DeviceArray((n, n), np.float32)
+    gpu.polygon_mask_device(dm.ptr, n, n, verts)
+    m = dm.numpy()
+    desc = hip.make_desc(n, n, 1, 100000, 1e-10)
+    out, r = gpu.poisson_solve(desc, s, t, m)
+    cells = classify(m)
+    rhs = np.where(cells == 1, laplacian(s), 0.0)
+    ref, rep, _ = oracle.rbsor_omp(cells, rhs, t.astype(np.float64), 0.0, 100000, 1e-10, r.omega,
+                                   stall_window=0)
+    inside = cells.astype(bool)
+    d = np.abs(out.reshape(n, n).astype(np.float64) - ref)[inside]
+    print(f"[large] star 512: gpu {r.iterations} sweeps, oracle rb {rep.iterations} sweeps, "
+          f"max |gpu - oracle| {d.max():.3e}", flush=True)
+    assert r.converged and rep.converged
+    assert d.max() < 1e-6
+    assert out.reshape(n, n)[~inside].tobytes() == t[~inside].tobytes()
+    r0 = residual_norm(cells, rhs, t, 0.0, t)
+    assert residual_norm(cells, rhs, t, 0.0, out.reshape(n, n)) / r0 < 1e-5
