@@ -269,6 +269,7 @@ def main():
         err = float(np.abs(out.numpy() - out_tight.numpy()).max())
         line["config"]["max_abs_err_vs_fp64_1e-12"] = err
         line["config"]["tight_sweeps"] = rt.iterations
+        line["config"]["tight_stop"] = f"{rt.stop_reason} at {rt.final_residual:.2e}"
         out_tight.free()
     for a in (src, tgt, msk, out):
         a.free()

@@ -47,8 +47,10 @@ def test_seamless_clone_4096_rgba(gpu):
     tight, rt = _solve(hip.make_desc(n, n, 4, 100000, 1e-12), src, tgt, msk)
     err = float(np.abs(out[..., :3] - tight[..., :3]).max())
     print(f"[large] seamless 4096: max |u(1e-6) - u(1e-12)| = {err:.3e} "
-          f"({rt.iterations} sweeps to 1e-12)", flush=True)
-    assert rt.converged and err < 1e-5
+          f"({rt.iterations} sweeps, {rt.stop_reason} at {rt.final_residual:.2e})", flush=True)
+    # an FP64 residual of a field of size ~0.5 bottoms out near 1e-13..1e-12
+    # relative, so the tight solve may end as stalled just above 1e-12
+    assert rt.final_residual < 1e-11 and err < 1e-5
 
 
 def test_screened_8192(gpu):
@@ -69,8 +71,8 @@ def test_screened_8192(gpu):
                        g, a, m)
     err = float(np.abs(out - tight).max())
     print(f"[large] screened 8192: max |u(1e-7) - u(1e-12)| = {err:.3e} "
-          f"({rt.iterations} sweeps to 1e-12)", flush=True)
-    assert rt.converged and err < 1e-5
+          f"({rt.iterations} sweeps, {rt.stop_reason} at {rt.final_residual:.2e})", flush=True)
+    assert rt.final_residual < 1e-11 and err < 1e-5
 
 
 def test_star_512_against_oracle_red_black(gpu, oracle):
