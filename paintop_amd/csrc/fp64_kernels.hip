@@ -17,33 +17,78 @@ namespace ph {
 
 constexpr float MASK_THRESHOLD = 0.5f;  // poisson.rs:329
 
-// Classify every image pixel and grow the bounding box of interior cells
-// (bbox: xmin, ymin, xmax, ymax; int atomics, order independent).
-__global__ void k_classify(const float *__restrict__ mask, int w, int h,
-                           uint8_t *__restrict__ cls, int *__restrict__ bbox) {
+// Classify every image pixel (one block per 256 pixels of a row). Each block
+// folds its interior columns into the row's [min, max] (few atomics per
+// row); k_bbox then reduces the rows.
+__global__ __launch_bounds__(256) void k_classify(const float *__restrict__ mask, int w, int h,
+                                                  uint8_t *__restrict__ cls,
+                                                  int *__restrict__ rowx) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
-  if (x >= w) return;
-  const size_t idx = (size_t)y * w + x;
   uint8_t in = 0;
-  if (w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 && y < h - 1)
-    in = mask[idx] > MASK_THRESHOLD ? 1 : 0;
-  cls[idx] = in;
-  // wave-level pre-reduction keeps the atomics few
-  const unsigned long long ball = __ballot(in);
-  if (ball) {
-    int xmin = in ? x : 0x7fffffff, xmax = in ? x : -1;
+  if (x < w) {
+    const size_t idx = (size_t)y * w + x;
+    if (w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 && y < h - 1)
+      in = mask[idx] > MASK_THRESHOLD ? 1 : 0;
+    cls[idx] = in;
+  }
+  int xmin = in ? x : 0x7fffffff, xmax = in ? x : -1;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      xmin = min(xmin, __shfl_xor(xmin, off, 64));
-      xmax = max(xmax, __shfl_xor(xmax, off, 64));
+  for (int off = 32; off >= 1; off >>= 1) {
+    xmin = min(xmin, __shfl_xor(xmin, off, 64));
+    xmax = max(xmax, __shfl_xor(xmax, off, 64));
+  }
+  __shared__ int smin[4], smax[4];
+  if ((threadIdx.x & 63) == 0) {
+    smin[threadIdx.x >> 6] = xmin;
+    smax[threadIdx.x >> 6] = xmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    xmin = min(min(smin[0], smin[1]), min(smin[2], smin[3]));
+    xmax = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
+    if (xmax >= 0) {
+      atomicMin(&rowx[2 * y], xmin);
+      atomicMax(&rowx[2 * y + 1], xmax);
     }
-    if ((threadIdx.x & 63) == 0) {
-      atomicMin(&bbox[0], xmin);
-      atomicMax(&bbox[2], xmax);
-      atomicMin(&bbox[1], y);
-      atomicMax(&bbox[3], y);
+  }
+}
+
+// Bounding box of the interior cells from the per-row [min, max] (one block).
+__global__ __launch_bounds__(256) void k_bbox(const int *__restrict__ rowx, int h,
+                                              int *__restrict__ bbox) {
+  int xmin = 0x7fffffff, xmax = -1, ymin = 0x7fffffff, ymax = -1;
+  for (int y = threadIdx.x; y < h; y += blockDim.x)
+    if (rowx[2 * y + 1] >= 0) {
+      xmin = min(xmin, rowx[2 * y]);
+      xmax = max(xmax, rowx[2 * y + 1]);
+      ymin = min(ymin, y);
+      ymax = max(ymax, y);
     }
+  __shared__ int s[4][256];
+  s[0][threadIdx.x] = xmin;
+  s[1][threadIdx.x] = ymin;
+  s[2][threadIdx.x] = xmax;
+  s[3][threadIdx.x] = ymax;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      s[0][threadIdx.x] = min(s[0][threadIdx.x], s[0][threadIdx.x + k]);
+      s[1][threadIdx.x] = min(s[1][threadIdx.x], s[1][threadIdx.x + k]);
+      s[2][threadIdx.x] = max(s[2][threadIdx.x], s[2][threadIdx.x + k]);
+      s[3][threadIdx.x] = max(s[3][threadIdx.x], s[3][threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) bbox[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Per-row [min, max] start empty.
+__global__ void k_rowx_init(int *__restrict__ rowx, int h) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y < h) {
+    rowx[2 * y] = 0x7fffffff;
+    rowx[2 * y + 1] = -1;
   }
 }
 

@@ -58,6 +58,7 @@ struct PhCtl {
   double tol;
   double refine_abs2; // refine when the squared inner residual falls to this
   double best;        // best relative residual so far (stall guard)
+  double last_rel;    // relative residual of sweep sweeps_final
   double prev_black;  // (1 - omega)^2 x black pre-update residual sum, last half-sweep of the previous launch
   double *history;    // relative residual per sweep, capacity max_it
   uint32_t max_it;

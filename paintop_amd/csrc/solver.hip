@@ -42,6 +42,8 @@ __global__ void k_sor(const PhTile *, int, const PhInst *, const PhCtl *, double
 template <int K>
 __global__ void k_sor_control(const PhInst *, PhCtl *, const double *, int);
 __global__ void k_classify(const float *, int, int, uint8_t *, int *);
+__global__ void k_bbox(const int *, int, int *);
+__global__ void k_rowx_init(int *, int);
 __global__ void k_tile_occupancy(const uint8_t *, int, int, int, int, int, int, int, const int2 *,
                                  int *);
 __global__ void k_coarse_count(const uint8_t *, int, int, int, int, int, int, int, int *);
@@ -448,7 +450,7 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t ph_up = (size_t)H + 2 * PH_PAD + 4 + hb + 2 * K;
   const size_t plane_up = pw_up * ph_up;
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
-  return npx + 8 * 1024 + ntile_up * (sizeof(int2) + sizeof(int)) +
+  return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) + ntile_up * (sizeof(int2) + sizeof(int)) +
          plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
          (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
          (size_t)C * ph_up * sizeof(double) +
@@ -493,13 +495,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
 
   // --- 1. classification and bounding boxes (one readback for all images)
   int *bbox_d = g_pool.take<int>(4 * (size_t)nitem);
-  std::vector<int> bbox_h(4 * (size_t)nitem);
-  for (int i = 0; i < nitem; ++i) {
-    bbox_h[4 * i + 0] = bbox_h[4 * i + 1] = 0x7fffffff;
-    bbox_h[4 * i + 2] = bbox_h[4 * i + 3] = -1;
-  }
-  HIP_OK(hipMemcpyAsync(bbox_d, bbox_h.data(), bbox_h.size() * sizeof(int), hipMemcpyHostToDevice,
-                        st));
+  std::vector<int> bbox_h(4 * (size_t)nitem, -1);
   for (int i = 0; i < nitem; ++i) {
     const ItemIn &it = iw[i].in;
     const size_t npx = (size_t)it.W * it.H;
@@ -508,13 +504,18 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     HIP_OK(hipMemcpyAsync(it.out, it.tgt, npx * it.C * sizeof(float), hipMemcpyDeviceToDevice,
                           st));
     iw[i].cls_img = g_pool.take<uint8_t>(npx);
+    int *rowx = g_pool.take<int>(2 * (size_t)it.H);
+    hipLaunchKernelGGL(k_rowx_init, dim3((it.H + 255) / 256), dim3(256), 0, st, rowx, it.H);
     hipLaunchKernelGGL(k_classify, dim3((it.W + 255) / 256, it.H), dim3(256), 0, st, it.mask,
-                       it.W, it.H, iw[i].cls_img, bbox_d + 4 * i);
+                       it.W, it.H, iw[i].cls_img, rowx);
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(256), 0, st, rowx, it.H, bbox_d + 4 * i);
     HIP_OK(hipGetLastError());
   }
   HIP_OK(hipMemcpyAsync(bbox_h.data(), bbox_d, bbox_h.size() * sizeof(int), hipMemcpyDeviceToHost,
                         st));
   HIP_OK(hipStreamSynchronize(st));
+  for (int i = 0; i < nitem; ++i)
+    if ((size_t)iw[i].in.W * iw[i].in.H == 0) bbox_h[4 * i + 2] = -1;
 
   // --- 2. planes, tile occupancy, coarse masks for the tuner (one readback)
   std::vector<int2> all_origins;
@@ -780,6 +781,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     };
     std::vector<Pending> pend;
     std::vector<double> hist_stage;
+    struct Trend {
+      uint32_t m = 0;
+      double rel = 0.0;
+    };
+    std::vector<Trend> trend(nact);
     uint32_t total_launches = 0, total_refine = 0;
     double sweep_ms = 0.0;
     while (remaining > 0) {
@@ -874,6 +880,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       }
       // next batch: up to the next scheduled halt (refinement or sweep limit) of
       // the running instances, so no launch finds every instance halted
+      // and, once the residual decays steadily, not far past the sweep at
+      // which the observed decay rate reaches the tolerance
       {
         uint32_t need = 0;
         for (int a = 0; a < nact; ++a) {
@@ -881,7 +889,20 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           if (inst[act[a]].R.done || Cc.halt != PH_RUN) continue;
           uint32_t next = std::min(Cc.sweep_limit, Cc.last_refine + Cc.refine_every);
           if (Cc.mode == PH_MODE_REDO) next = Cc.sweep_limit;
-          const uint32_t todo = next > Cc.sweeps_done ? next - Cc.sweeps_done : 1;
+          uint32_t todo = next > Cc.sweeps_done ? next - Cc.sweeps_done : 1;
+          Trend &T = trend[a];
+          if (Cc.mode == PH_MODE_NORMAL && Cc.sweeps_final >= T.m + 16 && T.rel > 0.0 &&
+              Cc.last_rel > 0.0 && Cc.last_rel < T.rel) {
+            const double rate = log(Cc.last_rel / T.rel) / (double)(Cc.sweeps_final - T.m);
+            const double left = log(Cc.tol / Cc.last_rel) / rate;  // sweeps to tolerance
+            const double lead = (double)Cc.sweeps_done - (double)Cc.sweeps_final;
+            const double est = std::max(1.0, 1.15 * left - lead + 8.0);
+            if (est < (double)todo) todo = (uint32_t)est;
+          }
+          if (Cc.sweeps_final >= T.m + 16) {
+            T.m = Cc.sweeps_final;
+            T.rel = Cc.last_rel;
+          }
           need = std::max(need, (todo + (uint32_t)(K / 2) - 1) / (uint32_t)(K / 2));
         }
         batch = (int)std::max<uint32_t>(1, std::min<uint32_t>(need, 64));

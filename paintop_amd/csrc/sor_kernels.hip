@@ -71,6 +71,7 @@ __device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
   const double rel = sqrt(res2) / C.r0;
   C.history[m - 1] = rel;
   C.sweeps_final = m;
+  C.last_rel = rel;
   if (rel <= C.tol) {
     C.reason = PH_REASON_CONVERGED;
     C.halt_sweep = m;
