@@ -111,8 +111,8 @@ typedef struct ph_report {
   double device_ms;          /* solve time on the device stream (PH_FLAG_TIMING) */
   double omega_used;
   uint32_t half_sweeps_used;
-  uint32_t warm_start_used;  /* 1 when the learned start was kept */
-  double warm_start_rel;     /* ||r(warm start)|| / ||r(anchor start)||, 0 if unused */
+  uint32_t warm_start_used;  /* channels that kept the learned start (0: plain start) */
+  double warm_start_rel;     /* worst channel's ||r(warm start)|| / ||r(anchor start)||, 0 if not tried */
   double sweep_ms;           /* hipEvent time of the sweep-kernel batches (PH_FLAG_TIMING) */
 } ph_report;
 
@@ -209,6 +209,78 @@ ph_status ph_gauss_noise_device(float *d_out, uint64_t n, uint64_t seed, double 
 /* Binary axis-aligned ellipse mask: 1 where ((x+.5-cx)/rx)^2 + ((y+.5-cy)/ry)^2 <= 1. */
 ph_status ph_ellipse_mask_device(float *d_out, uint32_t width, uint32_t height, double cx,
                                  double cy, double rx, double ry, void *stream, ph_error *err);
+
+/* ---- learned warm start ------------------------------------------------- *
+ * No reference counterpart: paintop's solvers always start from the anchor
+ * (poisson.rs:169-170). The warm start keeps the reference's relative-residual
+ * denominator ||r(anchor start)|| (poisson.rs:172) and its stop rules.
+ *
+ * The network: a 4-level U-Net (3x3 convolutions, channels 16/32/64/64, BF16
+ * weights and activations, FP32 accumulation on the matrix cores, FP32 1x1
+ * head) on a res x res grid, input [batch][res][res][4] FP32 (channel 0: the
+ * right-hand side of the correction problem, 1: boundary values with 0 in the
+ * interior, 2: interior fraction, 3: zero), output [batch][res][res] FP32.
+ * Parameters are ph_unet_param_storage() FP32 values in the layout
+ * documented in paintop_amd/csrc/unet_internal.h (padding slots must be 0). */
+
+/* FP32 values of the parameter storage (padding included). */
+uint64_t ph_unet_param_storage(void);
+/* Learnable values (padding excluded). */
+uint64_t ph_unet_param_count(void);
+
+typedef struct ph_unet ph_unet; /* a network with its training state and buffers */
+
+typedef struct ph_train_opts {
+  float learning_rate;
+  float beta1, beta2, epsilon; /* Adam */
+  float weight_decay;          /* decoupled (AdamW) */
+  int32_t apply_update;        /* 0: compute loss and gradients only */
+} ph_train_opts;
+
+/* Buffers for batches up to max_batch at resolution res (a multiple of 8 in
+ * [16, 1024]); params = NULL draws a seeded He-normal initialisation. */
+ph_status ph_unet_create(int32_t max_batch, int32_t res, const float *params, uint64_t seed,
+                         ph_unet **net, ph_error *err);
+void ph_unet_destroy(ph_unet *net);
+
+/* d_out = net(d_in) for `batch` inputs (device buffers). */
+ph_status ph_unet_forward(ph_unet *net, const float *d_in, float *d_out, int32_t batch,
+                          void *stream, ph_error *err);
+
+/* One training step: forward, loss = sum_p w[p] (out[p] - target[p])^2 / n,
+ * backward, and (if opts->apply_update) an AdamW update. *loss (host) gets
+ * the loss before the update. */
+ph_status ph_unet_train_step(ph_unet *net, const float *d_in, const float *d_target,
+                             const float *d_weight, int32_t batch, const ph_train_opts *opts,
+                             double *loss, void *stream, ph_error *err);
+
+/* Training state to / from host arrays of ph_unet_param_storage() floats
+ * (any pointer may be NULL); the gradients are those of the last step. */
+ph_status ph_unet_get_state(ph_unet *net, float *params, float *grads, float *adam_m,
+                            float *adam_v, int32_t *step, ph_error *err);
+ph_status ph_unet_set_state(ph_unet *net, const float *params, const float *adam_m,
+                            const float *adam_v, int32_t step, ph_error *err);
+
+/* The weights the solver's warm start (PH_FLAG_WARM_START_NET) runs with;
+ * params = NULL clears them. */
+ph_status ph_warm_start_set_weights(const float *params, uint64_t n_params, ph_error *err);
+
+/* Net input for one channel of an image (and, when d_solution is given, the
+ * training target): the region [x0, x0 + res*f) x [y0, y0 + res*f) of the
+ * image, averaged over f x f blocks. With interior = (mask > 0.5 and not on
+ * the image border) (poisson.rs:340-355) and P = source:
+ *   in[..][0] = f^2 * mean(interior ? lambda*(P - anchor) : 0)
+ *   in[..][1] = mean(!interior and inside the image ? anchor - P : 0)
+ *   in[..][2] = mean(interior)
+ *   target    = mean over in-image cells of (solution - P)
+ *   weight    = fraction of the block inside the image.
+ * d_in is [res][res][4], d_target / d_weight [res][res]. */
+ph_status ph_warm_coarsen_device(const float *d_source, const float *d_anchor,
+                                 const float *d_mask, const float *d_solution, uint32_t width,
+                                 uint32_t height, uint32_t channels, uint32_t channel,
+                                 double lambda, int32_t x0, int32_t y0, int32_t factor,
+                                 int32_t res, float *d_in, float *d_target, float *d_weight,
+                                 void *stream, ph_error *err);
 
 #ifdef __cplusplus
 }

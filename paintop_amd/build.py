@@ -11,7 +11,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpaintop_hip.so")
-SOURCES = ["sor_kernels.hip", "fp64_kernels.hip", "gen_kernels.hip", "solver.hip"]
+SOURCES = [
+    "sor_kernels.hip", "fp64_kernels.hip", "gen_kernels.hip", "solver.hip",
+    "unet_kernels.hip", "unet.hip", "warm_kernels.hip",
+]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

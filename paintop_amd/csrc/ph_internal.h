@@ -11,6 +11,7 @@
 // (red = even).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #define PH_PAD 16          // >= the largest half-sweep count per launch
@@ -29,9 +30,9 @@ struct PhInst {
   int32_t pw, ph;   // plane width / height in cells
   int32_t ox, oy;   // image coordinates of plane cell (0, 0)
   int32_t img_w, img_h, channels, channel; // source image geometry for gather/scatter
-  double omega;        // SOR factor of half-sweeps h < omega_late_from (0-based h)
-  double omega_late;   // SOR factor of half-sweeps h >= omega_late_from
-  uint32_t omega_late_from;
+  double omega;        // SOR factor (after the ramp)
+  double ramp_log;     // ln(2 - omega) / ramp_sweeps
+  uint32_t ramp_sweeps; // sweeps of the warm-start ramp (0: none)
   uint32_t pad0_;
   float dg;         // 4 + lambda (FP32)
   double dg64;      // 4 + lambda (FP64)
@@ -82,8 +83,14 @@ struct PhCtl {
 };
 
 // SOR factor of half-sweep h (0-based, counted from the first sweep).
+// SOR factor of half-sweep h (0-based). Both half-sweeps of sweep k = h/2
+// use one factor. After a warm start, sweep k < ramp_sweeps uses
+// 2 - (2 - omega)^(k / ramp_sweeps): Gauss-Seidel first (the high-frequency
+// error the warm start leaves decays fastest there), then omega.
 __host__ __device__ inline double ph_omega_at(const PhInst &I, uint64_t h) {
-  return h < I.omega_late_from ? I.omega : I.omega_late;
+  const uint64_t k = h >> 1;
+  if (k >= I.ramp_sweeps) return I.omega;
+  return 2.0 - exp((double)k * I.ramp_log);
 }
 
 struct PhTile {

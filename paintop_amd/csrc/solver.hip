@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../../include/paintop_hip.h"
+#include "host_common.h"
 #include "ph_internal.h"
 
 namespace ph {
@@ -51,17 +52,18 @@ __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *
 __global__ void k_apply_correction(PhInst, int);
 __global__ void k_residual(PhInst, double *, uint32_t *, double *);
 __global__ void k_scatter(PhInst, float *);
+__global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
+                          int, double, int, int, int, int, float *, float *, float *);
+__global__ void k_warm_apply(PhInst, const float *, const float *, int, int, int, int);
+bool warm_net_ready();
+int warm_net_res();
+int warm_net_batch();
+void warm_net_forward(const float *d_in, float *d_out, int batch, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // errors
 
-struct Fail {
-  int cls;
-  std::string code;
-  std::string msg;
-};
-
-static Fail fail(int cls, const char *code, const char *fmt, ...) {
+Fail fail(int cls, const char *code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -70,22 +72,14 @@ static Fail fail(int cls, const char *code, const char *fmt, ...) {
   return Fail{cls, code, buf};
 }
 
-#define HIP_OK(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t _e = (expr);                                                            \
-    if (_e != hipSuccess)                                                              \
-      throw fail(PH_CLASS_EXECUTION, "E_HIP_RUNTIME", "%s failed: %s (%s:%d)", #expr, \
-                 hipGetErrorString(_e), __FILE__, __LINE__);                           \
-  } while (0)
-
-static void put_error(ph_error *err, const Fail &f) {
+void put_error(ph_error *err, const Fail &f) {
   if (!err) return;
   err->error_class = f.cls;
   snprintf(err->code, sizeof(err->code), "%s", f.code.c_str());
   snprintf(err->message, sizeof(err->message), "%s", f.msg.c_str());
 }
 
-static void clear_error(ph_error *err) {
+void clear_error(ph_error *err) {
   if (!err) return;
   err->error_class = -1;
   err->code[0] = 0;
@@ -95,17 +89,9 @@ static void clear_error(ph_error *err) {
 // ---------------------------------------------------------------------------
 // device context (one per process, behind a mutex)
 
-static std::mutex g_mu;
-static int g_forced = 0;
-
-struct Ctx {
-  bool probed = false;
-  bool ok = false;
-  std::string why;
-  int dev = 0;
-  hipDeviceProp_t prop{};
-};
-static Ctx g_ctx;
+std::mutex g_mu;
+int g_forced = 0;
+Ctx g_ctx;
 
 static bool forced_unavailable() {
   if (g_forced) return true;
@@ -113,7 +99,7 @@ static bool forced_unavailable() {
   return e && e[0] == '1';
 }
 
-static void ensure_device() {
+void ensure_device() {
   if (forced_unavailable())
     throw fail(PH_CLASS_POLICY, "E_GPU_UNAVAILABLE",
                "no compatible GPU is available: forced unavailable");
@@ -304,6 +290,15 @@ static double omega_for(double mu, double lambda) {
 // residual of red/black SOR may rise before it falls, lasts ~1/(2 - omega)
 // sweeps, so a tuned omega gets a window of 3/(2 - omega) (16 for any
 // omega <= 1.81). A fixed omega from the caller keeps the reference's 16.
+// Sweeps of the omega ramp after a kept warm start: a multiple of 1/(2 - omega),
+// which grows with the domain's size like the SOR transient itself.
+static uint32_t warm_ramp_sweeps(double omega) {
+  double fct = 2.5;
+  if (const char *e = getenv("PAINTOP_HIP_RAMP_FACTOR")) fct = atof(e);
+  if (!(fct >= 0.0)) fct = 0.0;
+  return (uint32_t)std::min(1e5, std::ceil(fct / (2.0 - omega)));
+}
+
 static uint32_t stall_window_for(double omega, bool tuned) {
   if (!tuned) return PH_STALL_WINDOW;
   const double w = ceil(3.0 / std::max(2.0 - omega, 1e-6));
@@ -466,6 +461,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int hb = d->tile_rows ? (int)d->tile_rows : 48;
   const bool timing = (d->flags & PH_FLAG_TIMING) != 0;
   const bool tuned = d->omega <= 0.0;
+  const bool want_warm =
+      (d->flags & PH_FLAG_WARM_START_NET) && !(d->flags & PH_FLAG_FORCE_PLAIN);
+  if (want_warm && !warm_net_ready())
+    throw fail(PH_CLASS_POLICY, "E_HIP_WARM_START_UNSET",
+               "warm start requested but no network weights are set (ph_warm_start_set_weights)");
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (timing) {
     HIP_OK(hipEventCreate(&ev0));
@@ -475,6 +475,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   {
     size_t need = 64 * 1024 + (size_t)nitem * 64;
     for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
+    if (want_warm) {
+      const size_t R2 = (size_t)warm_net_res() * warm_net_res();
+      size_t nch = 0;
+      for (const ItemIn &it : items) nch += it.C;
+      need += (R2 * 4 * warm_net_batch() + R2 * nch) * sizeof(float) + 1024;
+    }
     g_pool.reset(need);
   }
   std::vector<ItemWork> iw(nitem);
@@ -629,8 +635,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       I.dg64 = 4.0 + d->lambda;
       I.dg = (float)I.dg64;
       I.omega = w.omega;
-      I.omega_late = w.omega;
-      I.omega_late_from = 0xffffffffu;
+      I.ramp_log = 0.0;
+      I.ramp_sweeps = 0;
       I.lambda = d->lambda;
       X.pgrd = dim3((w.pitch + 255) / 256, w.phh);
       X.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
@@ -682,6 +688,73 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     iw[X.item].in.rep->unknowns += (uint64_t)iw[X.item].n_interior;
   }
 
+  // --- 5b. learned warm start: x = P + bilinear(net(coarse inputs)) at the
+  // interior, kept only when its FP64 residual is no worse than the anchor
+  // start's; the relative residual keeps the anchor denominator r0.
+  std::vector<double> rstart(ninst, 0.0);
+  std::vector<char> warm_kept(ninst, 0);
+  for (int k = 0; k < ninst; ++k) rstart[k] = inst[k].r0;
+  if (want_warm && d->lambda == 0.0 && !act.empty()) {
+    const int R = warm_net_res(), NB = warm_net_batch();
+    const size_t R2 = (size_t)R * R;
+    const int nw = (int)act.size();
+    float *in_d = g_pool.take<float>(R2 * 4 * NB);
+    float *h_d = g_pool.take<float>(R2 * nw);
+    struct Geo { int x0, y0, f; };
+    std::vector<Geo> geo(nw);
+    for (int j = 0; j < nw; ++j) {
+      const ItemWork &w = iw[inst[act[j]].item];
+      const int f = std::max(1, (std::max(w.rw, w.rh) + R - 1) / R);
+      geo[j] = Geo{w.rx0 + w.rw / 2 - (R * f) / 2, w.ry0 + w.rh / 2 - (R * f) / 2, f};
+    }
+    for (int j0 = 0; j0 < nw; j0 += NB) {
+      const int nb = std::min(NB, nw - j0);
+      for (int j = 0; j < nb; ++j) {
+        const InstWork &X = inst[act[j0 + j]];
+        const ItemIn &it = iw[X.item].in;
+        const Geo &g = geo[j0 + j];
+        hipLaunchKernelGGL(k_coarsen, dim3(R * R), dim3(64), 0, st, it.src, it.tgt, it.mask,
+                           nullptr, it.W, it.H, it.C, X.channel, d->lambda, g.x0, g.y0, g.f, R,
+                           in_d + R2 * 4 * j, nullptr, nullptr);
+      }
+      HIP_OK(hipGetLastError());
+      warm_net_forward(in_d, h_d + R2 * j0, nb, st);
+    }
+    for (int j = 0; j < nw; ++j) {
+      const InstWork &X = inst[act[j]];
+      const Geo &g = geo[j];
+      hipLaunchKernelGGL(k_warm_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.src,
+                         h_d + R2 * j, R, g.x0, g.y0, g.f);
+      HIP_OK(hipGetLastError());
+      launch_residual(act[j]);
+    }
+    HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int j = 0; j < nw; ++j) {
+      const int k = act[j];
+      InstWork &X = inst[k];
+      const ItemWork &w = iw[X.item];
+      const double rw = resv[2 * k];
+      ph_report *rep = w.in.rep;
+      rep->warm_start_rel = std::max(rep->warm_start_rel, std::isfinite(rw) ? rw / X.r0 : INFINITY);
+      if (std::isfinite(rw) && rw <= X.r0) {
+        warm_kept[k] = 1;
+        rstart[k] = rw;
+        rep->warm_start_used += 1;
+        const uint32_t n = warm_ramp_sweeps(w.omega);
+        X.I.ramp_sweeps = n;
+        X.I.ramp_log = n ? log(2.0 - w.omega) / n : 0.0;
+      } else {
+        // fall back to the plain start: x = anchor again, g = -r(anchor)
+        hipLaunchKernelGGL(k_assemble, X.pgrd, dim3(256), 0, st, X.I, w.in.src, w.in.tgt,
+                           w.cls_img, w.cls_plane);
+        HIP_OK(hipGetLastError());
+        launch_residual(k);
+      }
+    }
+  }
+
   const int nact = (int)act.size();
   if (nact > 0) {
     // the tile list of the running instances (rebuilt as instances finish)
@@ -721,14 +794,16 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       memset(&Cc, 0, sizeof(Cc));
       Cc.r0 = X.r0;
       Cc.tol = d->tolerance;
-      Cc.refine_abs2 = (REFINE_RATIO * X.r0) * (REFINE_RATIO * X.r0);
+      const double rs = rstart[act[a]];
+      Cc.refine_abs2 = (REFINE_RATIO * rs) * (REFINE_RATIO * rs);
       Cc.best = INFINITY;
       Cc.history = X.hist;
       Cc.max_it = d->max_iterations;
       Cc.sweep_limit = d->max_iterations;
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
-      Cc.stall_window = w.stall_window;
+      // the stall guard's window starts after a warm start's omega ramp
+      Cc.stall_window = w.stall_window + X.I.ramp_sweeps;
       Cc.refine_every = REFINE_EVERY;
       Cc.last_refine = 0;
     }

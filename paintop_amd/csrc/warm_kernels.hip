@@ -1,0 +1,136 @@
+// warm_kernels.hip -- the learned warm start's data path.
+//
+// The solution is written as u = P + h with P = the source (guidance) image.
+// For the blend operator the source's own 5-point Laplacian is the right-hand
+// side (poisson_blend.rs:186-236, poisson.rs:364-373), so h satisfies
+//   (Laplacian - lambda) h = lambda * (P - anchor)   at interior cells
+//   h = anchor - P                                   on the Dirichlet ring
+// which for lambda = 0 is a Laplace problem: h is smooth inside the mask and
+// carries no fine texture. The network predicts h on a coarse grid (f x f
+// block means of a region around the mask); the solver's start is
+// x = P + bilinear(h) at interior cells.
+#include <math.h>
+
+#include "../../include/paintop_hip.h"
+#include "host_common.h"
+#include "ph_internal.h"
+
+namespace ph {
+
+constexpr float WARM_MASK_THRESHOLD = 0.5f;  // poisson.rs:329
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One wave per coarse cell; lanes stride the block's columns, sums in a fixed
+// order (lane-local rows, then the butterfly), so results are deterministic.
+__global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
+                                                const float *__restrict__ anc,
+                                                const float *__restrict__ mask,
+                                                const float *__restrict__ sol, int W, int H, int C,
+                                                int ch, double lambda, int x0, int y0, int f,
+                                                int res, float *__restrict__ in4,
+                                                float *__restrict__ tgt, float *__restrict__ wt) {
+  const int c = blockIdx.x;
+  const int cx = c % res, cy = c / res;
+  const int lane = threadIdx.x;
+  double s_rhs = 0.0, s_bnd = 0.0, s_int = 0.0, s_t = 0.0, s_img = 0.0;
+  for (int dy = 0; dy < f; ++dy) {
+    const int iy = y0 + cy * f + dy;
+    if (iy < 0 || iy >= H) continue;
+    for (int dx = lane; dx < f; dx += 64) {
+      const int ix = x0 + cx * f + dx;
+      if (ix < 0 || ix >= W) continue;
+      const size_t q = (size_t)iy * W + ix;
+      const bool in = ix >= 1 && ix < W - 1 && iy >= 1 && iy < H - 1 &&
+                      mask[q] > WARM_MASK_THRESHOLD;
+      const double P = (double)src[q * C + ch], A = (double)anc[q * C + ch];
+      s_img += 1.0;
+      if (in) {
+        s_int += 1.0;
+        s_rhs += lambda * (P - A);
+        if (sol) s_t += (double)sol[q * C + ch] - P;
+      } else {
+        s_bnd += A - P;
+        if (sol) s_t += A - P;
+      }
+    }
+  }
+  s_rhs = wave_sum_d(s_rhs);
+  s_bnd = wave_sum_d(s_bnd);
+  s_int = wave_sum_d(s_int);
+  s_t = wave_sum_d(s_t);
+  s_img = wave_sum_d(s_img);
+  if (lane == 0) {
+    const double inv = 1.0 / ((double)f * f);
+    in4[(size_t)c * 4 + 0] = (float)s_rhs;  // f^2 * mean
+    in4[(size_t)c * 4 + 1] = (float)(s_bnd * inv);
+    in4[(size_t)c * 4 + 2] = (float)(s_int * inv);
+    in4[(size_t)c * 4 + 3] = 0.0f;
+    if (tgt) tgt[c] = s_img > 0.0 ? (float)(s_t / s_img) : 0.0f;
+    if (wt) wt[c] = (float)(s_img * inv);
+  }
+}
+
+// x = P + bilinear(h) at the interior cells of an instance's plane (cell
+// centres: image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5,
+// clamped to the grid).
+__global__ void k_warm_apply(PhInst I, const float *__restrict__ src, const float *__restrict__ hc,
+                             int res, int x0, int y0, int f) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pw) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  if (!I.cls[p]) return;
+  const int ix = I.ox + px, iy = I.oy + py;
+  const double inv = 1.0 / f;
+  double u = ((ix - x0) + 0.5) * inv - 0.5, v = ((iy - y0) + 0.5) * inv - 0.5;
+  u = fmin(fmax(u, 0.0), (double)(res - 1));
+  v = fmin(fmax(v, 0.0), (double)(res - 1));
+  const int i0 = min((int)u, res - 1), j0 = min((int)v, res - 1);
+  const int i1 = min(i0 + 1, res - 1), j1 = min(j0 + 1, res - 1);
+  const double a = u - i0, b = v - j0;
+  const double h00 = hc[(size_t)j0 * res + i0], h01 = hc[(size_t)j0 * res + i1];
+  const double h10 = hc[(size_t)j1 * res + i0], h11 = hc[(size_t)j1 * res + i1];
+  const double h = (1.0 - b) * ((1.0 - a) * h00 + a * h01) + b * ((1.0 - a) * h10 + a * h11);
+  const size_t q = (size_t)iy * I.img_w + ix;
+  I.x[p] = (double)src[q * I.channels + I.channel] + h;
+}
+
+}  // namespace ph
+
+using namespace ph;
+
+extern "C" ph_status ph_warm_coarsen_device(const float *d_source, const float *d_anchor,
+                                            const float *d_mask, const float *d_solution,
+                                            uint32_t width, uint32_t height, uint32_t channels,
+                                            uint32_t channel, double lambda, int32_t x0,
+                                            int32_t y0, int32_t factor, int32_t res, float *d_in,
+                                            float *d_target, float *d_weight, void *stream,
+                                            ph_error *err) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  clear_error(err);
+  try {
+    ensure_device();
+    if (!d_source || !d_anchor || !d_mask || !d_in)
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
+    if (channels == 0 || channel >= channels || factor < 1 || res < 1 || res > 4096)
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT",
+                 "coarsen: channel %u of %u, factor %d, resolution %d", channel, channels, factor,
+                 res);
+    if (d_solution && (!d_target || !d_weight))
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "coarsen: target/weight buffers required");
+    hipLaunchKernelGGL(k_coarsen, dim3(res * res), dim3(64), 0, (hipStream_t)stream, d_source,
+                       d_anchor, d_mask, d_solution, (int)width, (int)height, (int)channels,
+                       (int)channel, lambda, x0, y0, factor, res, d_in,
+                       d_solution ? d_target : nullptr, d_solution ? d_weight : nullptr);
+    HIP_OK(hipGetLastError());
+    return PH_OK;
+  } catch (const Fail &f) {
+    put_error(err, f);
+    return f.code == "E_GPU_UNAVAILABLE" ? PH_UNAVAILABLE : PH_ERROR;
+  }
+}

@@ -34,6 +34,16 @@ EXPORTED_SYMBOLS = (
     "ph_ellipse_mask_device",
     "ph_gauss_noise_device",
     "ph_poisson_solve_batch_device",
+    "ph_unet_param_storage",
+    "ph_unet_param_count",
+    "ph_unet_create",
+    "ph_unet_destroy",
+    "ph_unet_forward",
+    "ph_unet_train_step",
+    "ph_unet_get_state",
+    "ph_unet_set_state",
+    "ph_warm_start_set_weights",
+    "ph_warm_coarsen_device",
 )
 
 
@@ -114,6 +124,17 @@ class PhDeviceInfo(ctypes.Structure):
     ]
 
 
+class PhTrainOpts(ctypes.Structure):
+    _fields_ = [
+        ("learning_rate", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("epsilon", ctypes.c_float),
+        ("weight_decay", ctypes.c_float),
+        ("apply_update", ctypes.c_int32),
+    ]
+
+
 class HipCallError(RuntimeError):
     """A failed C-ABI call, carrying the paintop error class and code."""
 
@@ -167,6 +188,31 @@ def lib():
                                                 ctypes.POINTER(PhBatchItem), ctypes.c_uint32, vp,
                                                 ctypes.POINTER(PhReport), ctypes.POINTER(PhError)]
     L.ph_poisson_solve_batch_device.restype = ctypes.c_int
+    fp = ctypes.POINTER(ctypes.c_float)
+    pe = ctypes.POINTER(PhError)
+    L.ph_unet_param_storage.restype = ctypes.c_uint64
+    L.ph_unet_param_count.restype = ctypes.c_uint64
+    L.ph_unet_create.argtypes = [ctypes.c_int32, ctypes.c_int32, fp, ctypes.c_uint64,
+                                 ctypes.POINTER(vp), pe]
+    L.ph_unet_create.restype = ctypes.c_int
+    L.ph_unet_destroy.argtypes = [vp]
+    L.ph_unet_destroy.restype = None
+    L.ph_unet_forward.argtypes = [vp, vp, vp, ctypes.c_int32, vp, pe]
+    L.ph_unet_forward.restype = ctypes.c_int
+    L.ph_unet_train_step.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(PhTrainOpts),
+                                     ctypes.POINTER(ctypes.c_double), vp, pe]
+    L.ph_unet_train_step.restype = ctypes.c_int
+    L.ph_unet_get_state.argtypes = [vp, fp, fp, fp, fp, ctypes.POINTER(ctypes.c_int32), pe]
+    L.ph_unet_get_state.restype = ctypes.c_int
+    L.ph_unet_set_state.argtypes = [vp, fp, fp, fp, ctypes.c_int32, pe]
+    L.ph_unet_set_state.restype = ctypes.c_int
+    L.ph_warm_start_set_weights.argtypes = [fp, ctypes.c_uint64, pe]
+    L.ph_warm_start_set_weights.restype = ctypes.c_int
+    L.ph_warm_coarsen_device.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, vp, vp, vp, vp, pe]
+    L.ph_warm_coarsen_device.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -342,3 +388,103 @@ def poisson_solve_batch_device(desc: PhSolveDesc, items, stream: int = 0,
                                              ctypes.byref(err))
     _check(st, err)
     return [_result(reps[i], hists[i]) for i in range(n)]
+
+
+# ---------------------------------------------------------------------------
+# learned warm start
+
+def _fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def unet_param_storage() -> int:
+    """FP32 values of the network's parameter storage (padding included)."""
+    return int(lib().ph_unet_param_storage())
+
+
+def unet_param_count() -> int:
+    """Learnable values of the network (padding excluded)."""
+    return int(lib().ph_unet_param_count())
+
+
+class UNet:
+    """The warm-start network with its training state (device-resident)."""
+
+    def __init__(self, max_batch: int, res: int, params: np.ndarray | None = None, seed: int = 0):
+        self.max_batch, self.res = int(max_batch), int(res)
+        h = ctypes.c_void_p()
+        err = PhError()
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        st = lib().ph_unet_create(self.max_batch, self.res, _fptr(p), seed & 0xFFFFFFFFFFFFFFFF,
+                                  ctypes.byref(h), ctypes.byref(err))
+        _check(st, err)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().ph_unet_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, d_in: int, d_out: int, batch: int, stream: int = 0):
+        err = PhError()
+        _check(lib().ph_unet_forward(self._h, d_in, d_out, batch, stream or None,
+                                     ctypes.byref(err)), err)
+
+    def train_step(self, d_in: int, d_target: int, d_weight: int, batch: int, lr: float,
+                   beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0, update=True,
+                   stream: int = 0) -> float:
+        o = PhTrainOpts(lr, beta1, beta2, eps, weight_decay, 1 if update else 0)
+        loss = ctypes.c_double(0.0)
+        err = PhError()
+        _check(lib().ph_unet_train_step(self._h, d_in, d_target, d_weight, batch, ctypes.byref(o),
+                                        ctypes.byref(loss), stream or None, ctypes.byref(err)),
+               err)
+        return float(loss.value)
+
+    def state(self):
+        """(params, grads, adam_m, adam_v, step) as host arrays."""
+        n = unet_param_storage()
+        arrs = [np.zeros(n, np.float32) for _ in range(4)]
+        step = ctypes.c_int32(0)
+        err = PhError()
+        _check(lib().ph_unet_get_state(self._h, *(_fptr(a) for a in arrs), ctypes.byref(step),
+                                       ctypes.byref(err)), err)
+        return (*arrs, int(step.value))
+
+    def set_state(self, params, adam_m=None, adam_v=None, step: int = 0):
+        err = PhError()
+        conv = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+                for a in (params, adam_m, adam_v)]
+        _check(lib().ph_unet_set_state(self._h, *(_fptr(a) for a in conv), int(step),
+                                       ctypes.byref(err)), err)
+
+
+def warm_start_set_weights(params: np.ndarray | None) -> None:
+    """Set (or clear, with None) the weights of the solver's learned warm start."""
+    err = PhError()
+    if params is None:
+        st = lib().ph_warm_start_set_weights(None, 0, ctypes.byref(err))
+    else:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        st = lib().ph_warm_start_set_weights(_fptr(p), p.size, ctypes.byref(err))
+    _check(st, err)
+
+
+def warm_coarsen_device(d_source, d_anchor, d_mask, d_solution, width, height, channels, channel,
+                        lam, x0, y0, factor, res, d_in, d_target=None, d_weight=None,
+                        stream: int = 0):
+    err = PhError()
+    st = lib().ph_warm_coarsen_device(d_source, d_anchor, d_mask, d_solution or None, width,
+                                      height, channels, channel, lam, x0, y0, factor, res, d_in,
+                                      d_target or None, d_weight or None, stream or None,
+                                      ctypes.byref(err))
+    _check(st, err)

@@ -1,0 +1,195 @@
+"""The warm-start network's HIP kernels against the CPU reference
+(oracle/unet_ref.py, float64 with the GPU's bf16 storage points emulated):
+forward output, loss and every layer's weight gradient; determinism of a
+training step; the AdamW update; a short training run that must fit a fixed
+batch; the coarsening kernel against numpy; and the solver's fallback to the
+plain start when the network's start is worse (injected bad weights)."""
+import numpy as np
+import pytest
+
+from paintop_amd import hip
+from oracle import unet_ref as U
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    from paintop_amd.device import DeviceArray
+
+    return DeviceArray.from_numpy(np.ascontiguousarray(a, dtype=np.float32))
+
+
+def _case(B, R, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((B, R, R, 4)).astype(np.float32)
+    x[..., 3] = 0
+    y = rng.standard_normal((B, R, R)).astype(np.float32)
+    w = rng.random((B, R, R)).astype(np.float32)
+    return x, y, w
+
+
+@pytest.mark.parametrize("B,R", [(2, 32), (1, 64)])
+def test_forward_and_gradients_match_cpu_reference(gpu, B, R):
+    from paintop_amd.device import DeviceArray
+
+    p = U.init_params(7)
+    x, y, w = _case(B, R, 5)
+    net = hip.UNet(B, R, p)
+    dx, dy, dw = _dev(x), _dev(y), _dev(w)
+    dout = DeviceArray((B, R, R), np.float32)
+    net.forward(dx.ptr, dout.ptr, B)
+    out = dout.numpy()
+    ref, _ = U.forward(p, x, rnd=True)
+    ferr = float(np.abs(out - ref).max() / np.abs(ref).max())
+    loss = net.train_step(dx.ptr, dy.ptr, dw.ptr, B, 0.0, update=False)
+    _, g, _, _, step = net.state()
+    loss_ref, g_ref, _ = U.loss_and_grads(p, x, y, w, rnd=True)
+    print(f"[unet] B={B} R={R}: forward max rel err {ferr:.2e}, loss {loss:.6f} "
+          f"ref {loss_ref:.6f}", flush=True)
+    assert step == 0
+    assert ferr < 1e-2
+    assert abs(loss - loss_ref) < 1e-2 * loss_ref
+    lay, hoff, total = U.layout()
+    worst = 0.0
+    for li, (ow, ob, cout, cp, kp) in enumerate(lay):
+        for a, b_ in ((ow, ob), (ob, ob + cout)):
+            e = np.linalg.norm(g[a:b_] - g_ref[a:b_]) / max(np.linalg.norm(g_ref[a:b_]), 1e-30)
+            worst = max(worst, e)
+            assert e < 3e-2, (li, a == ow, e)
+        # padding slots stay exactly zero
+        gl = g[ow:ow + cout * kp].reshape(cout, kp)
+        assert np.all(gl[:, 9 * cp:] == 0)
+    eh = np.linalg.norm(g[hoff:] - g_ref[hoff:]) / np.linalg.norm(g_ref[hoff:])
+    print(f"[unet] worst per-layer gradient rel err {worst:.2e}, head {eh:.2e}", flush=True)
+    assert eh < 1e-2
+    net.close()
+
+
+def test_training_step_is_deterministic_and_adam_is_exact(gpu):
+    B, R = 2, 32
+    p = U.init_params(9)
+    x, y, w = _case(B, R, 6)
+    dx, dy, dw = _dev(x), _dev(y), _dev(w)
+    a, b = hip.UNet(B, R, p), hip.UNet(B, R, p)
+    la = a.train_step(dx.ptr, dy.ptr, dw.ptr, B, 1e-3, weight_decay=1e-4)
+    lb = b.train_step(dx.ptr, dy.ptr, dw.ptr, B, 1e-3, weight_decay=1e-4)
+    pa, ga, ma, va, sa = a.state()
+    pb, gb, mb, vb, sb = b.state()
+    assert la == lb and sa == sb == 1
+    assert pa.tobytes() == pb.tobytes() and ga.tobytes() == gb.tobytes()
+    # first AdamW step from zero moments: m = (1-b1) g, v = (1-b2) g^2
+    g = ga.astype(np.float64)
+    p64 = p.astype(np.float64)
+    mh = g  # bias-corrected
+    vh = g * g
+    expect = p64 - 1e-3 * (mh / (np.sqrt(vh) + 1e-8) + 1e-4 * p64)
+    assert np.allclose(pa, expect, rtol=0, atol=2e-6)
+    assert np.allclose(ma, 0.1 * g, rtol=1e-6, atol=1e-12)
+    # padding stays zero through the update
+    lay, _, _ = U.layout()
+    ow, ob, cout, cp, kp = lay[0]
+    w0 = pa[ow:ow + cout * kp].reshape(cout, kp)
+    assert np.all(w0[:, 9 * cp:] == 0) and np.all(w0[:, :9 * cp].reshape(cout, 9, cp)[:, :, 3] == 0)
+    a.close()
+    b.close()
+
+
+def test_training_fits_a_fixed_batch(gpu):
+    """A smooth target (a fixed linear map of smoothed inputs) is fitted: the
+    loss falls at least 20-fold in 300 AdamW steps."""
+    B, R = 4, 32
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((B, R, R, 4)).astype(np.float32)
+    x[..., 3] = 0
+    from scipy.ndimage import uniform_filter
+
+    y = np.stack([uniform_filter(x[b, ..., 1], 5) + 0.5 * x[b, ..., 2] for b in range(B)])
+    y = y.astype(np.float32)
+    w = np.ones((B, R, R), np.float32)
+    dx, dy, dw = _dev(x), _dev(y), _dev(w)
+    net = hip.UNet(B, R, seed=3)
+    l0 = net.train_step(dx.ptr, dy.ptr, dw.ptr, B, 0.0, update=False)
+    for _ in range(300):
+        loss = net.train_step(dx.ptr, dy.ptr, dw.ptr, B, 2e-3)
+    l1 = net.train_step(dx.ptr, dy.ptr, dw.ptr, B, 0.0, update=False)
+    print(f"[unet] fit: loss {l0:.4e} -> {l1:.4e}", flush=True)
+    assert l1 < l0 / 20
+    net.close()
+
+
+def test_coarsen_matches_numpy(gpu):
+    from paintop_amd.device import DeviceArray
+    from tests.helpers import classify
+
+    rng = np.random.default_rng(2)
+    H, W, C, R, f = 70, 90, 3, 16, 5
+    s = rng.random((H, W, C)).astype(np.float32)
+    t = rng.random((H, W, C)).astype(np.float32)
+    u = rng.random((H, W, C)).astype(np.float32)
+    m = (rng.random((H, W)) > 0.4).astype(np.float32)
+    x0, y0, ch, lam = -7, -3, 1, 0.25
+    ds, dt, dm, du = _dev(s), _dev(t), _dev(m), _dev(u)
+    din = DeviceArray((R, R, 4), np.float32)
+    dtg = DeviceArray((R, R), np.float32)
+    dwt = DeviceArray((R, R), np.float32)
+    gpu.warm_coarsen_device(ds.ptr, dt.ptr, dm.ptr, du.ptr, W, H, C, ch, lam, x0, y0, f, R,
+                            din.ptr, dtg.ptr, dwt.ptr)
+    gi, gt, gw = din.numpy(), dtg.numpy(), dwt.numpy()
+    inside = classify(m).astype(bool)
+    P, A, S = s[..., ch].astype(np.float64), t[..., ch].astype(np.float64), u[..., ch].astype(np.float64)
+    for cy in range(R):
+        for cx in range(R):
+            ys = slice(max(0, y0 + cy * f), max(0, min(H, y0 + cy * f + f)))
+            xs = slice(max(0, x0 + cx * f), max(0, min(W, x0 + cx * f + f)))
+            ins = inside[ys, xs]
+            n_img = ins.size
+            rhs = np.sum(np.where(ins, lam * (P[ys, xs] - A[ys, xs]), 0.0))
+            bnd = np.sum(np.where(ins, 0.0, A[ys, xs] - P[ys, xs])) / f ** 2
+            tg = np.sum(np.where(ins, S[ys, xs] - P[ys, xs], A[ys, xs] - P[ys, xs]))
+            exp = [rhs, bnd, ins.sum() / f ** 2, 0.0]
+            assert np.allclose(gi[cy, cx], exp, rtol=1e-5, atol=1e-6), (cy, cx, gi[cy, cx], exp)
+            assert np.isclose(gt[cy, cx], tg / n_img if n_img else 0.0, rtol=1e-5, atol=1e-6)
+            assert np.isclose(gw[cy, cx], n_img / f ** 2, rtol=1e-6)
+
+
+def test_bad_weights_fall_back_to_the_plain_start(gpu):
+    """Injected bad weights: the network's start is worse than the anchor
+    start, so the solver keeps the plain start and the result is bit-identical
+    to the plain solve."""
+    from paintop_amd.device import DeviceArray
+    from paintop_amd.inputs import batch_case
+
+    cases = batch_case(3, seed0=950)
+    bad = U.init_params(5) * 40.0
+    gpu.warm_start_set_weights(bad)
+    try:
+        for (w, h, s, t, m) in cases:
+            o1 = DeviceArray((h, w), np.float32)
+            o2 = DeviceArray((h, w), np.float32)
+            d = hip.make_desc(w, h, 1, 20000, 1e-6)
+            r1 = gpu.poisson_solve_device(d, s.ptr, t.ptr, m.ptr, o1.ptr)
+            d.flags = hip.PH_FLAG_WARM_START_NET
+            r2 = gpu.poisson_solve_device(d, s.ptr, t.ptr, m.ptr, o2.ptr)
+            print(f"[unet] fallback {w}x{h}: warm rel {r2.warm_start_rel:.3e}, "
+                  f"kept {r2.warm_start_used}", flush=True)
+            assert r2.warm_start_used == 0 and r2.warm_start_rel > 1.0
+            assert o1.numpy().tobytes() == o2.numpy().tobytes()
+            assert r1.residual_history == r2.residual_history
+            # FORCE_PLAIN wins over the warm-start flag
+            d.flags = hip.PH_FLAG_WARM_START_NET | hip.PH_FLAG_FORCE_PLAIN
+            r3 = gpu.poisson_solve_device(d, s.ptr, t.ptr, m.ptr, o2.ptr)
+            assert r3.warm_start_rel == 0.0 and r3.residual_history == r1.residual_history
+    finally:
+        gpu.warm_start_set_weights(None)
+
+
+def test_warm_start_without_weights_is_an_error(gpu):
+    from paintop_amd.device import DeviceArray
+    from paintop_amd.inputs import batch_case
+
+    (w, h, s, t, m), = batch_case(1, seed0=960)
+    o = DeviceArray((h, w), np.float32)
+    d = hip.make_desc(w, h, 1, 20000, 1e-6, flags=hip.PH_FLAG_WARM_START_NET)
+    with pytest.raises(hip.HipCallError) as e:
+        gpu.poisson_solve_device(d, s.ptr, t.ptr, m.ptr, o.ptr)
+    assert e.value.code == "E_HIP_WARM_START_UNSET" and e.value.error_class == "policy"
