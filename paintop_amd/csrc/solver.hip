@@ -299,6 +299,16 @@ static uint32_t warm_ramp_sweeps(double omega) {
   return (uint32_t)std::min(1e5, std::ceil(fct / (2.0 - omega)));
 }
 
+// Sweeps of the warm-start race before the FP64 judgement: a quarter of the
+// ramp, within [8, 96] (always shorter than the warm member's stall window,
+// which is the tuned window plus the ramp).
+static uint32_t warm_probe_sweeps(uint32_t ramp) {
+  return std::min<uint32_t>(96, std::max<uint32_t>(8, ramp / 4));
+}
+// A warm start whose initial residual exceeds the anchor start's this many
+// times is dropped without a race.
+static const double WARM_MAX_START_RATIO = 1e6;
+
 static uint32_t stall_window_for(double omega, bool tuned) {
   if (!tuned) return PH_STALL_WINDOW;
   const double w = ceil(3.0 / std::max(2.0 - omega, 1e-6));
@@ -479,6 +489,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       const size_t R2 = (size_t)warm_net_res() * warm_net_res();
       size_t nch = 0;
       for (const ItemIn &it : items) nch += it.C;
+      // the plain twins of the warm-start race
+      for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
       need += (R2 * 4 * warm_net_batch() + R2 * nch) * sizeof(float) + 1024;
     }
     g_pool.reset(need);
@@ -647,11 +659,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       inst.push_back(X);
     }
   }
-  const int ninst = (int)inst.size();
-  double *resv_d = g_pool.take<double>(2 * (size_t)std::max(ninst, 1));
-  uint32_t *arr_d = g_pool.take<uint32_t>((size_t)std::max(ninst, 1));
-  HIP_OK(hipMemsetAsync(arr_d, 0, std::max(ninst, 1) * sizeof(uint32_t), st));
-  std::vector<double> resv(2 * (size_t)std::max(ninst, 1));
+  const int ninst = (int)inst.size();  // the images' instances (twins come after)
+  const int ncap = std::max(want_warm ? 2 * ninst : ninst, 1);
+  double *resv_d = g_pool.take<double>(2 * (size_t)ncap);
+  uint32_t *arr_d = g_pool.take<uint32_t>((size_t)ncap);
+  HIP_OK(hipMemsetAsync(arr_d, 0, ncap * sizeof(uint32_t), st));
+  std::vector<double> resv(2 * (size_t)ncap);
   auto launch_residual = [&](int k) {
     const InstWork &X = inst[k];
     hipLaunchKernelGGL(k_residual, dim3(X.I.ph), dim3(256), 0, st, X.I, X.rowsum, arr_d + k,
@@ -689,10 +702,24 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   }
 
   // --- 5b. learned warm start: x = P + bilinear(net(coarse inputs)) at the
-  // interior, kept only when its FP64 residual is no worse than the anchor
-  // start's; the relative residual keeps the anchor denominator r0.
-  std::vector<double> rstart(ninst, 0.0);
-  std::vector<char> warm_kept(ninst, 0);
+  // interior cells of instance k, raced against a plain twin (x = anchor) that
+  // runs the same first sweeps in the same launches. At the probe sweep the
+  // solver's own FP64 residuals decide: the smaller one continues, the other
+  // is dropped; when the plain twin wins, its state replaces the warm one, so
+  // the result is the plain solve's, bit for bit. Both keep the anchor-start
+  // denominator r0 (poisson.rs:172).
+  std::vector<double> rstart;
+  std::vector<int> twin, probe;
+  std::vector<char> is_twin, silent;
+  auto grow = [&]() {
+    const size_t n = inst.size();
+    rstart.resize(n, 0.0);
+    twin.resize(n, -1);
+    probe.resize(n, 0);
+    is_twin.resize(n, 0);
+    silent.resize(n, 0);
+  };
+  grow();
   for (int k = 0; k < ninst; ++k) rstart[k] = inst[k].r0;
   if (want_warm && d->lambda == 0.0 && !act.empty()) {
     const int R = warm_net_res(), NB = warm_net_batch();
@@ -721,37 +748,60 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       warm_net_forward(in_d, h_d + R2 * j0, nb, st);
     }
     for (int j = 0; j < nw; ++j) {
-      const InstWork &X = inst[act[j]];
+      const int k = act[j];
+      const InstWork &X = inst[k];
       const Geo &g = geo[j];
       hipLaunchKernelGGL(k_warm_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.src,
                          h_d + R2 * j, R, g.x0, g.y0, g.f);
       HIP_OK(hipGetLastError());
-      launch_residual(act[j]);
+      launch_residual(k);
     }
     HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
     for (int j = 0; j < nw; ++j) {
       const int k = act[j];
-      InstWork &X = inst[k];
-      const ItemWork &w = iw[X.item];
+      const ItemWork &w = iw[inst[k].item];
       const double rw = resv[2 * k];
       ph_report *rep = w.in.rep;
-      rep->warm_start_rel = std::max(rep->warm_start_rel, std::isfinite(rw) ? rw / X.r0 : INFINITY);
-      if (std::isfinite(rw) && rw <= X.r0) {
-        warm_kept[k] = 1;
-        rstart[k] = rw;
-        rep->warm_start_used += 1;
-        const uint32_t n = warm_ramp_sweeps(w.omega);
-        X.I.ramp_sweeps = n;
-        X.I.ramp_log = n ? log(2.0 - w.omega) / n : 0.0;
-      } else {
-        // fall back to the plain start: x = anchor again, g = -r(anchor)
-        hipLaunchKernelGGL(k_assemble, X.pgrd, dim3(256), 0, st, X.I, w.in.src, w.in.tgt,
-                           w.cls_img, w.cls_plane);
+      const double r0 = inst[k].r0;
+      rep->warm_start_rel = std::max(rep->warm_start_rel, std::isfinite(rw) ? rw / r0 : INFINITY);
+      if (!(std::isfinite(rw) && rw <= WARM_MAX_START_RATIO * r0)) {
+        // no race for a start this far off: back to the anchor start at once
+        hipLaunchKernelGGL(k_assemble, inst[k].pgrd, dim3(256), 0, st, inst[k].I, w.in.src,
+                           w.in.tgt, w.cls_img, w.cls_plane);
         HIP_OK(hipGetLastError());
         launch_residual(k);
+        continue;
       }
+      // the plain twin: its own planes, the anchor start
+      InstWork T = inst[k];
+      const size_t plane = (size_t)w.pitch * w.phh;
+      T.I.e[0] = g_pool.take<float>(plane);
+      T.I.e[1] = g_pool.take<float>(plane);
+      T.I.g = g_pool.take<float>(plane);
+      T.I.x = g_pool.take<double>(plane);
+      T.I.bp = g_pool.take<double>(plane);
+      T.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
+      T.rowsum = g_pool.take<double>(w.phh);
+      hipLaunchKernelGGL(k_assemble, T.pgrd, dim3(256), 0, st, T.I, w.in.src, w.in.tgt,
+                         w.cls_img, w.cls_plane);
+      HIP_OK(hipGetLastError());
+      inst.push_back(T);
+      const int kt = (int)inst.size() - 1;
+      grow();
+      launch_residual(kt);
+      InstWork &Xw = inst[k];
+      const uint32_t n = warm_ramp_sweeps(w.omega);
+      Xw.I.ramp_sweeps = n;
+      Xw.I.ramp_log = n ? log(2.0 - w.omega) / n : 0.0;
+      twin[k] = kt;
+      twin[kt] = k;
+      is_twin[kt] = 1;
+      rstart[k] = rw;
+      rstart[kt] = r0;
+      probe[k] = probe[kt] = (int)std::min<uint32_t>(warm_probe_sweeps(n), d->max_iterations);
+      act.push_back(kt);
     }
   }
 
@@ -799,7 +849,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       Cc.best = INFINITY;
       Cc.history = X.hist;
       Cc.max_it = d->max_iterations;
-      Cc.sweep_limit = d->max_iterations;
+      Cc.sweep_limit = probe[act[a]] ? (uint32_t)probe[act[a]] : d->max_iterations;
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
       // the stall guard's window starts after a warm start's omega ramp
@@ -838,6 +888,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       Cc.refine_abs2 = (REFINE_RATIO * gnorm) * (REFINE_RATIO * gnorm);
     };
 
+    std::vector<char> held(nact, 0), hold_refine(nact, 0), resolved(inst.size(), 0);
+    std::vector<double> pval(inst.size(), -1.0);
+    std::vector<int> slot(inst.size(), -1);
+    for (int a = 0; a < nact; ++a) slot[act[a]] = a;
     const int PF = prefetch_rows(K);
     int batch = 4;
     int remaining = nact;
@@ -882,7 +936,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       pend.clear();
       for (int a = 0; a < nact; ++a) {
         PhCtl &Cc = ctl[a];
-        if (inst[act[a]].R.done) continue;
+        if (inst[act[a]].R.done || held[a]) continue;
         switch (Cc.halt) {
           case PH_RUN:
             break;
@@ -902,6 +956,17 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
             pend.push_back({a, Cc.halt_sweep, (int)Cc.reason});
             break;
           case PH_HALT_LIMIT:
+            if (twin[act[a]] >= 0 && !resolved[act[a]] && Cc.sweeps_done < Cc.max_it) {
+              // warm-start race probe: judged on the device's own residual of
+              // the last finalised sweep, with no refinement, so neither
+              // member's trajectory changes
+              held[a] = 1;
+              hold_refine[a] = Cc.refine_pending ? 1 : 0;
+              pval[act[a]] = Cc.last_rel * Cc.r0;
+              break;
+            }
+            pend.push_back({a, Cc.sweeps_done, -1});
+            break;
           case PH_HALT_REFINE:
             pend.push_back({a, Cc.sweeps_done, -1});
             break;
@@ -933,12 +998,46 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                                 hipMemcpyHostToDevice, st));
           Cc.sweeps_final = P.m;
           Cc.sweeps_done = P.m;
+          const int k = act[P.a];
+          const bool paired = twin[k] >= 0 && !resolved[k];
           if (rel <= Cc.tol) finish(P.a, PH_STOP_CONVERGED, P.m, rel);
           else if (P.reason == PH_REASON_STALLED) finish(P.a, PH_STOP_STALLED, P.m, rel);
           else if (stall_step(Cc, rel)) finish(P.a, PH_STOP_STALLED, P.m, rel);
           else if (P.m >= Cc.max_it) finish(P.a, PH_STOP_MAX_ITERATIONS, P.m, rel);
           else resume(P.a, gnorm);
-          if (inst[act[P.a]].R.done) --remaining;
+          if (inst[k].R.done) {
+            --remaining;
+            if (paired) pval[k] = gnorm;
+          }
+        }
+      }
+      {
+        // warm-start races whose two members both have a value: the smaller
+        // residual continues (a tie goes to the plain start)
+        for (int k = 0; k < (int)inst.size(); ++k) {
+          if (is_twin[k] || twin[k] < 0 || resolved[k]) continue;
+          const int kt = twin[k];
+          if (pval[k] < 0.0 || pval[kt] < 0.0) continue;
+          const bool warm_wins = pval[k] < pval[kt];
+          const int win = warm_wins ? k : kt, lose = warm_wins ? kt : k;
+          resolved[k] = resolved[kt] = 1;
+          silent[lose] = 1;
+          if (!inst[lose].R.done) {
+            inst[lose].R.done = true;
+            inst[lose].R.iterations = ctl[slot[lose]].sweeps_done;
+            ctl[slot[lose]].halt = PH_HALT_DONE;
+            --remaining;
+          }
+          if (held[slot[win]]) {
+            const int a = slot[win];
+            held[a] = 0;
+            PhCtl &Cc = ctl[a];
+            Cc.sweep_limit = Cc.max_it;
+            // a refinement that was due at the probe launch happens now, as it
+            // would have without the probe
+            Cc.halt = hold_refine[a] ? PH_HALT_REFINE : PH_RUN;
+          }
+          if (warm_wins) iw[inst[k].item].in.rep->warm_start_used += 1;
         }
       }
       HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
@@ -961,7 +1060,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         uint32_t need = 0;
         for (int a = 0; a < nact; ++a) {
           const PhCtl &Cc = ctl[a];
-          if (inst[act[a]].R.done || Cc.halt != PH_RUN) continue;
+          if (inst[act[a]].R.done || Cc.halt != PH_RUN || held[a]) continue;
           uint32_t next = std::min(Cc.sweep_limit, Cc.last_refine + Cc.refine_every);
           if (Cc.mode == PH_MODE_REDO) next = Cc.sweep_limit;
           uint32_t todo = next > Cc.sweeps_done ? next - Cc.sweeps_done : 1;
@@ -1001,10 +1100,29 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       hipEventDestroy(eb1);
     }
     for (int a = 0; a < nact; ++a) {
-      const InstWork &X = inst[act[a]];
+      const int k = act[a];
+      InstWork &X = inst[k];
+      // every sweep run counts as work, the dropped member's included
+      iw[X.item].in.rep->unknown_sweeps += (uint64_t)iw[X.item].n_interior * X.R.iterations;
+      if (is_twin[k] || silent[k]) continue;
+      if (twin[k] >= 0 && silent[k] == 0 && resolved[k] == 0)
+        throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "warm-start race left unresolved");
+    }
+    for (int k = 0; k < ninst; ++k) {
+      if (twin[k] >= 0 && silent[k]) {  // the plain twin won: its result is the image's
+        InstWork &X = inst[k];
+        const InstWork &T = inst[twin[k]];
+        X.I = T.I;
+        X.R = T.R;
+        X.hist = T.hist;
+      }
+    }
+    for (int a = 0; a < nact; ++a) {
+      const int k = act[a];
+      if (k >= ninst || !inst[k].R.active) continue;
+      const InstWork &X = inst[k];
       hipLaunchKernelGGL(k_scatter, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.out);
       HIP_OK(hipGetLastError());
-      iw[X.item].in.rep->unknown_sweeps += (uint64_t)iw[X.item].n_interior * X.R.iterations;
     }
     for (int i = 0; i < nitem; ++i) {
       iw[i].in.rep->launches = total_launches;

@@ -87,6 +87,11 @@ __device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
       C.halt_sweep = m;
       return true;
     }
+    // Half a window without progress: the FP32 correction may have reached
+    // its rounding floor (~ 8 ulp(|e|) / (2 - omega) in residual terms) before
+    // the 1e3-fold trigger; an FP64 refinement restarts it from e = 0 so a
+    // real stall, not the FP32 floor, is what the stall rule sees.
+    if (C.since == (C.stall_window >> 1)) C.refine_pending = 1;
   }
   if (res2 <= C.refine_abs2) C.refine_pending = 1;
   return false;

@@ -153,6 +153,7 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
     Y = np.zeros((n, RES, RES), np.float32)
     Wt = np.zeros((n, RES, RES), np.float32)
     sweeps = np.zeros(n, np.int64)
+    dropped = []
     desc = hip.make_desc(1, 1, 1, 100000, tol)
     t0 = time.time()
     for c0 in range(0, n, chunk):
@@ -190,9 +191,14 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
                 a.free()
         for a in (din, dt, dw):
             a.free()
-        if not all(r.converged for r in reps):
-            bad = [first + c0 + j for j, r in enumerate(reps) if not r.converged]
-            raise RuntimeError(f"training instances {bad[:8]} did not converge")
+        for j, r in enumerate(reps):
+            # a solve that ends (stalled) within 10x the tolerance is still a
+            # clean target; anything else is dropped (zero loss weight)
+            if not (r.converged or r.final_residual <= 10 * tol):
+                Wt[c0 + j] = 0.0
+                dropped.append((first + c0 + j, r.stop_reason, r.final_residual))
+                log(f"[warmstart] instance {first + c0 + j} dropped: {r.stop_reason} at "
+                    f"{r.final_residual:.2e} after {r.iterations} sweeps")
         log(f"[warmstart] pairs {min(n, c0 + chunk)}/{n}  {time.time() - t0:.1f}s  "
             f"sweeps median {int(np.median(sweeps[:c0 + len(specs)]))}")
     return X, Y, Wt, sweeps
