@@ -158,6 +158,18 @@ ph_status ph_poisson_solve_device(const ph_solve_desc *desc, const float *d_sour
                                   float *d_out_candidate, void *stream, ph_report *report,
                                   ph_error *err);
 
+/* ph_poisson_solve_device from a caller's initial field d_start (same layout
+ * as d_out_candidate; only its interior samples are read), e.g. the previous
+ * frame's solution. No reference counterpart (the reference starts from the
+ * anchor, poisson.rs:169-170). The start races a plain (anchor-start) twin
+ * for a short probe and the smaller residual continues, exactly as the
+ * learned warm start does; the relative residual keeps the anchor-start
+ * denominator (poisson.rs:172). */
+ph_status ph_poisson_solve_start_device(const ph_solve_desc *desc, const float *d_source,
+                                        const float *d_target, const float *d_mask,
+                                        const float *d_start, float *d_out_candidate,
+                                        void *stream, ph_report *report, ph_error *err);
+
 /* One image of a batched solve: device buffers in the ResourceValue layout. */
 typedef struct ph_batch_item {
   uint32_t width;
@@ -218,8 +230,9 @@ ph_status ph_ellipse_mask_device(float *d_out, uint32_t width, uint32_t height, 
  * The network: a 4-level U-Net (3x3 convolutions, channels 16/32/64/64, BF16
  * weights and activations, FP32 accumulation on the matrix cores, FP32 1x1
  * head) on a res x res grid, input [batch][res][res][4] FP32 (channel 0: the
- * right-hand side of the correction problem, 1: boundary values with 0 in the
- * interior, 2: interior fraction, 3: zero), output [batch][res][res] FP32.
+ * right-hand side of the correction problem, 1: boundary values (0 off the
+ * Dirichlet ring), 2: interior fraction, 3: ring density), output
+ * [batch][res][res] FP32.
  * Parameters are ph_unet_param_storage() FP32 values in the layout
  * documented in paintop_amd/csrc/unet_internal.h (padding slots must be 0). */
 
@@ -235,6 +248,7 @@ typedef struct ph_train_opts {
   float beta1, beta2, epsilon; /* Adam */
   float weight_decay;          /* decoupled (AdamW) */
   int32_t apply_update;        /* 0: compute loss and gradients only */
+  float pyramid_weight;        /* alpha of the multi-scale term (0: weighted MSE) */
 } ph_train_opts;
 
 /* Buffers for batches up to max_batch at resolution res (a multiple of 8 in
@@ -247,9 +261,12 @@ void ph_unet_destroy(ph_unet *net);
 ph_status ph_unet_forward(ph_unet *net, const float *d_in, float *d_out, int32_t batch,
                           void *stream, ph_error *err);
 
-/* One training step: forward, loss = sum_p w[p] (out[p] - target[p])^2 / n,
- * backward, and (if opts->apply_update) an AdamW update. *loss (host) gets
- * the loss before the update. */
+/* One training step: forward, loss, backward, and (if opts->apply_update) an
+ * AdamW update. loss = (1/n) sum_p [w[p] d[p]^2 + alpha sum_s m_s(p)^2] with
+ * d = out - target and m_s(p) the mean of w d over the 2x2, 4x4, 8x8 and
+ * 16x16 block holding p (alpha = opts->pyramid_weight; res % 16 == 0 when
+ * alpha > 0); the multi-scale term weighs the smooth error that costs SOR the
+ * most sweeps. *loss (host) gets the loss before the update. */
 ph_status ph_unet_train_step(ph_unet *net, const float *d_in, const float *d_target,
                              const float *d_weight, int32_t batch, const ph_train_opts *opts,
                              double *loss, void *stream, ph_error *err);
@@ -267,13 +284,15 @@ ph_status ph_warm_start_set_weights(const float *params, uint64_t n_params, ph_e
 
 /* Net input for one channel of an image (and, when d_solution is given, the
  * training target): the region [x0, x0 + res*f) x [y0, y0 + res*f) of the
- * image, averaged over f x f blocks. With interior = (mask > 0.5 and not on
- * the image border) (poisson.rs:340-355) and P = source:
- *   in[..][0] = f^2 * mean(interior ? lambda*(P - anchor) : 0)
- *   in[..][1] = mean(!interior and inside the image ? anchor - P : 0)
- *   in[..][2] = mean(interior)
- *   target    = mean over in-image cells of (solution - P)
- *   weight    = fraction of the block inside the image.
+ * image in f x f blocks. With interior = (mask > 0.5 and not on the image
+ * border) (poisson.rs:340-355), ring = non-interior cells with an interior
+ * 4-neighbour (the Dirichlet data) and P = source, per block:
+ *   in[..][0] = sum over interior cells of lambda*(P - anchor)
+ *   in[..][1] = mean over ring cells of (anchor - P), 0 without ring cells
+ *   in[..][2] = interior cells / f^2
+ *   in[..][3] = ring cells / f
+ *   target    = mean over interior and ring cells of (solution - P)
+ *   weight    = 1 with interior cells, 0.5 with ring cells only, else 0.
  * d_in is [res][res][4], d_target / d_weight [res][res]. */
 ph_status ph_warm_coarsen_device(const float *d_source, const float *d_anchor,
                                  const float *d_mask, const float *d_solution, uint32_t width,

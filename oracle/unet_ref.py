@@ -40,6 +40,7 @@ LAYERS = [
     (0, "D0A", None, 16, 0, 16, "D0"),
 ]
 POOLS = {"E0": "P0", "E1": "P1", "E2": "P2"}
+DIL = [1, 1, 1, 1, 1, 1, 2, 4, 1, 1, 1, 1, 1, 1]  # dilation of each layer's 3x3 taps
 HEAD_C = 16
 
 
@@ -61,9 +62,8 @@ def layout():
 
 def param_count() -> int:
     n = 0
-    for li, (_, _, _, ca, cb, cout, _) in enumerate(LAYERS):
-        cin = 3 if li == 0 else ca + cb
-        n += 9 * cin * cout + cout
+    for (_, _, _, ca, cb, cout, _) in LAYERS:
+        n += 9 * (ca + cb) * cout + cout
     return n + HEAD_C + 1
 
 
@@ -84,26 +84,29 @@ def _conv_weights(p, li, rnd):
     return w, p[ob:ob + cout].astype(np.float64)
 
 
-def _patches(x):
-    """x [B,H,W,C] -> [B,H,W,9,C], tap t = 3*(dy+1) + (dx+1), zero padding."""
+def _patches(x, dil=1):
+    """x [B,H,W,C] -> [B,H,W,9,C], tap t at offset dil*(t//3 - 1, t%3 - 1),
+    zero padding."""
     B, H, W, C = x.shape
-    xp = np.zeros((B, H + 2, W + 2, C))
-    xp[:, 1:-1, 1:-1] = x
+    q = dil
+    xp = np.zeros((B, H + 2 * q, W + 2 * q, C))
+    xp[:, q:q + H, q:q + W] = x
     out = np.empty((B, H, W, 9, C))
     for t in range(9):
-        dy, dx = t // 3 - 1, t % 3 - 1
-        out[:, :, :, t] = xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+        dy, dx = (t // 3 - 1) * dil, (t % 3 - 1) * dil
+        out[:, :, :, t] = xp[:, q + dy:q + dy + H, q + dx:q + dx + W]
     return out
 
 
-def _patches_adjoint(d, H, W):
+def _patches_adjoint(d, H, W, dil=1):
     """Adjoint of _patches: [B,H,W,9,C] -> [B,H,W,C]."""
     B, C = d.shape[0], d.shape[-1]
-    xp = np.zeros((B, H + 2, W + 2, C))
+    q = dil
+    xp = np.zeros((B, H + 2 * q, W + 2 * q, C))
     for t in range(9):
-        dy, dx = t // 3 - 1, t % 3 - 1
-        xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W] += d[:, :, :, t]
-    return xp[:, 1:-1, 1:-1]
+        dy, dx = (t // 3 - 1) * dil, (t % 3 - 1) * dil
+        xp[:, q + dy:q + dy + H, q + dx:q + dx + W] += d[:, :, :, t]
+    return xp[:, q:q + H, q:q + W]
 
 
 def _up(x):
@@ -134,7 +137,7 @@ def forward(params, x, rnd=False):
     for li, (_, a, b, ca, cb, cout, o) in enumerate(LAYERS):
         xin = acts[a] if b is None else np.concatenate([acts[a], _up(acts[b])], axis=-1)
         w, bias = _conv_weights(p, li, rnd)
-        P = _patches(xin)
+        P = _patches(xin, DIL[li])
         z = np.einsum("bhwtc,otc->bhwo", P, w) + bias
         acts[o] = r(np.maximum(z, 0.0))
         cache["inputs"][li] = (P, xin.shape)
@@ -148,16 +151,27 @@ def forward(params, x, rnd=False):
     return out, cache
 
 
-def loss_and_grads(params, x, target, weight, rnd=False):
-    """Loss sum(w (out - y)^2) / n and its gradient in the storage layout."""
+def _block_mean_up(e, s):
+    """Mean of e over 2^s x 2^s blocks, repeated back to full resolution."""
+    B, H, W = e.shape
+    b = 1 << s
+    m = e.reshape(B, H // b, b, W // b, b).mean(axis=(2, 4))
+    return np.repeat(np.repeat(m, b, axis=1), b, axis=2)
+
+
+def loss_and_grads(params, x, target, weight, rnd=False, pyramid=0.0):
+    """Loss (1/n) sum_p [w d^2 + alpha sum_s m_s(p)^2] (d = out - target, m_s
+    the block means of w d over 2x2 .. 16x16 blocks) and its gradient in the
+    storage layout."""
     p = np.asarray(params, dtype=np.float64)
     r = bf16 if rnd else (lambda a: np.asarray(a, dtype=np.float64))
     out, cache = forward(p, x, rnd)
     acts = cache["acts"]
     n = out.size
     diff = out - target
-    loss = float(np.sum(weight * diff * diff) / n)
-    dout = 2.0 * weight * diff / n
+    ms = [_block_mean_up(weight * diff, s) for s in range(1, 5)] if pyramid else []
+    loss = float((np.sum(weight * diff * diff) + pyramid * sum(np.sum(m * m) for m in ms)) / n)
+    dout = 2.0 * weight * (diff + pyramid * sum(ms)) / n
     lay, _, total = layout()
     g = np.zeros(total)
     hw = p[total - HEAD_C - 1:total - 1]
@@ -179,7 +193,7 @@ def loss_and_grads(params, x, target, weight, rnd=False):
         if li == 0:
             break
         w, _ = _conv_weights(p, li, rnd)
-        dxin = _patches_adjoint(np.einsum("bhwo,otc->bhwtc", dy, w), shp[1], shp[2])
+        dxin = _patches_adjoint(np.einsum("bhwo,otc->bhwtc", dy, w), shp[1], shp[2], DIL[li])
         if b is None:
             if a in ("P0", "P1", "P2"):
                 # input is a pooled skip tensor: route to the skip's gradient
@@ -205,11 +219,8 @@ def init_params(seed: int = 0) -> np.ndarray:
     lay, _, total = layout()
     p = np.zeros(total, dtype=np.float32)
     for li, (ow, ob, cout, cp, kp) in enumerate(lay):
-        cin = 3 if li == 0 else cp
         w = np.zeros((cout, kp), dtype=np.float32)
-        blk = rng.standard_normal((cout, 9, cp)) * np.sqrt(2.0 / (9 * cin))
-        if li == 0:
-            blk[:, :, 3] = 0.0
+        blk = rng.standard_normal((cout, 9, cp)) * np.sqrt(2.0 / (9 * cp))
         w[:, :9 * cp] = blk.reshape(cout, 9 * cp)
         p[ow:ow + cout * kp] = w.ravel()
         p[ob:ob + cout] = rng.standard_normal(cout) * 0.05

@@ -41,7 +41,7 @@ namespace ph {
 template <int K, int PF>
 __global__ void k_sor(const PhTile *, int, const PhInst *, const PhCtl *, double *, int);
 template <int K>
-__global__ void k_sor_control(const PhInst *, PhCtl *, const double *, int);
+__global__ void k_sor_control(const PhInst *, PhCtl *, const double *, int, int);
 __global__ void k_classify(const float *, int, int, uint8_t *, int *);
 __global__ void k_bbox(const int *, int, int *);
 __global__ void k_rowx_init(int *, int);
@@ -55,6 +55,7 @@ __global__ void k_scatter(PhInst, float *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
                           int, double, int, int, int, int, float *, float *, float *);
 __global__ void k_warm_apply(PhInst, const float *, const float *, int, int, int, int);
+__global__ void k_start_apply(PhInst, const float *);
 bool warm_net_ready();
 int warm_net_res();
 int warm_net_batch();
@@ -293,7 +294,7 @@ static double omega_for(double mu, double lambda) {
 // Sweeps of the omega ramp after a kept warm start: a multiple of 1/(2 - omega),
 // which grows with the domain's size like the SOR transient itself.
 static uint32_t warm_ramp_sweeps(double omega) {
-  double fct = 2.5;
+  double fct = 1.0;  // measured best for the shipped network (2048^2 and 4096^2 blends)
   if (const char *e = getenv("PAINTOP_HIP_RAMP_FACTOR")) fct = atof(e);
   if (!(fct >= 0.0)) fct = 0.0;
   return (uint32_t)std::min(1e5, std::ceil(fct / (2.0 - omega)));
@@ -339,13 +340,91 @@ static void launch_sor(int K, int PF, int ntiles, int ninst, hipStream_t st, con
 #undef PH_SOR
   HIP_OK(hipGetLastError());
   switch (K) {
-    case 2: hipLaunchKernelGGL(k_sor_control<2>, cgrid, cblock, 0, st, insts, ctl, partials, ninst); break;
-    case 4: hipLaunchKernelGGL(k_sor_control<4>, cgrid, cblock, 0, st, insts, ctl, partials, ninst); break;
-    case 8: hipLaunchKernelGGL(k_sor_control<8>, cgrid, cblock, 0, st, insts, ctl, partials, ninst); break;
-    case 12: hipLaunchKernelGGL(k_sor_control<12>, cgrid, cblock, 0, st, insts, ctl, partials, ninst); break;
-    default: hipLaunchKernelGGL(k_sor_control<16>, cgrid, cblock, 0, st, insts, ctl, partials, ninst); break;
+    case 2: hipLaunchKernelGGL(k_sor_control<2>, cgrid, cblock, 0, st, insts, ctl, partials, ninst, ntiles); break;
+    case 4: hipLaunchKernelGGL(k_sor_control<4>, cgrid, cblock, 0, st, insts, ctl, partials, ninst, ntiles); break;
+    case 8: hipLaunchKernelGGL(k_sor_control<8>, cgrid, cblock, 0, st, insts, ctl, partials, ninst, ntiles); break;
+    case 12: hipLaunchKernelGGL(k_sor_control<12>, cgrid, cblock, 0, st, insts, ctl, partials, ninst, ntiles); break;
+    default: hipLaunchKernelGGL(k_sor_control<16>, cgrid, cblock, 0, st, insts, ctl, partials, ninst, ntiles); break;
   }
   HIP_OK(hipGetLastError());
+}
+
+// Sweep batches as hipGraphs: a batch of n launch pairs (k_sor +
+// k_sor_control) is replayed from a graph of 2^j pairs per set bit of n,
+// captured once per kernel-argument set on a private stream (default
+// priority) and launched on the solve's stream, so a host iteration costs a
+// few graph launches instead of up to 128 kernel launches. The graphs hold
+// only kernel nodes. PAINTOP_HIP_GRAPH=0 launches the kernels directly.
+struct SweepGraphs {
+  struct Entry {
+    int K, PF, ntiles, ninst, hb, n;
+    const void *tiles, *insts, *ctl, *partials;
+    hipGraphExec_t exec;
+  };
+  hipStream_t cs = nullptr;
+  std::vector<Entry> cache;
+  bool broken = false;
+  void clear() {
+    for (Entry &e : cache) (void)hipGraphExecDestroy(e.exec);
+    cache.clear();
+  }
+  hipGraphExec_t get(int K, int PF, int ntiles, int ninst, const PhTile *tiles,
+                     const PhInst *insts, PhCtl *ctl, double *partials, int hb, int n) {
+    for (const Entry &e : cache)
+      if (e.K == K && e.PF == PF && e.ntiles == ntiles && e.ninst == ninst && e.hb == hb &&
+          e.n == n && e.tiles == tiles && e.insts == insts && e.ctl == ctl &&
+          e.partials == partials)
+        return e.exec;
+    if (!cs) HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    if (cache.size() >= 64) clear();
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    try {
+      for (int i = 0; i < n; ++i) launch_sor(K, PF, ntiles, ninst, cs, tiles, insts, ctl, partials, hb);
+    } catch (...) {
+      (void)hipStreamEndCapture(cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_OK(hipStreamEndCapture(cs, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_OK(e);
+    cache.push_back(Entry{K, PF, ntiles, ninst, hb, n, tiles, insts, ctl, partials, ex});
+    return ex;
+  }
+};
+static SweepGraphs g_graphs;
+
+// PAINTOP_HIP_TRACE=1: the warm-start race's events on stderr.
+static bool trace_on() {
+  const char *e = getenv("PAINTOP_HIP_TRACE");
+  return e && e[0] == '1';
+}
+
+static bool use_graphs() {
+  const char *e = getenv("PAINTOP_HIP_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+// n launch pairs on st: from cached graphs when enabled, else directly.
+static void launch_sor_batch(int K, int PF, int ntiles, int ninst, hipStream_t st,
+                             const PhTile *tiles, const PhInst *insts, PhCtl *ctl,
+                             double *partials, int hb, int n) {
+  if (n <= 0 || ntiles <= 0) return;
+  if (n == 1 || !use_graphs() || g_graphs.broken) {
+    for (int i = 0; i < n; ++i) launch_sor(K, PF, ntiles, ninst, st, tiles, insts, ctl, partials, hb);
+    return;
+  }
+  for (int b = 6; b >= 0; --b)
+    if (n & (1 << b)) {
+      hipGraphExec_t ex = g_graphs.get(K, PF, ntiles, ninst, tiles, insts, ctl, partials, hb, 1 << b);
+      HIP_OK(hipGraphLaunch(ex, st));
+    }
+  if (n >= 128)
+    for (int i = 0; i < (n >> 7) * 128; ++i)
+      launch_sor(K, PF, ntiles, ninst, st, tiles, insts, ctl, partials, hb);
 }
 
 // Rows a sweep tile loads ahead of use (1, 3 or 5; PAINTOP_HIP_PREFETCH overrides).
@@ -388,6 +467,7 @@ struct ItemIn {
   const float *src, *tgt, *mask;
   float *out;
   ph_report *rep;
+  const float *start = nullptr;  // caller's warm start (same layout as out), or null
 };
 
 struct ItemWork {
@@ -473,6 +553,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const bool tuned = d->omega <= 0.0;
   const bool want_warm =
       (d->flags & PH_FLAG_WARM_START_NET) && !(d->flags & PH_FLAG_FORCE_PLAIN);
+  bool race_any = want_warm;
+  for (const ItemIn &it : items) race_any = race_any || it.start != nullptr;
   if (want_warm && !warm_net_ready())
     throw fail(PH_CLASS_POLICY, "E_HIP_WARM_START_UNSET",
                "warm start requested but no network weights are set (ph_warm_start_set_weights)");
@@ -485,12 +567,13 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   {
     size_t need = 64 * 1024 + (size_t)nitem * 64;
     for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
+    // the plain twins of the warm-start race
+    if (race_any)
+      for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
     if (want_warm) {
       const size_t R2 = (size_t)warm_net_res() * warm_net_res();
       size_t nch = 0;
       for (const ItemIn &it : items) nch += it.C;
-      // the plain twins of the warm-start race
-      for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
       need += (R2 * 4 * warm_net_batch() + R2 * nch) * sizeof(float) + 1024;
     }
     g_pool.reset(need);
@@ -660,7 +743,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     }
   }
   const int ninst = (int)inst.size();  // the images' instances (twins come after)
-  const int ncap = std::max(want_warm ? 2 * ninst : ninst, 1);
+  const int ncap = std::max(race_any ? 2 * ninst : ninst, 1);
   double *resv_d = g_pool.take<double>(2 * (size_t)ncap);
   uint32_t *arr_d = g_pool.take<uint32_t>((size_t)ncap);
   HIP_OK(hipMemsetAsync(arr_d, 0, ncap * sizeof(uint32_t), st));
@@ -721,46 +804,65 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   };
   grow();
   for (int k = 0; k < ninst; ++k) rstart[k] = inst[k].r0;
-  if (want_warm && d->lambda == 0.0 && !act.empty()) {
-    const int R = warm_net_res(), NB = warm_net_batch();
-    const size_t R2 = (size_t)R * R;
-    const int nw = (int)act.size();
-    float *in_d = g_pool.take<float>(R2 * 4 * NB);
-    float *h_d = g_pool.take<float>(R2 * nw);
-    struct Geo { int x0, y0, f; };
-    std::vector<Geo> geo(nw);
-    for (int j = 0; j < nw; ++j) {
-      const ItemWork &w = iw[inst[act[j]].item];
-      const int f = std::max(1, (std::max(w.rw, w.rh) + R - 1) / R);
-      geo[j] = Geo{w.rx0 + w.rw / 2 - (R * f) / 2, w.ry0 + w.rh / 2 - (R * f) / 2, f};
+  // candidates: images with a caller's start field, and (with the network
+  // flag) every other Poisson-blend instance
+  std::vector<int> cand, net_list;
+  for (int k : act) {
+    if (iw[inst[k].item].in.start) cand.push_back(k);
+    else if (want_warm && d->lambda == 0.0) {
+      cand.push_back(k);
+      net_list.push_back(k);
     }
-    for (int j0 = 0; j0 < nw; j0 += NB) {
-      const int nb = std::min(NB, nw - j0);
-      for (int j = 0; j < nb; ++j) {
-        const InstWork &X = inst[act[j0 + j]];
-        const ItemIn &it = iw[X.item].in;
-        const Geo &g = geo[j0 + j];
-        hipLaunchKernelGGL(k_coarsen, dim3(R * R), dim3(64), 0, st, it.src, it.tgt, it.mask,
-                           nullptr, it.W, it.H, it.C, X.channel, d->lambda, g.x0, g.y0, g.f, R,
-                           in_d + R2 * 4 * j, nullptr, nullptr);
+  }
+  if (!cand.empty()) {
+    const int nw = (int)cand.size();
+    if (!net_list.empty()) {
+      const int R = warm_net_res(), NB = warm_net_batch();
+      const size_t R2 = (size_t)R * R;
+      const int nn = (int)net_list.size();
+      float *in_d = g_pool.take<float>(R2 * 4 * NB);
+      float *h_d = g_pool.take<float>(R2 * nn);
+      struct Geo { int x0, y0, f; };
+      std::vector<Geo> geo(nn);
+      for (int j = 0; j < nn; ++j) {
+        const ItemWork &w = iw[inst[net_list[j]].item];
+        const int f = std::max(1, (std::max(w.rw, w.rh) + R - 1) / R);
+        geo[j] = Geo{w.rx0 + w.rw / 2 - (R * f) / 2, w.ry0 + w.rh / 2 - (R * f) / 2, f};
       }
-      HIP_OK(hipGetLastError());
-      warm_net_forward(in_d, h_d + R2 * j0, nb, st);
+      for (int j0 = 0; j0 < nn; j0 += NB) {
+        const int nb = std::min(NB, nn - j0);
+        for (int j = 0; j < nb; ++j) {
+          const InstWork &X = inst[net_list[j0 + j]];
+          const ItemIn &it = iw[X.item].in;
+          const Geo &g = geo[j0 + j];
+          hipLaunchKernelGGL(k_coarsen, dim3(R * R), dim3(64), 0, st, it.src, it.tgt, it.mask,
+                             nullptr, it.W, it.H, it.C, X.channel, d->lambda, g.x0, g.y0, g.f,
+                             R, in_d + R2 * 4 * j, nullptr, nullptr);
+        }
+        HIP_OK(hipGetLastError());
+        warm_net_forward(in_d, h_d + R2 * j0, nb, st);
+      }
+      for (int j = 0; j < nn; ++j) {
+        const InstWork &X = inst[net_list[j]];
+        const Geo &g = geo[j];
+        hipLaunchKernelGGL(k_warm_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.src,
+                           h_d + R2 * j, R, g.x0, g.y0, g.f);
+        HIP_OK(hipGetLastError());
+      }
     }
-    for (int j = 0; j < nw; ++j) {
-      const int k = act[j];
+    for (int k : cand) {
       const InstWork &X = inst[k];
-      const Geo &g = geo[j];
-      hipLaunchKernelGGL(k_warm_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.src,
-                         h_d + R2 * j, R, g.x0, g.y0, g.f);
-      HIP_OK(hipGetLastError());
+      if (iw[X.item].in.start) {
+        hipLaunchKernelGGL(k_start_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.start);
+        HIP_OK(hipGetLastError());
+      }
       launch_residual(k);
     }
     HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
     for (int j = 0; j < nw; ++j) {
-      const int k = act[j];
+      const int k = cand[j];
       const ItemWork &w = iw[inst[k].item];
       const double rw = resv[2 * k];
       ph_report *rep = w.in.rep;
@@ -800,7 +902,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       is_twin[kt] = 1;
       rstart[k] = rw;
       rstart[kt] = r0;
-      probe[k] = probe[kt] = (int)std::min<uint32_t>(warm_probe_sweeps(n), d->max_iterations);
+      // the probe ends on a launch boundary (a multiple of K/2 sweeps) that a
+      // plain solve also passes through, so the plain member's trajectory is
+      // the plain solve's, bit for bit
+      const uint32_t half = (uint32_t)(K / 2);
+      const uint32_t pr = (warm_probe_sweeps(n) + half - 1) / half * half;
+      probe[k] = probe[kt] = (int)std::min<uint32_t>(pr, d->max_iterations);
       act.push_back(kt);
     }
   }
@@ -888,6 +995,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       Cc.refine_abs2 = (REFINE_RATIO * gnorm) * (REFINE_RATIO * gnorm);
     };
 
+    const bool tracing = trace_on();
     std::vector<char> held(nact, 0), hold_refine(nact, 0), resolved(inst.size(), 0);
     std::vector<double> pval(inst.size(), -1.0);
     std::vector<int> slot(inst.size(), -1);
@@ -919,10 +1027,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     double sweep_ms = 0.0;
     while (remaining > 0) {
       if (timing) HIP_OK(hipEventRecord(eb0, st));
-      for (int i = 0; i < batch; ++i) {
-        if (ntiles > 0) launch_sor(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb);
-        total_launches += 1;
-      }
+      launch_sor_batch(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb, batch);
+      total_launches += (uint32_t)batch;
       if (timing) HIP_OK(hipEventRecord(eb1, st));
       HIP_OK(hipMemcpyAsync(ctl.data(), ctl_d, nact * sizeof(PhCtl), hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
@@ -963,6 +1069,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
               held[a] = 1;
               hold_refine[a] = Cc.refine_pending ? 1 : 0;
               pval[act[a]] = Cc.last_rel * Cc.r0;
+              if (tracing)
+                fprintf(stderr, "[race] hold inst %d (%s) at sweep %u (final %u) rel %.3e\n",
+                        act[a], is_twin[act[a]] ? "plain" : "warm", Cc.sweeps_done,
+                        Cc.sweeps_final, Cc.last_rel);
               break;
             }
             pend.push_back({a, Cc.sweeps_done, -1});
@@ -982,7 +1092,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           HIP_OK(hipGetLastError());
           launch_residual(k);
         }
-        HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double),
+        // every instance's slot, the warm-start race's plain twins included
+        HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
                               hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         total_refine += (uint32_t)pend.size();
@@ -1008,6 +1119,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           if (inst[k].R.done) {
             --remaining;
             if (paired) pval[k] = gnorm;
+            if (paired && tracing)
+              fprintf(stderr, "[race] inst %d (%s) finished at sweep %u rel %.3e\n", k,
+                      is_twin[k] ? "plain" : "warm", P.m, rel);
+          } else if (paired && tracing && P.m < (uint32_t)probe[k]) {
+            fprintf(stderr, "[race] inst %d (%s) evaluated at sweep %u before the probe\n", k,
+                    is_twin[k] ? "plain" : "warm", P.m);
           }
         }
       }
@@ -1020,6 +1137,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           if (pval[k] < 0.0 || pval[kt] < 0.0) continue;
           const bool warm_wins = pval[k] < pval[kt];
           const int win = warm_wins ? k : kt, lose = warm_wins ? kt : k;
+          if (tracing)
+            fprintf(stderr, "[race] judge inst %d: warm %.3e (sweep %u) plain %.3e (sweep %u) -> %s\n",
+                    k, pval[k], ctl[slot[k]].sweeps_done, pval[kt], ctl[slot[kt]].sweeps_done,
+                    warm_wins ? "warm" : "plain");
           resolved[k] = resolved[kt] = 1;
           silent[lose] = 1;
           if (!inst[lose].R.done) {
@@ -1199,9 +1320,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
 }
 
 static void solve_device(const ph_solve_desc *d, const float *src, const float *tgt,
-                         const float *mask, float *out, hipStream_t st, ph_report *rep) {
+                         const float *mask, float *out, hipStream_t st, ph_report *rep,
+                         const float *start = nullptr) {
   std::vector<ItemIn> items{
-      ItemIn{(int)d->width, (int)d->height, (int)d->channels, src, tgt, mask, out, rep}};
+      ItemIn{(int)d->width, (int)d->height, (int)d->channels, src, tgt, mask, out, rep, start}};
   solve_items(d, items, st);
 }
 
@@ -1272,6 +1394,19 @@ extern "C" ph_status ph_poisson_solve_device(const ph_solve_desc *desc, const fl
     if ((size_t)desc->width * desc->height > 0 && (!d_source || !d_target || !d_mask || !d_out))
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
     solve_device(desc, d_source, d_target, d_mask, d_out, (hipStream_t)stream, rep);
+  });
+}
+
+extern "C" ph_status ph_poisson_solve_start_device(const ph_solve_desc *desc,
+                                                   const float *d_source, const float *d_target,
+                                                   const float *d_mask, const float *d_start,
+                                                   float *d_out, void *stream, ph_report *rep,
+                                                   ph_error *err) {
+  return run_guarded(desc, rep, err, [&]() {
+    if ((size_t)desc->width * desc->height > 0 &&
+        (!d_source || !d_target || !d_mask || !d_out || !d_start))
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
+    solve_device(desc, d_source, d_target, d_mask, d_out, (hipStream_t)stream, rep, d_start);
   });
 }
 

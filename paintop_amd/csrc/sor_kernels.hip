@@ -275,7 +275,7 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const double tot = wave_sum((double)acc[s]);
-    if (lane == 0) partials[(size_t)b * K + s] = tot;
+    if (lane == 0) partials[(size_t)s * ntiles + b] = tot;  // stage-major: coalesced sums
   }
 }
 
@@ -286,7 +286,7 @@ template <int K>
 __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict__ insts,
                                                         PhCtl *ctl,
                                                         const double *__restrict__ partials,
-                                                        int ninst) {
+                                                        int ninst, int ntiles) {
   __shared__ double A[K];
   const int a = blockIdx.x;
   if (a >= ninst) return;
@@ -295,11 +295,19 @@ __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict
   const PhInst &I = insts[a];
   const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double v = 0.0;
-  for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += partials[(size_t)t * K + s];
+  const double *ps = partials + (size_t)s * ntiles;
+  for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += ps[t];
   v = wave_sum(v);
   if (lane == 0) A[s] = v;
   __syncthreads();
-  if (threadIdx.x == 0) control<K>(C, I, A);
+  if (threadIdx.x == 0) {
+    // one lane runs the stop rules on register copies: a chain of dependent
+    // global loads and stores would dominate this kernel otherwise
+    PhCtl c = C;
+    const PhInst in = I;
+    control<K>(c, in, A);
+    C = c;
+  }
 }
 
 #define PH_INSTANTIATE(K, PF)                                                                \
@@ -317,10 +325,10 @@ PH_INSTANTIATE(12, 3)
 PH_INSTANTIATE(16, 3)
 PH_INSTANTIATE(8, 5)
 PH_INSTANTIATE(16, 5)
-template __global__ void k_sor_control<2>(const PhInst *, PhCtl *, const double *, int);
-template __global__ void k_sor_control<4>(const PhInst *, PhCtl *, const double *, int);
-template __global__ void k_sor_control<8>(const PhInst *, PhCtl *, const double *, int);
-template __global__ void k_sor_control<12>(const PhInst *, PhCtl *, const double *, int);
-template __global__ void k_sor_control<16>(const PhInst *, PhCtl *, const double *, int);
+template __global__ void k_sor_control<2>(const PhInst *, PhCtl *, const double *, int, int);
+template __global__ void k_sor_control<4>(const PhInst *, PhCtl *, const double *, int, int);
+template __global__ void k_sor_control<8>(const PhInst *, PhCtl *, const double *, int, int);
+template __global__ void k_sor_control<12>(const PhInst *, PhCtl *, const double *, int, int);
+template __global__ void k_sor_control<16>(const PhInst *, PhCtl *, const double *, int, int);
 
 }  // namespace ph

@@ -33,7 +33,7 @@ struct UnLayout {
       o += (size_t)d.cout * kp[l];
       off_b[l] = o;
       o += d.cout;
-      const int cin_real = (l == 0) ? 3 : cp[l];
+      const int cin_real = cp[l];
       count += (size_t)9 * cin_real * d.cout + d.cout;
     }
     off_head = o;
@@ -79,7 +79,7 @@ static void init_params(std::vector<float> &p, uint64_t seed) {
   auto normal = [&]() { return sqrt(-2.0 * log(next())) * cos(6.283185307179586 * next()); };
   for (int l = 0; l < UN_LAYERS; ++l) {
     const UnLayerDef &d = UN_DEF[l];
-    const int cin_real = (l == 0) ? 3 : g_lay.cp[l];
+    const int cin_real = g_lay.cp[l];
     const double sd = sqrt(2.0 / (9.0 * cin_real));
     for (int co = 0; co < d.cout; ++co)
       for (int t = 0; t < 9; ++t)
@@ -90,11 +90,11 @@ static void init_params(std::vector<float> &p, uint64_t seed) {
   for (int c = 0; c < UN_HEAD_C; ++c) p[g_lay.off_head + c] = (float)(sd * normal());
 }
 
-// Zero every padding slot (input channel 3 of the first layer, K padding).
+// Every K-padding slot must be zero.
 static void check_padding(const float *p) {
   for (int l = 0; l < UN_LAYERS; ++l) {
     const UnLayerDef &d = UN_DEF[l];
-    const int cin_real = (l == 0) ? 3 : g_lay.cp[l];
+    const int cin_real = g_lay.cp[l];
     for (int co = 0; co < d.cout; ++co)
       for (int k = 0; k < g_lay.kp[l]; ++k) {
         const int t = k / g_lay.cp[l], ci = k % g_lay.cp[l];
@@ -115,7 +115,7 @@ struct UnNet {
   uint16_t *dz[UN_LAYERS] = {};
   uint16_t *dcat[3] = {}, *dpool[3] = {};
   float *out = nullptr;
-  float *part = nullptr, *part_db = nullptr, *head_part = nullptr;
+  float *part = nullptr, *part_db = nullptr, *head_part = nullptr, *pyr = nullptr;
   double *loss = nullptr;
   size_t part_cap = 0, part_db_cap = 0;
   std::vector<void *> allocs;
@@ -185,6 +185,7 @@ struct UnNet {
     part = alloc<float>(part_cap);
     part_db = alloc<float>(part_db_cap);
     head_part = alloc<float>((size_t)cdiv(npix(0, mb), 256) * (UN_HEAD_C + 2));
+    pyr = alloc<float>(npix(0, mb) / 3 + 64);  // sum of the 2x2..16x16 levels < npix/3
     loss = alloc<double>(1);
     std::vector<float> p;
     if (host_params) {
@@ -260,6 +261,7 @@ struct UnNet {
       A.h = r;
       A.w = r;
       A.relu = 1;
+      A.dil = d.dil;
       conv(A, st);
       if (d.out == UB_E0 || d.out == UB_E1 || d.out == UB_E2) {
         const int pb = d.out == UB_E0 ? UB_P0 : d.out == UB_E1 ? UB_P1 : UB_P2;
@@ -293,6 +295,7 @@ struct UnNet {
     A.h = r;
     A.w = r;
     A.chunk = dw_chunk(l, batch);
+    A.dil = d.dil;
     const int nchunk = (int)cdiv(npix(d.level, batch), A.chunk);
     hipLaunchKernelGGL(k_conv3_dw, dim3(nchunk, A.kp / A.kr), dim3(256), 0, st, A);
     const int nw = d.cout * A.kp + d.cout;
@@ -321,17 +324,24 @@ struct UnNet {
     A.h = r;
     A.w = r;
     A.relu = 0;
+    A.dil = d.dil;
     conv(A, st);
   }
 
   // Loss and gradients for one batch after forward(); returns nothing (the
   // loss stays on the device in `loss`).
-  void backward(const float *d_target, const float *d_weight, int batch, hipStream_t st) {
+  void backward(const float *d_target, const float *d_weight, int batch, float alpha,
+                hipStream_t st) {
     const int np0 = (int)npix(0, batch);
     const int nhb = (int)cdiv(np0, 256);
+    if (alpha != 0.0f) {
+      if (res % 16) throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "pyramid loss needs res % 16 == 0");
+      hipLaunchKernelGGL(k_err_pyramid, dim3(batch * (res / 16) * (res / 16)), dim3(256), 0, st,
+                         out, d_target, d_weight, res, pyr);
+    }
     hipLaunchKernelGGL(k_head_bwd, dim3(nhb), dim3(256), 0, st, out, d_target, d_weight,
-                       act[UB_D0], master + g_lay.off_head, np0, 1.0f / (float)np0, dz[13],
-                       head_part);
+                       act[UB_D0], master + g_lay.off_head, np0, 1.0f / (float)np0, pyr, res,
+                       alpha, dz[13], head_part);
     hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, st, head_part, nhb,
                        grad + g_lay.off_head, loss);
     // decoder level 0
@@ -495,7 +505,9 @@ extern "C" ph_status ph_unet_train_step(ph_unet *net, const float *d_in, const f
     hipStream_t st = (hipStream_t)stream;
     UnNet &n = net->net;
     n.forward(d_in, n.out, batch, st);
-    n.backward(d_target, d_weight, batch, st);
+    if (!(opts->pyramid_weight >= 0.0f))
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "pyramid_weight must be >= 0");
+    n.backward(d_target, d_weight, batch, opts->pyramid_weight, st);
     if (opts->apply_update) n.adam(*opts, st);
     if (loss) HIP_OK(hipMemcpyAsync(loss, n.loss, sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));

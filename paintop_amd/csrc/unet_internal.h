@@ -4,18 +4,20 @@
 //   level 0 (res)     e0a 4->16, e0b 16->16             pool
 //   level 1 (res/2)   e1a 16->32, e1b 32->32            pool
 //   level 2 (res/4)   e2a 32->64, e2b 64->64            pool
-//   level 3 (res/8)   b3a 64->64, b3b 64->64
+//   level 3 (res/8)   b3a 64->64 (dilation 2), b3b 64->64 (dilation 4)
 //   level 2           d2a cat(e2b, up(b3b)) 128->64, d2b 64->64
 //   level 1           d1a cat(e1b, up(d2b)) 96->32,  d1b 32->32
 //   level 0           d0a cat(e0b, up(d1b)) 48->16,  d0b 16->16
 //   head              1x1, 16->1, FP32
-// Every 3x3 convolution has a bias and a ReLU. The input has 3 channels
-// (stored as 4, the fourth always zero): the right-hand side of the correction
-// problem, the boundary values (zero in the interior) and the interior fraction.
+// The dilated bottleneck widens the receptive field to the whole grid (the
+// harmonic correction at a cell depends on the entire boundary). Every 3x3
+// convolution has a bias and a ReLU. The input has 4 channels: the
+// right-hand side of the correction problem, the boundary values (zero off
+// the Dirichlet ring), the interior fraction and the ring density.
 //
 // Parameter storage (FP32, also the on-disk layout): per convolution
-// W[cout][kp] (k = tap*cp + ci, cp = input channels padded to a multiple of 4,
-// kp = 9*cp rounded up to 16, padding zero) followed by bias[cout]; then the
+// W[cout][kp] (k = tap*cp + ci, cp = input channels, kp = 9*cp rounded up
+// to 16, padding zero) followed by bias[cout]; then the
 // head's 16 weights and its bias.
 #pragma once
 #include <stdint.h>
@@ -38,24 +40,25 @@ struct UnLayerDef {
   int ca, cb;      // channels of in_a / in_b
   int cout;
   int out;         // activation buffer written
+  int dil;         // dilation of the 3x3 taps
 };
 
 // clang-format off
 constexpr UnLayerDef UN_DEF[UN_LAYERS] = {
-    {0, UB_X,   -1,     4,  0, 16, UB_A0},
-    {0, UB_A0,  -1,    16,  0, 16, UB_E0},
-    {1, UB_P0,  -1,    16,  0, 32, UB_A1},
-    {1, UB_A1,  -1,    32,  0, 32, UB_E1},
-    {2, UB_P1,  -1,    32,  0, 64, UB_A2},
-    {2, UB_A2,  -1,    64,  0, 64, UB_E2},
-    {3, UB_P2,  -1,    64,  0, 64, UB_A3},
-    {3, UB_A3,  -1,    64,  0, 64, UB_B3},
-    {2, UB_E2,  UB_B3, 64, 64, 64, UB_D2A},
-    {2, UB_D2A, -1,    64,  0, 64, UB_D2},
-    {1, UB_E1,  UB_D2, 32, 64, 32, UB_D1A},
-    {1, UB_D1A, -1,    32,  0, 32, UB_D1},
-    {0, UB_E0,  UB_D1, 16, 32, 16, UB_D0A},
-    {0, UB_D0A, -1,    16,  0, 16, UB_D0},
+    {0, UB_X,   -1,     4,  0, 16, UB_A0, 1},
+    {0, UB_A0,  -1,    16,  0, 16, UB_E0, 1},
+    {1, UB_P0,  -1,    16,  0, 32, UB_A1, 1},
+    {1, UB_A1,  -1,    32,  0, 32, UB_E1, 1},
+    {2, UB_P1,  -1,    32,  0, 64, UB_A2, 1},
+    {2, UB_A2,  -1,    64,  0, 64, UB_E2, 1},
+    {3, UB_P2,  -1,    64,  0, 64, UB_A3, 2},
+    {3, UB_A3,  -1,    64,  0, 64, UB_B3, 4},
+    {2, UB_E2,  UB_B3, 64, 64, 64, UB_D2A, 1},
+    {2, UB_D2A, -1,    64,  0, 64, UB_D2, 1},
+    {1, UB_E1,  UB_D2, 32, 64, 32, UB_D1A, 1},
+    {1, UB_D1A, -1,    32,  0, 32, UB_D1, 1},
+    {0, UB_E0,  UB_D1, 16, 32, 16, UB_D0A, 1},
+    {0, UB_D0A, -1,    16,  0, 16, UB_D0, 1},
 };
 // clang-format on
 
@@ -71,6 +74,7 @@ struct UnConvArgs {
   int ca, cb, kp, cout;
   int n, h, w;
   int relu;
+  int dil;               // dilation of the 3x3 taps
 };
 
 struct UnDwArgs {
@@ -82,6 +86,7 @@ struct UnDwArgs {
   int cout, kp, kr;    // kr: K columns per block
   int n, h, w;
   int chunk;           // pixels per chunk (multiple of 64)
+  int dil;             // dilation of the 3x3 taps
 };
 
 template <int MB, int NB>
@@ -95,8 +100,11 @@ __global__ void k_skip_grad(const uint16_t *dcat, int ccat, const uint16_t *dpoo
 __global__ void k_up_grad(const uint16_t *dcat, int ccat, int c1, const uint16_t *prev, int c2,
                           int n, int h, int w, uint16_t *dz);
 __global__ void k_head(const uint16_t *d0, const float *wh, int npix, float *out);
+__global__ void k_err_pyramid(const float *out, const float *y, const float *wt, int res,
+                              float *pyr);
 __global__ void k_head_bwd(const float *out, const float *y, const float *wt, const uint16_t *d0,
-                           const float *wh, int npix, float inv_n, uint16_t *dz, float *part);
+                           const float *wh, int npix, float inv_n, const float *pyr, int res,
+                           float alpha, uint16_t *dz, float *part);
 __global__ void k_head_reduce(const float *part, int nblocks, float *gh, double *loss);
 __global__ void k_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1,
                        float b2, float eps, float bc1, float bc2, float wd);

@@ -52,7 +52,8 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution, forward and input gradient.
+// 3x3 convolution (taps dilated by A.dil), forward and input gradient (the
+// adjoint of a dilated correlation is the same dilation with flipped taps).
 // One wave computes NB*16 pixels x MB*16 output channels; a block has 4 waves.
 // ---------------------------------------------------------------------------
 template <int MB, int NB>
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void k_conv3(UnConvArgs A) {
   if (cin >= 16) {
     // cin is a multiple of 16: one tap per group of 16-wide K chunks
     for (int t = 0; t < 9; ++t) {
-      const int dy = t / 3 - 1, dx = t % 3 - 1;
+      const int dy = (t / 3 - 1) * A.dil, dx = (t % 3 - 1) * A.dil;
       const uint16_t *sa[NB];
       const uint16_t *sb[NB];
       bool ok[NB];
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void k_conv3(UnConvArgs A) {
     // cin == 4 (the network input): one 16-wide K chunk spans four taps
     for (int k0 = 0; k0 < A.kp; k0 += 16) {
       const int t = (k0 >> 2) + grp;
-      const int dy = t / 3 - 1, dx = t % 3 - 1;
+      const int dy = (t / 3 - 1) * A.dil, dx = (t % 3 - 1) * A.dil;
       bf16x4 bv[NB];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_conv3_dw(UnDwArgs A) {
       if (p < pend && t < 9) {
         const int img = p / hw, rem = p - img * hw;
         const int y = rem / A.w, x = rem - y * A.w;
-        const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+        const int yy = y + (t / 3 - 1) * A.dil, xx = x + (t % 3 - 1) * A.dil;
         if (yy >= 0 && yy < A.h && xx >= 0 && xx < A.w) {
           if (ci < A.ca)
             v = ld4(A.a + ((size_t)(img * A.h + yy) * A.w + xx) * A.ca + ci);
@@ -380,7 +381,42 @@ __global__ void k_head(const uint16_t *__restrict__ d0, const float *__restrict_
   out[p] = s;
 }
 
-// loss = sum_p wt[p] * (out[p] - y[p])^2 / npix. dout = 2 wt (out - y) / npix.
+// Block means of the weighted error e = wt * (out - y) over 2x2, 4x4, 8x8
+// and 16x16 blocks, one 256-thread block per 16x16 tile (res a multiple of
+// 16), reduced in a fixed order. pyr holds, per image, the four levels
+// [res/2]^2, [res/4]^2, [res/8]^2, [res/16]^2 back to back.
+__global__ __launch_bounds__(256) void k_err_pyramid(const float *__restrict__ out,
+                                                     const float *__restrict__ y,
+                                                     const float *__restrict__ wt, int res,
+                                                     float *__restrict__ pyr) {
+  const int tiles = res >> 4;
+  const int img = blockIdx.x / (tiles * tiles), t = blockIdx.x % (tiles * tiles);
+  const int ty = t / tiles, tx = t % tiles;
+  const int ly = threadIdx.x >> 4, lx = threadIdx.x & 15;
+  const size_t p = ((size_t)img * res + ty * 16 + ly) * res + tx * 16 + lx;
+  __shared__ float e[256];
+  e[threadIdx.x] = wt[p] * (out[p] - y[p]);
+  __syncthreads();
+  const size_t per_img = (size_t)(res / 2) * (res / 2) + (res / 4) * (res / 4) +
+                         (res / 8) * (res / 8) + (res / 16) * (res / 16);
+  float *base = pyr + img * per_img;
+  int off = 0;
+  for (int lvl = 1; lvl <= 4; ++lvl) {
+    const int bs = 1 << lvl, nb = 16 >> lvl, r = res >> lvl;
+    if ((int)threadIdx.x < nb * nb) {
+      const int by = threadIdx.x / nb, bx = threadIdx.x % nb;
+      float sum = 0.0f;
+      for (int yy = 0; yy < bs; ++yy)
+        for (int xx = 0; xx < bs; ++xx) sum += e[(by * bs + yy) * 16 + bx * bs + xx];
+      base[off + (size_t)(ty * nb + by) * r + tx * nb + bx] = sum / (float)(bs * bs);
+    }
+    off += r * r;
+  }
+}
+
+// loss = (1/n) sum_p [wt (out - y)^2 + alpha sum_s m_s(p)^2] with m_s the
+// pyramid block means (alpha = 0: plain weighted MSE);
+// dout = (2/n) wt (out - y + alpha sum_s m_s(p)).
 // Writes dz_d0 (gradient at the pre-activation of the last decoder conv) and
 // per-block partials of [dwh (16), dbh, loss] for a fixed-order reduction.
 constexpr int UN_HEAD_PART = UN_HEAD_C + 2;
@@ -390,7 +426,9 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ out,
                                                   const float *__restrict__ wt,
                                                   const uint16_t *__restrict__ d0,
                                                   const float *__restrict__ wh, int npix,
-                                                  float inv_n, uint16_t *__restrict__ dz,
+                                                  float inv_n, const float *__restrict__ pyr,
+                                                  int res, float alpha,
+                                                  uint16_t *__restrict__ dz,
                                                   float *__restrict__ part) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float acc[UN_HEAD_PART];
@@ -398,9 +436,24 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ out,
   for (int j = 0; j < UN_HEAD_PART; ++j) acc[j] = 0.0f;
   if (p < npix) {
     const float diff = out[p] - y[p];
-    const float dout = 2.0f * wt[p] * diff * inv_n;
+    float msum = 0.0f, m2 = 0.0f;
+    if (alpha != 0.0f) {
+      const int hw = res * res, img = p / hw, rem = p - img * hw, py = rem / res, px = rem % res;
+      const size_t per_img = (size_t)(res / 2) * (res / 2) + (res / 4) * (res / 4) +
+                             (res / 8) * (res / 8) + (res / 16) * (res / 16);
+      const float *base = pyr + img * per_img;
+      int off = 0;
+      for (int lvl = 1; lvl <= 4; ++lvl) {
+        const int r = res >> lvl;
+        const float m = base[off + (size_t)(py >> lvl) * r + (px >> lvl)];
+        msum += m;
+        m2 = fmaf(m, m, m2);
+        off += r * r;
+      }
+    }
+    const float dout = 2.0f * wt[p] * fmaf(alpha, msum, diff) * inv_n;
     acc[UN_HEAD_C] = dout;
-    acc[UN_HEAD_C + 1] = wt[p] * diff * diff * inv_n;
+    acc[UN_HEAD_C + 1] = fmaf(alpha, m2, wt[p] * diff * diff) * inv_n;
 #pragma unroll
     for (int g = 0; g < UN_HEAD_C / 4; ++g) {
       const bf16x4 v = ld4(d0 + (size_t)p * UN_HEAD_C + g * 4);

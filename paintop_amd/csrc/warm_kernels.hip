@@ -25,8 +25,15 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
+__device__ __forceinline__ bool warm_interior(const float *mask, int W, int H, int ix, int iy) {
+  return ix >= 1 && ix < W - 1 && iy >= 1 && iy < H - 1 &&
+         mask[(size_t)iy * W + ix] > WARM_MASK_THRESHOLD;
+}
+
 // One wave per coarse cell; lanes stride the block's columns, sums in a fixed
 // order (lane-local rows, then the butterfly), so results are deterministic.
+// Ring cells are the non-interior cells with an interior 4-neighbour: the
+// Dirichlet data of the solve (poisson.rs:374-392 reads them as neighbours).
 __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
                                                 const float *__restrict__ anc,
                                                 const float *__restrict__ mask,
@@ -37,7 +44,7 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
   const int c = blockIdx.x;
   const int cx = c % res, cy = c / res;
   const int lane = threadIdx.x;
-  double s_rhs = 0.0, s_bnd = 0.0, s_int = 0.0, s_t = 0.0, s_img = 0.0;
+  double s_rhs = 0.0, s_ring = 0.0, n_int = 0.0, n_ring = 0.0, s_t = 0.0;
   for (int dy = 0; dy < f; ++dy) {
     const int iy = y0 + cy * f + dy;
     if (iy < 0 || iy >= H) continue;
@@ -45,33 +52,33 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
       const int ix = x0 + cx * f + dx;
       if (ix < 0 || ix >= W) continue;
       const size_t q = (size_t)iy * W + ix;
-      const bool in = ix >= 1 && ix < W - 1 && iy >= 1 && iy < H - 1 &&
-                      mask[q] > WARM_MASK_THRESHOLD;
       const double P = (double)src[q * C + ch], A = (double)anc[q * C + ch];
-      s_img += 1.0;
-      if (in) {
-        s_int += 1.0;
+      if (warm_interior(mask, W, H, ix, iy)) {
+        n_int += 1.0;
         s_rhs += lambda * (P - A);
         if (sol) s_t += (double)sol[q * C + ch] - P;
-      } else {
-        s_bnd += A - P;
+      } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
+                 warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
+        n_ring += 1.0;
+        s_ring += A - P;
         if (sol) s_t += A - P;
       }
     }
   }
   s_rhs = wave_sum_d(s_rhs);
-  s_bnd = wave_sum_d(s_bnd);
-  s_int = wave_sum_d(s_int);
+  s_ring = wave_sum_d(s_ring);
+  n_int = wave_sum_d(n_int);
+  n_ring = wave_sum_d(n_ring);
   s_t = wave_sum_d(s_t);
-  s_img = wave_sum_d(s_img);
   if (lane == 0) {
     const double inv = 1.0 / ((double)f * f);
     in4[(size_t)c * 4 + 0] = (float)s_rhs;  // f^2 * mean
-    in4[(size_t)c * 4 + 1] = (float)(s_bnd * inv);
-    in4[(size_t)c * 4 + 2] = (float)(s_int * inv);
-    in4[(size_t)c * 4 + 3] = 0.0f;
-    if (tgt) tgt[c] = s_img > 0.0 ? (float)(s_t / s_img) : 0.0f;
-    if (wt) wt[c] = (float)(s_img * inv);
+    in4[(size_t)c * 4 + 1] = n_ring > 0.0 ? (float)(s_ring / n_ring) : 0.0f;
+    in4[(size_t)c * 4 + 2] = (float)(n_int * inv);
+    in4[(size_t)c * 4 + 3] = (float)(n_ring / f);
+    const double nt = n_int + n_ring;
+    if (tgt) tgt[c] = nt > 0.0 ? (float)(s_t / nt) : 0.0f;
+    if (wt) wt[c] = n_int > 0.0 ? 1.0f : (n_ring > 0.0 ? 0.5f : 0.0f);
   }
 }
 
@@ -98,6 +105,17 @@ __global__ void k_warm_apply(PhInst I, const float *__restrict__ src, const floa
   const double h = (1.0 - b) * ((1.0 - a) * h00 + a * h01) + b * ((1.0 - a) * h10 + a * h11);
   const size_t q = (size_t)iy * I.img_w + ix;
   I.x[p] = (double)src[q * I.channels + I.channel] + h;
+}
+
+// x = the caller's start field at the interior cells of an instance's plane.
+__global__ void k_start_apply(PhInst I, const float *__restrict__ start) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pw) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  if (!I.cls[p]) return;
+  const size_t q = (size_t)(I.oy + py) * I.img_w + (I.ox + px);
+  I.x[p] = (double)start[q * I.channels + I.channel];
 }
 
 }  // namespace ph

@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "ph_unet_set_state",
     "ph_warm_start_set_weights",
     "ph_warm_coarsen_device",
+    "ph_poisson_solve_start_device",
 )
 
 
@@ -132,6 +133,7 @@ class PhTrainOpts(ctypes.Structure):
         ("epsilon", ctypes.c_float),
         ("weight_decay", ctypes.c_float),
         ("apply_update", ctypes.c_int32),
+        ("pyramid_weight", ctypes.c_float),
     ]
 
 
@@ -168,6 +170,10 @@ def lib():
     L.ph_poisson_solve_device.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp, vp,
                                           ctypes.POINTER(PhReport), ctypes.POINTER(PhError)]
     L.ph_poisson_solve_device.restype = ctypes.c_int
+    L.ph_poisson_solve_start_device.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp, vp,
+                                                vp, ctypes.POINTER(PhReport),
+                                                ctypes.POINTER(PhError)]
+    L.ph_poisson_solve_start_device.restype = ctypes.c_int
     L.ph_fbm_device.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
@@ -240,7 +246,7 @@ class SolveResult:
     device_ms: float = 0.0
     omega: float = 0.0
     half_sweeps: int = 0
-    warm_start_used: bool = False
+    warm_start_used: int = 0  # channels that kept the learned start
     warm_start_rel: float = 0.0
     sweep_ms: float = 0.0
 
@@ -269,7 +275,7 @@ def _result(rep: PhReport, hist: np.ndarray) -> SolveResult:
         device_ms=float(rep.device_ms),
         omega=float(rep.omega_used),
         half_sweeps=int(rep.half_sweeps_used),
-        warm_start_used=bool(rep.warm_start_used),
+        warm_start_used=int(rep.warm_start_used),
         warm_start_rel=float(rep.warm_start_rel),
         sweep_ms=float(rep.sweep_ms),
     )
@@ -318,8 +324,11 @@ def poisson_solve(desc: PhSolveDesc, source: np.ndarray, target: np.ndarray, mas
 
 
 def poisson_solve_device(desc: PhSolveDesc, d_source: int, d_target: int, d_mask: int,
-                         d_out: int, stream: int = 0, want_history: bool = True) -> SolveResult:
-    """Device-buffer solve on raw device pointers (e.g. torch ``data_ptr()``)."""
+                         d_out: int, stream: int = 0, want_history: bool = True,
+                         d_start: int | None = None) -> SolveResult:
+    """Device-buffer solve on raw device pointers. With ``d_start`` the solve
+    races that initial field against the anchor start
+    (ph_poisson_solve_start_device)."""
     cap = int(desc.max_iterations)
     hist = np.zeros(max(cap, 1), dtype=np.float64)
     rep = PhReport()
@@ -327,8 +336,13 @@ def poisson_solve_device(desc: PhSolveDesc, d_source: int, d_target: int, d_mask
         rep.residual_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         rep.history_capacity = cap
     err = PhError()
-    st = lib().ph_poisson_solve_device(ctypes.byref(desc), d_source, d_target, d_mask, d_out,
-                                       stream or None, ctypes.byref(rep), ctypes.byref(err))
+    if d_start is None:
+        st = lib().ph_poisson_solve_device(ctypes.byref(desc), d_source, d_target, d_mask, d_out,
+                                           stream or None, ctypes.byref(rep), ctypes.byref(err))
+    else:
+        st = lib().ph_poisson_solve_start_device(ctypes.byref(desc), d_source, d_target, d_mask,
+                                                 d_start, d_out, stream or None,
+                                                 ctypes.byref(rep), ctypes.byref(err))
     _check(st, err)
     return _result(rep, hist)
 
@@ -441,8 +455,8 @@ class UNet:
 
     def train_step(self, d_in: int, d_target: int, d_weight: int, batch: int, lr: float,
                    beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0, update=True,
-                   stream: int = 0) -> float:
-        o = PhTrainOpts(lr, beta1, beta2, eps, weight_decay, 1 if update else 0)
+                   pyramid=0.0, stream: int = 0) -> float:
+        o = PhTrainOpts(lr, beta1, beta2, eps, weight_decay, 1 if update else 0, pyramid)
         loss = ctypes.c_double(0.0)
         err = PhError()
         _check(lib().ph_unet_train_step(self._h, d_in, d_target, d_weight, batch, ctypes.byref(o),

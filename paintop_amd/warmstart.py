@@ -9,10 +9,11 @@ residual ||r|| / ||r(anchor start)|| (poisson.rs:172) and stop rules.
 
 What the network predicts: with P = the source image, the blend's solution is
 u = P + h where h is harmonic inside the mask with boundary values
-anchor - P on the Dirichlet ring. The network maps the f x f block means of
-(rhs of the correction problem, boundary values with 0 in the interior,
-interior fraction) on a 128 x 128 grid around the mask to the block means of
-h; the solver starts from x = P + bilinear(h).
+anchor - P on the Dirichlet ring. The network maps the f x f blocks of
+(rhs of the correction problem, boundary values on the Dirichlet ring,
+interior fraction, ring density) on a 128 x 128 grid around the mask to the
+block means of h over interior and ring cells; the solver starts from
+x = P + bilinear(h).
 
 Usage::
 
@@ -205,15 +206,16 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
 
 
 def loss_weight(X: np.ndarray, Wt: np.ndarray) -> np.ndarray:
-    """Per-cell loss weight: in-image fraction x (0.2 + 0.8 x interior fraction)."""
-    return (Wt * (0.2 + 0.8 * X[..., 2])).astype(np.float32)
+    """Per-cell loss weight: the coarsening's weight (1 on blocks with interior
+    cells, 0.5 on blocks holding only Dirichlet-ring cells, 0 elsewhere)."""
+    return np.ascontiguousarray(Wt, dtype=np.float32)
 
 
 # ---------------------------------------------------------------------------
 # training
 
 def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, seed=1,
-          val_frac=0.05, ckpt=None, log=print, time_budget=None):
+          val_frac=0.05, ckpt=None, log=print, time_budget=None, pyramid=1.0):
     """AdamW on the GPU with a cosine schedule. Returns (params, history)."""
     from .device import DeviceArray
 
@@ -257,7 +259,8 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
         tot, cnt = 0.0, 0
         for v0 in range(ntr, n - batch + 1, batch):
             gather(range(v0, v0 + batch))
-            tot += net.train_step(bX.ptr, bY.ptr, bW.ptr, batch, 0.0, update=False)
+            tot += net.train_step(bX.ptr, bY.ptr, bW.ptr, batch, 0.0, update=False,
+                                  pyramid=pyramid)
             cnt += 1
         return tot / max(cnt, 1)
 
@@ -271,7 +274,8 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
         a = min(1.0, (step + 1) / warmup)
         cosf = 0.5 * (1 + math.cos(math.pi * step / max(total, 1)))
         cur = lr_min + (lr - lr_min) * cosf
-        loss = net.train_step(bX.ptr, bY.ptr, bW.ptr, batch, cur * a, weight_decay=1e-4)
+        loss = net.train_step(bX.ptr, bY.ptr, bW.ptr, batch, cur * a, weight_decay=1e-4,
+                              pyramid=pyramid)
         step += 1
         now = time.time()
         if now - last > 30 or step == total:
@@ -305,6 +309,7 @@ def main(argv=None):
     t.add_argument("--out", default=WEIGHTS_DIR)
     t.add_argument("--ckpt", default=None, help="checkpoint file (resume if present)")
     t.add_argument("--time-budget", type=float, default=None, help="seconds of training")
+    t.add_argument("--pyramid", type=float, default=1.0, help="multi-scale loss weight")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
@@ -313,10 +318,12 @@ def main(argv=None):
         gen_s = time.time() - t0
         Wl = loss_weight(X, Wt)
         params, info = train(X, Y, Wl, epochs=a.epochs, batch=a.batch, lr=a.lr, ckpt=a.ckpt,
-                             log=log, time_budget=a.time_budget)
+                             log=log, time_budget=a.time_budget, pyramid=a.pyramid)
         os.makedirs(a.out, exist_ok=True)
         np.save(os.path.join(a.out, "warm_start_unet.npy"), params.astype(np.float32))
         meta = dict(samples=a.samples, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                    pyramid=a.pyramid,
+                    dropped=int((Wt.reshape(len(Wt), -1).max(axis=1) == 0).sum()),
                     steps=info["steps"], val_loss=info["val_loss"],
                     history=info["history"][-20:], pair_generation_s=round(gen_s, 1),
                     train_s=round(time.time() - t0 - gen_s, 1),

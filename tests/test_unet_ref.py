@@ -27,8 +27,6 @@ def test_reference_gradients_match_finite_differences():
         v = np.zeros(total)
         blk = np.zeros((cout, kp))
         r = rng.standard_normal((cout, 9, cp))
-        if li == 0:
-            r[:, :, 3] = 0
         blk[:, :9 * cp] = r.reshape(cout, -1)
         v[ow:ow + cout * kp] = blk.ravel()
         v[ob:ob + cout] = rng.standard_normal(cout)
@@ -50,7 +48,7 @@ def test_reference_padding_gradients_are_zero():
     lay, _, _ = U.layout()
     ow, ob, cout, cp, kp = lay[0]
     g0 = g[ow:ow + cout * kp].reshape(cout, kp)
-    assert np.all(g0[:, 9 * cp:] == 0) and np.all(g0[:, :9 * cp].reshape(cout, 9, cp)[:, :, 3] == 0)
+    assert kp > 9 * cp and np.all(g0[:, 9 * cp:] == 0)
 
 
 def test_bf16_rounding_is_nearest_even():
@@ -69,4 +67,23 @@ def test_layout_matches_library():
 
     _, _, total = U.layout()
     assert hip.unet_param_storage() == total
-    assert hip.unet_param_count() == U.param_count() == 302849
+    assert hip.unet_param_count() == U.param_count() == 302993
+
+
+def test_reference_pyramid_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(12)
+    p = U.init_params(4).astype(np.float64)
+    x = rng.standard_normal((2, 32, 32, 4))
+    y = rng.standard_normal((2, 32, 32))
+    w = rng.random((2, 32, 32))
+    _, g, _ = U.loss_and_grads(p, x, y, w, pyramid=0.7)
+    _, hoff, total = U.layout()
+    v = np.zeros(total)
+    v[hoff:total] = rng.standard_normal(total - hoff)  # head: smooth in the loss
+    lp, _, _ = U.loss_and_grads(p + 1e-6 * v, x, y, w, pyramid=0.7)
+    lm, _, _ = U.loss_and_grads(p - 1e-6 * v, x, y, w, pyramid=0.7)
+    assert abs((lp - lm) / 2e-6 - g @ v) < 1e-6 * abs(g @ v)
+    # the multi-scale term of a constant error: each level adds the same amount
+    l0, _, out = U.loss_and_grads(p, x, y, np.ones_like(w))
+    l1, _, _ = U.loss_and_grads(p, x, out - 0.25, np.ones_like(w), pyramid=1.0)
+    assert abs(l1 - 5 * 0.25 ** 2) < 1e-12
