@@ -150,6 +150,91 @@ typedef __attribute__((address_space(1))) const v4f gcf4;
 typedef __attribute__((address_space(1))) v4f gf4;
 typedef __attribute__((address_space(1))) float gf1;
 
+// One ring period of U row steps of k_sor (see there). HEAD: the period
+// holds rows t < 2K - 1, the only ones with updates outside the dependence
+// cone (rows i < s); only such periods test for it.
+template <int K, int PF, bool HEAD>
+__device__ __forceinline__ void sor_period(int t0, int nsteps, int nrows, int hb, size_t pitch,
+                                           float (&E)[K + PF + 1][4], float (&G)[K + PF + 1][4],
+                                           float (&acc)[K], const float (&cs)[K],
+                                           const float (&cw)[4], bool lane_full, bool lane_any,
+                                           float dg, gcf4 *erow, gcf4 *grow, float *ob) {
+  constexpr int U = K + PF + 1;
+  constexpr int CPL = 4;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int t = t0 + u;
+    if (t < nsteps) {
+      if (t + PF < nrows) {  // row t+PF goes into the slot of row t+PF-U (< t-K, dead)
+        const int sl = (u + PF) % U;
+        const size_t off = (size_t)(t + PF) * (pitch / 4);
+        const v4f ev = erow[off];
+        const v4f gv = grow[off];
+        E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
+        G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
+      }
+      // Stage s on row i = t - s + 1. An update at (row i, stage s) reaches
+      // the valid core rows [K, K + hb) at stage K only if s <= i (the rows
+      // below the core are cut by the loop bound). The triangle i < s (a
+      // tenth of the row-stages at K = 16) is skipped in the HEAD periods;
+      // the test is uniform across the wave, and the skipped updates are
+      // closed under dependence, so no core value or residual sum changes.
+#pragma unroll
+      for (int s = 1; s <= K; ++s) {
+        if (HEAD && t - s + 1 < s) continue;
+        const int si = ((u - s + 1) % U + U) % U;
+        const int su = ((u - s) % U + U) % U;
+        const int sd = ((u - s + 2) % U + U) % U;
+        const int colour = (s - 1) & 1;
+        const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
+        const int i = t - s + 1;
+        const float rw = (i >= K && i < K + hb) ? 1.0f : 0.0f;  // valid-core row
+        float *Er = E[si];
+        const float *Eu = E[su];
+        const float *Ed = E[sd];
+        const float *Gr = G[si];
+        const float c = cs[s - 1];
+        float ra, rb;
+        if (j0 == 0) {
+          const float l0 = from_left(Er[3]);
+          ra = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
+          rb = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
+        } else {
+          const float r3n = from_right(Er[0]);
+          ra = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
+          rb = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
+        }
+        if constexpr (K % 4 == 0) {
+          // every lane is wholly inside or outside the core: the row test is
+          // uniform across the wave and the lane weight is applied at the end
+          if (i >= K && i < K + hb) {
+            acc[s - 1] = fmaf(ra, ra, acc[s - 1]);
+            acc[s - 1] = fmaf(rb, rb, acc[s - 1]);
+          }
+        } else {
+          acc[s - 1] = fmaf(ra * (rw * cw[j0]), ra, acc[s - 1]);
+          acc[s - 1] = fmaf(rb * (rw * cw[j0 + 2]), rb, acc[s - 1]);
+        }
+      }
+      // row t-K+1 has had all of its half-sweeps
+      const int w = t - K + 1;
+      if (w >= K && w < K + hb) {
+        const int sw = ((u - K + 1) % U + U) % U;
+        float *dst = ob + (size_t)w * pitch;
+        if (K % 4 == 0) {  // every lane is wholly inside or outside the core
+          if (lane_full) *(gf4 *)dst = v4f{E[sw][0], E[sw][1], E[sw][2], E[sw][3]};
+        } else if (lane_full) {
+          *(gf4 *)dst = v4f{E[sw][0], E[sw][1], E[sw][2], E[sw][3]};
+        } else if (lane_any) {
+#pragma unroll
+          for (int j = 0; j < CPL; ++j)
+            if (cw[j] != 0.0f) ((gf1 *)dst)[j] = E[sw][j];
+        }
+      }
+    }
+  }
+}
+
 // PF: rows loaded ahead of use (odd, so the ring size stays even and the
 // red/black colour of a ring slot is fixed at compile time).
 template <int K, int PF>
@@ -210,71 +295,17 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
     G[r][0] = gv.x; G[r][1] = gv.y; G[r][2] = gv.z; G[r][3] = gv.w;
   }
 
-  for (int t0 = 0; t0 < nsteps; t0 += U) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = t0 + u;
-      if (t < nsteps) {
-        if (t + PF < nrows) {  // row t+PF goes into the slot of row t+PF-U (< t-K, dead)
-          const int sl = (u + PF) % U;
-          const size_t off = (size_t)(t + PF) * (pitch / 4);
-          const v4f ev = erow[off];
-          const v4f gv = grow[off];
-          E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
-          G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
-        }
-        // Stage s on row i = t - s + 1. Rows above the tile (i < 0, zero slots)
-        // and rows i < s never reach the valid core; updating them anyway keeps
-        // the loop free of branches.
-#pragma unroll
-        for (int s = 1; s <= K; ++s) {
-          const int si = ((u - s + 1) % U + U) % U;
-          const int su = ((u - s) % U + U) % U;
-          const int sd = ((u - s + 2) % U + U) % U;
-          const int colour = (s - 1) & 1;
-          const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;  // first column of this colour
-          const int i = t - s + 1;
-          const float rw = (i >= K && i < K + hb) ? 1.0f : 0.0f;  // valid-core row
-          float *Er = E[si];
-          const float *Eu = E[su];
-          const float *Ed = E[sd];
-          const float *Gr = G[si];
-          const float c = cs[s - 1];
-          if (j0 == 0) {
-            const float l0 = from_left(Er[3]);
-            const float r0 = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
-            const float r2 = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
-            acc[s - 1] = fmaf(r0 * (rw * cw[0]), r0, acc[s - 1]);
-            acc[s - 1] = fmaf(r2 * (rw * cw[2]), r2, acc[s - 1]);
-          } else {
-            const float r3n = from_right(Er[0]);
-            const float r1 = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
-            const float r3 = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
-            acc[s - 1] = fmaf(r1 * (rw * cw[1]), r1, acc[s - 1]);
-            acc[s - 1] = fmaf(r3 * (rw * cw[3]), r3, acc[s - 1]);
-          }
-        }
-        // row t-K+1 has had all of its half-sweeps
-        const int w = t - K + 1;
-        if (w >= K && w < K + hb) {
-          const int sw = ((u - K + 1) % U + U) % U;
-          float *dst = ob + (size_t)w * pitch;
-          if (K % 4 == 0) {  // every lane is wholly inside or outside the core
-            if (lane_full) *(gf4 *)dst = v4f{E[sw][0], E[sw][1], E[sw][2], E[sw][3]};
-          } else if (lane_full) {
-            *(gf4 *)dst = v4f{E[sw][0], E[sw][1], E[sw][2], E[sw][3]};
-          } else if (lane_any) {
-#pragma unroll
-            for (int j = 0; j < CPL; ++j)
-              if (cw[j] != 0.0f) ((gf1 *)dst)[j] = E[sw][j];
-          }
-        }
-      }
-    }
-  }
+  int t0 = 0;
+  for (; t0 < nsteps && t0 < 2 * K - 1; t0 += U)
+    sor_period<K, PF, true>(t0, nsteps, nrows, hb, pitch, E, G, acc, cs, cw, lane_full, lane_any,
+                            dg, erow, grow, ob);
+  for (; t0 < nsteps; t0 += U)
+    sor_period<K, PF, false>(t0, nsteps, nrows, hb, pitch, E, G, acc, cs, cw, lane_full, lane_any,
+                             dg, erow, grow, ob);
 #pragma unroll
   for (int s = 0; s < K; ++s) {
-    const double tot = wave_sum((double)acc[s]);
+    const float a = (K % 4 == 0) ? (lane_full ? acc[s] : 0.0f) : acc[s];
+    const double tot = wave_sum((double)a);
     if (lane == 0) partials[(size_t)s * ntiles + b] = tot;  // stage-major: coalesced sums
   }
 }
