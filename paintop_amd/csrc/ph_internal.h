@@ -82,7 +82,6 @@ struct PhCtl {
   uint32_t pad_;
 };
 
-// SOR factor of half-sweep h (0-based, counted from the first sweep).
 // SOR factor of half-sweep h (0-based). Both half-sweeps of sweep k = h/2
 // use one factor. After a warm start, sweep k < ramp_sweeps uses
 // 2 - (2 - omega)^(k / ramp_sweeps): Gauss-Seidel first (the high-frequency
