@@ -28,7 +28,9 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -258,7 +260,10 @@ static double coarse_min_eigenvalue(const std::vector<uint8_t> &in, int cw, int 
     for (int k = 0; k < m; ++k) r[k] = v[k] - q[k];
     p = r;
     double rr = dot(r, r);
-    const double stop = 1e-16 * dot(v, v);
+    // relative residual 1e-6: inexact inverse iteration still converges to the
+    // smallest eigenpair (each solve starts at v / mu), and omega needs mu to
+    // about 1e-3 only
+    const double stop = 1e-12 * dot(v, v);
     for (int cg = 0; cg < 4 * m + 100 && rr > stop; ++cg) {
       apply(p, q);
       const double alpha = rr / dot(p, q);
@@ -510,21 +515,30 @@ static double mu_bound(double n_interior, int bw, int bh) {
   return std::min(std::max(mu_disc, mu_rect), 4.0);
 }
 
-// mu from a coarse mask of interior counts (cached per coarse mask).
-static double mu_from_counts(const int *counts, int f, int cw, int ch) {
-  std::vector<uint8_t> in((size_t)cw * ch);
+// The coarse mask of interior counts (a block is inside when at least half of
+// its cells are interior) and its cache key.
+static TuneKey coarse_mask(const int *counts, int f, int cw, int ch, std::vector<uint8_t> &in) {
+  in.assign((size_t)cw * ch, 0);
   uint64_t hsh = 1469598103934665603ull;  // FNV-1a of the coarse mask
   for (size_t i = 0; i < in.size(); ++i) {
     in[i] = 2 * counts[i] >= f * f ? 1 : 0;
     hsh = (hsh ^ in[i]) * 1099511628211ull;
   }
-  const TuneKey key{cw, ch, f, hsh};
+  return TuneKey{cw, ch, f, hsh};
+}
+
+static bool tune_lookup(const TuneKey &key, double &mu) {
   for (const auto &e : g_tune_cache)
-    if (e.first == key) return e.second;
-  const double mu = coarse_min_eigenvalue(in, cw, ch) / ((double)f * f);
+    if (e.first == key) {
+      mu = e.second;
+      return true;
+    }
+  return false;
+}
+
+static void tune_store(const TuneKey &key, double mu) {
   if (g_tune_cache.size() > 4096) g_tune_cache.erase(g_tune_cache.begin());
   g_tune_cache.push_back({key, mu});
-  return mu;
 }
 
 // Upper bound of one image's workspace, so the pool never regrows mid-call.
@@ -688,13 +702,53 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         w.live.push_back(w.origins[k]);
         w.n_interior += counts[w.origin_off + k];
       }
-    w.omega = d->omega;
-    if (tuned) {
-      const double mu = w.n_interior >= PRECISE_TUNE_MIN_CELLS
-                            ? mu_from_counts(coarse.data() + w.coarse_off, w.cf, w.cw, w.ch)
-                            : mu_bound(w.n_interior, w.rw, w.rh);
-      w.omega = omega_for(mu, d->lambda);
+  }
+  // SOR factor per image: the coarse-mask eigenvalue for large domains (cached
+  // per coarse mask; the masks not in the cache are solved on host threads in
+  // parallel), the geometric bound for small ones
+  std::vector<double> mu_of(nitem, 0.0);
+  if (tuned) {
+    struct Job { int item; TuneKey key; std::vector<uint8_t> in; double mu; };
+    std::vector<Job> jobs;
+    for (int i = 0; i < nitem; ++i) {
+      ItemWork &w = iw[i];
+      if (!w.any) continue;
+      if (w.n_interior < PRECISE_TUNE_MIN_CELLS) {
+        mu_of[i] = mu_bound(w.n_interior, w.rw, w.rh);
+        continue;
+      }
+      Job j;
+      j.item = i;
+      j.key = coarse_mask(coarse.data() + w.coarse_off, w.cf, w.cw, w.ch, j.in);
+      j.mu = -1.0;
+      if (tune_lookup(j.key, mu_of[i])) continue;
+      jobs.push_back(std::move(j));
     }
+    if (!jobs.empty()) {
+      const int nth = (int)std::min<size_t>(jobs.size(),
+                                            std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+      std::atomic<size_t> next{0};
+      auto worker = [&]() {
+        for (size_t q; (q = next.fetch_add(1)) < jobs.size();) {
+          Job &j = jobs[q];
+          const ItemWork &w = iw[j.item];
+          j.mu = coarse_min_eigenvalue(j.in, w.cw, w.ch) / ((double)w.cf * w.cf);
+        }
+      };
+      std::vector<std::thread> pool;
+      for (int t = 1; t < nth; ++t) pool.emplace_back(worker);
+      worker();
+      for (auto &th : pool) th.join();
+      for (const Job &j : jobs) {
+        mu_of[j.item] = j.mu;
+        tune_store(j.key, j.mu);
+      }
+    }
+  }
+  for (int i = 0; i < nitem; ++i) {
+    ItemWork &w = iw[i];
+    if (!w.any) continue;
+    w.omega = tuned ? omega_for(mu_of[i], d->lambda) : d->omega;
     w.stall_window = stall_window_for(w.omega, tuned);
     w.in.rep->omega_used = w.omega;
   }
