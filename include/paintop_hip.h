@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PH_ABI_VERSION 1
+#define PH_ABI_VERSION 2
 
 typedef enum ph_status {
   PH_OK = 0,
@@ -84,7 +84,7 @@ typedef struct ph_solve_desc {
   uint32_t max_iterations; /* 1..=100000 (poisson_blend.rs:71) */
   double tolerance;        /* (0, 1) */
   double lambda;           /* >= 0, finite; 0 for PH_OP_POISSON_BLEND */
-  double omega;            /* fixed SOR factor in (0, 2); 0 = Chebyshev schedule to the tuned value */
+  double omega;            /* fixed SOR factor in (0, 2); 0 = tuned per instance from the mask's coarse spectral radius */
   uint32_t flags;          /* PH_FLAG_* */
   uint32_t half_sweeps;    /* red/black half-sweeps per HBM round trip; 0 = tuned */
   int32_t op;              /* ph_op */
@@ -114,6 +114,8 @@ typedef struct ph_report {
   uint32_t warm_start_used;  /* channels that kept the learned start (0: plain start) */
   double warm_start_rel;     /* worst channel's ||r(warm start)|| / ||r(anchor start)||, 0 if not tried */
   double sweep_ms;           /* hipEvent time of the sweep-kernel batches (PH_FLAG_TIMING) */
+  uint32_t coarse_corrections; /* learned coarse corrections kept, all channels */
+  uint32_t reserved_;
 } ph_report;
 
 typedef struct ph_device_info {
@@ -278,6 +280,17 @@ ph_status ph_unet_get_state(ph_unet *net, float *params, float *grads, float *ad
 ph_status ph_unet_set_state(ph_unet *net, const float *params, const float *adam_m,
                             const float *adam_v, int32_t step, ph_error *err);
 
+/* Training batch from a device-resident dataset of n_samples examples
+ * (inputs [n][res][res][4], targets and weights [n][res][res]): example
+ * index[j] (host array) under the dihedral map mode[j] (host array or NULL;
+ * bit 2 transpose, bit 0 mirror x, bit 1 mirror y) becomes batch entry j,
+ * for j < batch <= 256. */
+ph_status ph_unet_gather_device(const float *d_in_all, const float *d_target_all,
+                                const float *d_weight_all, uint64_t n_samples,
+                                const int32_t *index, const int32_t *mode, int32_t batch,
+                                int32_t res, float *d_in, float *d_target, float *d_weight,
+                                void *stream, ph_error *err);
+
 /* The weights the solver's warm start (PH_FLAG_WARM_START_NET) runs with;
  * params = NULL clears them. */
 ph_status ph_warm_start_set_weights(const float *params, uint64_t n_params, ph_error *err);
@@ -286,20 +299,27 @@ ph_status ph_warm_start_set_weights(const float *params, uint64_t n_params, ph_e
  * training target): the region [x0, x0 + res*f) x [y0, y0 + res*f) of the
  * image in f x f blocks. With interior = (mask > 0.5 and not on the image
  * border) (poisson.rs:340-355), ring = non-interior cells with an interior
- * 4-neighbour (the Dirichlet data) and P = source, per block:
- *   in[..][0] = sum over interior cells of lambda*(P - anchor)
+ * 4-neighbour (the Dirichlet data), per block:
+ *   in[..][0] = sum over interior cells of rhs
  *   in[..][1] = mean over ring cells of (anchor - P), 0 without ring cells
  *   in[..][2] = interior cells / f^2
  *   in[..][3] = ring cells / f
  *   target    = mean over interior and ring cells of (solution - P)
  *   weight    = 1 with interior cells, 0.5 with ring cells only, else 0.
+ * mode 0 (warm start): P = source, rhs = lambda*(P - anchor).
+ * mode 1 (coarse correction): P = 0, rhs = the source's 5-point Laplacian
+ * (poisson.rs:364-373); with a zero anchor this is the correction problem
+ * Laplacian(e) = g, e = 0 on the ring, that the solver's coarse corrections
+ * hand to the network. The solver divides channels 0 and 1 of each input by
+ * their RMS over the grid and scales the prediction back; this entry point
+ * returns them unscaled.
  * d_in is [res][res][4], d_target / d_weight [res][res]. */
 ph_status ph_warm_coarsen_device(const float *d_source, const float *d_anchor,
                                  const float *d_mask, const float *d_solution, uint32_t width,
                                  uint32_t height, uint32_t channels, uint32_t channel,
                                  double lambda, int32_t x0, int32_t y0, int32_t factor,
-                                 int32_t res, float *d_in, float *d_target, float *d_weight,
-                                 void *stream, ph_error *err);
+                                 int32_t res, int32_t mode, float *d_in, float *d_target,
+                                 float *d_weight, void *stream, ph_error *err);
 
 #ifdef __cplusplus
 }

@@ -181,21 +181,40 @@ __global__ void k_apply_correction(PhInst I, int buf) {
   }
 }
 
-// g = -r(x) in FP64, stored FP32 (NaN at non-interior cells); per-row sums of
-// r^2 in a fixed order; the last row block of the instance adds the rows in
-// order and stores ||r|| in out[0] and ||r||^2 in out[1].
-__global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__ row_sums,
-                                                  uint32_t *__restrict__ arrival,
-                                                  double *__restrict__ out) {
+// g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and the per-row
+// sums of r^2 in a fixed order; k_residual_norm adds the rows in order and
+// stores ||r|| in out[0] and ||r||^2 in out[1]. (A last-block reduction inside
+// this kernel would cost one device-scope atomic per row block, which
+// serialise across the XCDs and took longer than the rows themselves.)
+// One block per plane row. Each thread takes cell pairs (px, px + 1), px even,
+// so the loads of a pair are independent 16-byte vectors; rows 0 and ph - 1
+// and columns 0 and pw - 1 are padding (no interior cells), so every
+// neighbour read stays inside the plane. Non-interior cells of a pair get
+// g = NaN, the value they hold already.
+__global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__ row_sums) {
   const int py = blockIdx.x;
   double s = 0.0;
-  for (int px = threadIdx.x; px < I.pw; px += blockDim.x) {
-    const size_t p = (size_t)py * I.pitch + px;
-    if (I.cls[p]) {
-      const double nsum = ((I.x[p - 1] + I.x[p + 1]) + I.x[p - I.pitch]) + I.x[p + I.pitch];
-      const double r = fma(I.dg64, -I.x[p], nsum) - I.bp[p];
-      I.g[p] = (float)(-r);
-      s = fma(r, r, s);
+  if (py >= 1 && py < I.ph - 1) {
+    const size_t row = (size_t)py * I.pitch;
+    for (int px = 2 * threadIdx.x; px < I.pw; px += 2 * blockDim.x) {
+      const size_t p = row + px;
+      const uchar2 c = *reinterpret_cast<const uchar2 *>(I.cls + p);
+      if (!(c.x | c.y)) continue;
+      const double2 xc = *reinterpret_cast<const double2 *>(I.x + p);
+      const double2 xu = *reinterpret_cast<const double2 *>(I.x + p - I.pitch);
+      const double2 xd = *reinterpret_cast<const double2 *>(I.x + p + I.pitch);
+      const double2 b = *reinterpret_cast<const double2 *>(I.bp + p);
+      const double xl = I.x[p - 1], xr = I.x[p + 2];
+      const double n0 = ((xl + xc.y) + xu.x) + xd.x;
+      const double n1 = ((xc.x + xr) + xu.y) + xd.y;
+      const double r0 = fma(I.dg64, -xc.x, n0) - b.x;
+      const double r1 = fma(I.dg64, -xc.y, n1) - b.y;
+      float2 gv;
+      gv.x = c.x ? (float)(-r0) : __builtin_nanf("");
+      gv.y = c.y ? (float)(-r1) : __builtin_nanf("");
+      *reinterpret_cast<float2 *>(I.g + p) = gv;
+      if (c.x) s = fma(r0, r0, s);
+      if (c.y) s = fma(r1, r1, s);
     }
   }
   __shared__ double red[256];
@@ -205,31 +224,27 @@ __global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__
     if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  __shared__ uint32_t ticket;
-  if (threadIdx.x == 0) {
-    row_sums[py] = red[0];
-    __threadfence();
-    ticket = atomicAdd(arrival, 1u);
-  }
+  if (threadIdx.x == 0) row_sums[py] = red[0];
+}
+
+// ||r|| from the row sums of k_residual, added in a fixed order: thread t
+// takes a contiguous run of rows, then a tree over the threads. One block.
+__global__ __launch_bounds__(256) void k_residual_norm(const double *__restrict__ row_sums,
+                                                       int nrows, double *__restrict__ out) {
+  __shared__ double red[256];
+  const int per = (nrows + 255) / 256;
+  double v = 0.0;
+  for (int r = threadIdx.x * per; r < min(nrows, (int)(threadIdx.x + 1) * per); ++r)
+    v += row_sums[r];
+  red[threadIdx.x] = v;
   __syncthreads();
-  if (ticket == (uint32_t)(gridDim.x - 1)) {
-    __threadfence();
-    // fixed-order sum of the row sums: thread t takes a contiguous run of rows
-    const int n = gridDim.x;
-    const int per = (n + 255) / 256;
-    double v = 0.0;
-    for (int r = threadIdx.x * per; r < min(n, (int)(threadIdx.x + 1) * per); ++r) v += row_sums[r];
-    red[threadIdx.x] = v;
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-      if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      out[0] = sqrt(red[0]);
-      out[1] = red[0];
-      *arrival = 0u;
-    }
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(red[0]);
+    out[1] = red[0];
   }
 }
 

@@ -33,7 +33,7 @@ struct PhInst {
   double omega;        // SOR factor (after the ramp)
   double ramp_log;     // ln(2 - omega) / ramp_sweeps
   uint32_t ramp_sweeps; // sweeps of the warm-start ramp (0: none)
-  uint32_t pad0_;
+  uint32_t ramp_from;   // first sweep of the current ramp (it restarts after a coarse correction)
   float dg;         // 4 + lambda (FP32)
   double dg64;      // 4 + lambda (FP64)
   double lambda;
@@ -83,13 +83,36 @@ struct PhCtl {
 };
 
 // SOR factor of half-sweep h (0-based). Both half-sweeps of sweep k = h/2
-// use one factor. After a warm start, sweep k < ramp_sweeps uses
-// 2 - (2 - omega)^(k / ramp_sweeps): Gauss-Seidel first (the high-frequency
-// error the warm start leaves decays fastest there), then omega.
+// use one factor. After a warm start or a coarse correction (at sweep
+// ramp_from), sweep k in [ramp_from, ramp_from + ramp_sweeps) uses
+// 2 - (2 - omega)^((k - ramp_from) / ramp_sweeps): Gauss-Seidel first (the
+// high-frequency error a coarse prediction leaves decays fastest there), then
+// omega.
 __host__ __device__ inline double ph_omega_at(const PhInst &I, uint64_t h) {
   const uint64_t k = h >> 1;
-  if (k >= I.ramp_sweeps) return I.omega;
-  return 2.0 - exp((double)k * I.ramp_log);
+  if (k < I.ramp_from || k - I.ramp_from >= I.ramp_sweeps) return I.omega;
+  return 2.0 - exp((double)(k - I.ramp_from) * I.ramp_log);
+}
+
+// Sweep m (1-based) runs inside an omega ramp. The residual of a ramp is not
+// monotone (it dips while omega is near 1, then rises as omega grows and the
+// smooth error shows), so the stall guard holds still during one.
+__host__ __device__ inline bool ph_in_ramp(const PhInst &I, uint32_t m) {
+  const uint32_t k = m - 1;
+  return I.ramp_sweeps != 0 && k >= I.ramp_from && k - I.ramp_from < I.ramp_sweeps;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of work items (tiles, rows) so neighbours, which share
+// halo rows, hit the same L2. A bijection of [0, n) for b < n.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int per = (n + 7) >> 3;
+  const int xcd = b & 7, slot = b >> 3;
+  const int full = n - (per - 1) * 8;  // XCDs that get `per` items
+  int t;
+  if (xcd < full) t = xcd * per + slot;
+  else t = full * per + (xcd - full) * (per - 1) + slot;
+  return t;
 }
 
 struct PhTile {

@@ -52,11 +52,18 @@ __global__ void k_tile_occupancy(const uint8_t *, int, int, int, int, int, int, 
 __global__ void k_coarse_count(const uint8_t *, int, int, int, int, int, int, int, int *);
 __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
 __global__ void k_apply_correction(PhInst, int);
-__global__ void k_residual(PhInst, double *, uint32_t *, double *);
+__global__ void k_residual(PhInst, double *);
+__global__ void k_residual_norm(const double *, int, double *);
 __global__ void k_scatter(PhInst, float *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
-                          int, double, int, int, int, int, float *, float *, float *);
-__global__ void k_warm_apply(PhInst, const float *, const float *, int, int, int, int);
+                          int, double, int, int, int, int, int, float *, float *, float *);
+__global__ void k_warm_apply(PhInst, const float *, const float *, const float *, int, int, int,
+                             int);
+__global__ void k_coarsen_plane(PhInst, int, int, int, int, float *);
+__global__ void k_in_norm(float *, int, float *);
+__global__ void k_correct_apply(PhInst, const float *, const float *, const double *, int, int,
+                                int, int, double);
+__global__ void k_corr_alpha(const float *, const float *, int, double, double *);
 __global__ void k_start_apply(PhInst, const float *);
 bool warm_net_ready();
 int warm_net_res();
@@ -298,8 +305,11 @@ static double omega_for(double mu, double lambda) {
 // omega <= 1.81). A fixed omega from the caller keeps the reference's 16.
 // Sweeps of the omega ramp after a kept warm start: a multiple of 1/(2 - omega),
 // which grows with the domain's size like the SOR transient itself.
-static uint32_t warm_ramp_sweeps(double omega) {
-  double fct = 1.0;  // measured best for the shipped network (2048^2 and 4096^2 blends)
+// Measured best for the shipped network on 2048^2 and 4096^2 blends: 1 for a
+// warm start alone, 1/4 with coarse-correction cycles (each cycle starts over
+// from a fresh correction, so a short ramp that reaches omega early pays).
+static uint32_t warm_ramp_sweeps(double omega, bool cycles) {
+  double fct = cycles ? 0.25 : 1.0;
   if (const char *e = getenv("PAINTOP_HIP_RAMP_FACTOR")) fct = atof(e);
   if (!(fct >= 0.0)) fct = 0.0;
   return (uint32_t)std::min(1e5, std::ceil(fct / (2.0 - omega)));
@@ -312,8 +322,42 @@ static uint32_t warm_probe_sweeps(uint32_t ramp) {
   return std::min<uint32_t>(96, std::max<uint32_t>(8, ramp / 4));
 }
 // A warm start whose initial residual exceeds the anchor start's this many
-// times is dropped without a race.
-static const double WARM_MAX_START_RATIO = 1e6;
+// times is dropped without a race. A coarse prediction leaves the fine
+// boundary texture unresolved, so good starts sit at 5-8x (the shipped
+// network, and even the exact coarse solution); far above that the start
+// carries large error everywhere, which a short probe can fail to expose.
+static const double WARM_MAX_START_RATIO = 32.0;
+
+// Learned coarse corrections. After a kept warm start, every cycle of sweeps
+// ends in an FP64 refinement whose residual g = -r(x) defines the correction
+// problem Laplacian(e) = g with e = 0 on the ring; the network solves it on
+// the warm start's coarse grid and x += bilinear(e), after which the omega
+// ramp restarts. The cycle is a multiple of the ramp (PAINTOP_HIP_CYCLE_FACTOR,
+// default 2), on a launch boundary; PAINTOP_HIP_CORRECTIONS caps the count per
+// channel (0 = none).
+static uint32_t warm_max_corrections() {
+  long n = 16;
+  if (const char *e = getenv("PAINTOP_HIP_CORRECTIONS")) n = atol(e);
+  return (uint32_t)std::max(0L, std::min(n, 1000L));
+}
+static uint32_t warm_cycle_sweeps(uint32_t ramp, uint32_t half) {
+  double fct = 2.0;
+  if (const char *e = getenv("PAINTOP_HIP_CYCLE_FACTOR")) fct = atof(e);
+  if (!(fct > 0.0)) fct = 1.0;
+  const uint32_t n = (uint32_t)std::min(1e5, std::ceil(fct * ramp));
+  return std::max<uint32_t>(2 * half, (n + half - 1) / half * half);
+}
+// Corrections go on while each cycle ends at least this many times below the
+// end of the previous one (plain SOR gains about e per 1/(2 - omega) sweeps).
+static const double CORR_MIN_GAIN = 4.0;
+// Largest step of a correction (its Galerkin step on the coarse grid is
+// clamped to this).
+static const double CORR_MAX_STEP = 1.5;
+// A good correction leaves the residual within about 1.6x of the uncorrected
+// one (bilinear interpolation error at the fine scale, which the next sweeps
+// remove). One that raises it more than this is taken back; the cycle then
+// runs on uncorrected, and a second one taken back in a row ends the cycles.
+static const double CORR_MAX_JUMP = 4.0;
 
 static uint32_t stall_window_for(double omega, bool tuned) {
   if (!tuned) return PH_STALL_WINDOW;
@@ -588,7 +632,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       const size_t R2 = (size_t)warm_net_res() * warm_net_res();
       size_t nch = 0;
       for (const ItemIn &it : items) nch += it.C;
-      need += (R2 * 4 * warm_net_batch() + R2 * nch) * sizeof(float) + 1024;
+      need += (R2 * 4 * warm_net_batch() + (R2 + 1) * nch) * sizeof(float) +
+              nch * sizeof(double) + 1280;
     }
     g_pool.reset(need);
   }
@@ -602,6 +647,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     rep->unknown_sweeps = 0;
     rep->warm_start_used = 0;
     rep->warm_start_rel = 0.0;
+    rep->coarse_corrections = 0;
+    rep->reserved_ = 0;
     rep->sweep_ms = 0.0;
     rep->device_ms = 0.0;
     rep->half_sweeps_used = (uint32_t)K;
@@ -799,12 +846,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int ninst = (int)inst.size();  // the images' instances (twins come after)
   const int ncap = std::max(race_any ? 2 * ninst : ninst, 1);
   double *resv_d = g_pool.take<double>(2 * (size_t)ncap);
-  uint32_t *arr_d = g_pool.take<uint32_t>((size_t)ncap);
-  HIP_OK(hipMemsetAsync(arr_d, 0, ncap * sizeof(uint32_t), st));
   std::vector<double> resv(2 * (size_t)ncap);
   auto launch_residual = [&](int k) {
     const InstWork &X = inst[k];
-    hipLaunchKernelGGL(k_residual, dim3(X.I.ph), dim3(256), 0, st, X.I, X.rowsum, arr_d + k,
+    hipLaunchKernelGGL(k_residual, dim3(X.I.ph), dim3(256), 0, st, X.I, X.rowsum);
+    hipLaunchKernelGGL(k_residual_norm, dim3(1), dim3(256), 0, st, X.rowsum, X.I.ph,
                        resv_d + 2 * k);
     HIP_OK(hipGetLastError());
   };
@@ -848,6 +894,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   std::vector<double> rstart;
   std::vector<int> twin, probe;
   std::vector<char> is_twin, silent;
+  // coarse-correction cycles of network instances: next cycle end (0 = none),
+  // cycle length, corrections kept, residual at the previous cycle end
+  std::vector<uint32_t> cyc_next, cyc_len, cyc_count, cyc_undone;
+  std::vector<double> cyc_prev;
   auto grow = [&]() {
     const size_t n = inst.size();
     rstart.resize(n, 0.0);
@@ -855,6 +905,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     probe.resize(n, 0);
     is_twin.resize(n, 0);
     silent.resize(n, 0);
+    cyc_next.resize(n, 0);
+    cyc_len.resize(n, 0);
+    cyc_count.resize(n, 0);
+    cyc_undone.resize(n, 0);
+    cyc_prev.resize(n, 0.0);
   };
   grow();
   for (int k = 0; k < ninst; ++k) rstart[k] = inst[k].r0;
@@ -868,39 +923,50 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       net_list.push_back(k);
     }
   }
+  // coarse-grid framing of each network instance (kept for the corrections)
+  struct Geo { int x0, y0, f; };
+  std::vector<Geo> wgeo(ninst, Geo{0, 0, 1});
+  const int NR = warm_net_res(), NNB = warm_net_batch();
+  const size_t NR2 = (size_t)NR * NR;
+  float *net_in = nullptr, *net_h = nullptr, *net_scale = nullptr;
+  double *corr_alpha = nullptr;
+  if (!net_list.empty()) {
+    net_in = g_pool.take<float>(NR2 * 4 * NNB);
+    // one prediction and one input scale per network instance (the warm
+    // starts, then the corrections of any subset of them)
+    net_h = g_pool.take<float>(NR2 * net_list.size());
+    net_scale = g_pool.take<float>(net_list.size());
+    corr_alpha = g_pool.take<double>(net_list.size());
+  }
+  const uint32_t max_corr = warm_max_corrections();
   if (!cand.empty()) {
     const int nw = (int)cand.size();
     if (!net_list.empty()) {
-      const int R = warm_net_res(), NB = warm_net_batch();
-      const size_t R2 = (size_t)R * R;
       const int nn = (int)net_list.size();
-      float *in_d = g_pool.take<float>(R2 * 4 * NB);
-      float *h_d = g_pool.take<float>(R2 * nn);
-      struct Geo { int x0, y0, f; };
-      std::vector<Geo> geo(nn);
-      for (int j = 0; j < nn; ++j) {
-        const ItemWork &w = iw[inst[net_list[j]].item];
-        const int f = std::max(1, (std::max(w.rw, w.rh) + R - 1) / R);
-        geo[j] = Geo{w.rx0 + w.rw / 2 - (R * f) / 2, w.ry0 + w.rh / 2 - (R * f) / 2, f};
+      for (int k : net_list) {
+        const ItemWork &w = iw[inst[k].item];
+        const int f = std::max(1, (std::max(w.rw, w.rh) + NR - 1) / NR);
+        wgeo[k] = Geo{w.rx0 + w.rw / 2 - (NR * f) / 2, w.ry0 + w.rh / 2 - (NR * f) / 2, f};
       }
-      for (int j0 = 0; j0 < nn; j0 += NB) {
-        const int nb = std::min(NB, nn - j0);
+      for (int j0 = 0; j0 < nn; j0 += NNB) {
+        const int nb = std::min(NNB, nn - j0);
         for (int j = 0; j < nb; ++j) {
           const InstWork &X = inst[net_list[j0 + j]];
           const ItemIn &it = iw[X.item].in;
-          const Geo &g = geo[j0 + j];
-          hipLaunchKernelGGL(k_coarsen, dim3(R * R), dim3(64), 0, st, it.src, it.tgt, it.mask,
+          const Geo &g = wgeo[net_list[j0 + j]];
+          hipLaunchKernelGGL(k_coarsen, dim3(NR * NR), dim3(64), 0, st, it.src, it.tgt, it.mask,
                              nullptr, it.W, it.H, it.C, X.channel, d->lambda, g.x0, g.y0, g.f,
-                             R, in_d + R2 * 4 * j, nullptr, nullptr);
+                             NR, 0, net_in + NR2 * 4 * j, nullptr, nullptr);
         }
+        hipLaunchKernelGGL(k_in_norm, dim3(nb), dim3(256), 0, st, net_in, NR, net_scale + j0);
         HIP_OK(hipGetLastError());
-        warm_net_forward(in_d, h_d + R2 * j0, nb, st);
+        warm_net_forward(net_in, net_h + NR2 * j0, nb, st);
       }
       for (int j = 0; j < nn; ++j) {
         const InstWork &X = inst[net_list[j]];
-        const Geo &g = geo[j];
+        const Geo &g = wgeo[net_list[j]];
         hipLaunchKernelGGL(k_warm_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.src,
-                           h_d + R2 * j, R, g.x0, g.y0, g.f);
+                           net_h + NR2 * j, net_scale + j, NR, g.x0, g.y0, g.f);
         HIP_OK(hipGetLastError());
       }
     }
@@ -948,7 +1014,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       grow();
       launch_residual(kt);
       InstWork &Xw = inst[k];
-      const uint32_t n = warm_ramp_sweeps(w.omega);
+      const uint32_t n = warm_ramp_sweeps(w.omega, !w.in.start && max_corr > 0);
       Xw.I.ramp_sweeps = n;
       Xw.I.ramp_log = n ? log(2.0 - w.omega) / n : 0.0;
       twin[k] = kt;
@@ -963,6 +1029,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       const uint32_t pr = (warm_probe_sweeps(n) + half - 1) / half * half;
       probe[k] = probe[kt] = (int)std::min<uint32_t>(pr, d->max_iterations);
       act.push_back(kt);
+      if (!w.in.start && max_corr > 0) {
+        // the first cycle ends past the probe, so the race is settled first
+        cyc_len[k] = warm_cycle_sweeps(n, half);
+        cyc_next[k] = std::max<uint32_t>(cyc_len[k], (uint32_t)probe[k] + half);
+      }
     }
   }
 
@@ -1030,7 +1101,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       ctl[a].halt = PH_HALT_DONE;
     };
     // stall rule of the reference for a host-evaluated sweep (poisson.rs:220-228)
-    auto stall_step = [&](PhCtl &Cc, double rel) -> bool {
+    auto stall_step = [&](int a, PhCtl &Cc, uint32_t m, double rel) -> bool {
+      if (ph_in_ramp(ai[a], m)) return false;  // on hold inside an omega ramp
       if (rel < Cc.best * (1.0 - PH_STALL_IMPROVEMENT)) {
         Cc.best = rel;
         Cc.since = 0;
@@ -1039,12 +1111,18 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       Cc.since += 1;
       return Cc.since >= Cc.stall_window;
     };
+    // sweep limit of a running instance: the end of its correction cycle, if any
+    auto run_limit = [&](int a) -> uint32_t {
+      const PhCtl &Cc = ctl[a];
+      const uint32_t c = cyc_next[act[a]];
+      return c > Cc.sweeps_done ? std::min(Cc.max_it, c) : Cc.max_it;
+    };
     auto resume = [&](int a, double gnorm) {
       PhCtl &Cc = ctl[a];
       Cc.last_refine = Cc.sweeps_done;
       Cc.halt = PH_RUN;
       Cc.mode = PH_MODE_NORMAL;
-      Cc.sweep_limit = Cc.max_it;
+      Cc.sweep_limit = run_limit(a);
       Cc.refine_pending = 0;
       Cc.refine_abs2 = (REFINE_RATIO * gnorm) * (REFINE_RATIO * gnorm);
     };
@@ -1079,6 +1157,133 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     std::vector<Trend> trend(nact);
     uint32_t total_launches = 0, total_refine = 0;
     double sweep_ms = 0.0;
+    // learned coarse corrections at the cycle ends collected by pass 2 (all
+    // instances of one pass share the network batches and one readback)
+    struct Corr {
+      int a;
+      uint32_t m;
+      double gnorm;
+    };
+    std::vector<Corr> corr;
+    std::vector<double> corr_stage;
+    auto correct = [&]() {
+      // cycles that stopped paying for themselves end here, without a correction
+      std::vector<Corr> go;
+      for (const Corr &c : corr) {
+        const int k = act[c.a];
+        const double rel = c.gnorm / inst[k].r0;
+        // (after a correction taken back, the cycle ran uncorrected: no test)
+        const bool gain = cyc_count[k] == 0 || cyc_undone[k] || rel * CORR_MIN_GAIN <= cyc_prev[k];
+        cyc_prev[k] = rel;
+        if (!gain || cyc_count[k] >= max_corr || !(c.gnorm > 0.0)) {
+          cyc_next[k] = 0;
+          ctl[c.a].sweep_limit = run_limit(c.a);
+          if (trace_on())
+            fprintf(stderr, "[corr] inst %d: cycles end at sweep %u (rel %.3e, %u kept)\n", k,
+                    c.m, rel, cyc_count[k]);
+          continue;
+        }
+        go.push_back(c);
+      }
+      if (go.empty()) return;
+      const int nc = (int)go.size();
+      for (int c0 = 0; c0 < nc; c0 += NNB) {
+        const int nb = std::min(NNB, nc - c0);
+        for (int j = 0; j < nb; ++j) {
+          const int k = act[go[c0 + j].a];
+          const Geo &g = wgeo[k];
+          hipLaunchKernelGGL(k_coarsen_plane, dim3(NR * NR), dim3(64), 0, st, inst[k].I, g.x0,
+                             g.y0, g.f, NR, net_in + NR2 * 4 * j);
+        }
+        hipLaunchKernelGGL(k_in_norm, dim3(nb), dim3(256), 0, st, net_in, NR, net_scale + c0);
+        HIP_OK(hipGetLastError());
+        warm_net_forward(net_in, net_h + NR2 * c0, nb, st);
+        for (int j = 0; j < nb; ++j)
+          hipLaunchKernelGGL(k_corr_alpha, dim3(1), dim3(1024), 0, st, net_in + NR2 * 4 * j,
+                             net_h + NR2 * (c0 + j), NR, CORR_MAX_STEP, corr_alpha + c0 + j);
+        HIP_OK(hipGetLastError());
+      }
+      for (int j = 0; j < nc; ++j) {
+        const int k = act[go[j].a];
+        const InstWork &X = inst[k];
+        const Geo &g = wgeo[k];
+        hipLaunchKernelGGL(k_correct_apply, X.pgrd, dim3(256), 0, st, X.I, net_h + NR2 * j,
+                           net_scale + j, corr_alpha + j, NR, g.x0, g.y0, g.f, 1.0);
+        HIP_OK(hipGetLastError());
+        launch_residual(k);
+      }
+      std::vector<double> alpha(nc);
+      HIP_OK(hipMemcpyAsync(alpha.data(), corr_alpha, nc * sizeof(double), hipMemcpyDeviceToHost,
+                            st));
+      HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      std::vector<char> undo(nc, 0);
+      std::vector<double> jump(nc, 0.0);
+      bool any_undo = false;
+      for (int j = 0; j < nc; ++j) {
+        const int k = act[go[j].a];
+        jump[j] = resv[2 * k] / go[j].gnorm;
+        if (alpha[j] != 0.0 && !(std::isfinite(jump[j]) && jump[j] <= CORR_MAX_JUMP)) {
+          const InstWork &X = inst[k];
+          const Geo &g = wgeo[k];
+          hipLaunchKernelGGL(k_correct_apply, X.pgrd, dim3(256), 0, st, X.I, net_h + NR2 * j,
+                             net_scale + j, corr_alpha + j, NR, g.x0, g.y0, g.f, -1.0);
+          HIP_OK(hipGetLastError());
+          launch_residual(k);
+          undo[j] = 1;
+          any_undo = true;
+        }
+      }
+      if (any_undo) {
+        HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+      }
+      corr_stage.assign(nc, 0.0);
+      for (int j = 0; j < nc; ++j) {
+        const int a = go[j].a, k = act[a];
+        const uint32_t m = go[j].m;
+        PhCtl &Cc = ctl[a];
+        const double gnorm = resv[2 * k];
+        const double rel = gnorm / Cc.r0;
+        if (trace_on())
+          fprintf(stderr, "[corr] inst %d sweep %u: rel %.3e -> %.3e (step %.3f%s)\n", k, m,
+                  go[j].gnorm / Cc.r0, go[j].gnorm * jump[j] / Cc.r0, alpha[j],
+                  undo[j] ? ", taken back" : "");
+        if (alpha[j] == 0.0 || (undo[j] && cyc_undone[k])) {
+          cyc_next[k] = 0;  // no usable correction: the cycles end
+        } else if (undo[j]) {
+          cyc_undone[k] = 1;
+          cyc_next[k] = m + cyc_len[k];
+        } else {
+          cyc_undone[k] = 0;
+          cyc_count[k] += 1;
+          iw[inst[k].item].in.rep->coarse_corrections += 1;
+          cyc_next[k] = m + cyc_len[k];
+          // the omega ramp restarts at the corrected state
+          ai[a].ramp_from = m;
+          inst[k].I.ramp_from = m;
+          HIP_OK(hipMemcpyAsync(inst_d + a, &ai[a], sizeof(PhInst), hipMemcpyHostToDevice, st));
+          corr_stage[j] = rel;
+          HIP_OK(hipMemcpyAsync(Cc.history + (m - 1), &corr_stage[j], sizeof(double),
+                                hipMemcpyHostToDevice, st));
+          // the residual dips and then rises through the restarted ramp: the
+          // stall guard starts again after it, the decay estimate from here
+          Cc.best = INFINITY;
+          Cc.since = 0;
+          Cc.last_rel = rel;
+          trend[a] = Trend{m, rel};
+        }
+        if (rel <= Cc.tol) {
+          finish(a, PH_STOP_CONVERGED, m, rel);
+          --remaining;
+        } else {
+          resume(a, gnorm);
+        }
+      }
+    };
+
     while (remaining > 0) {
       if (timing) HIP_OK(hipEventRecord(eb0, st));
       launch_sor_batch(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb, batch);
@@ -1153,6 +1358,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         total_refine += (uint32_t)pend.size();
         // pass 2: the reference's rules on the exact FP64 residual of sweep m
         hist_stage.assign(pend.size(), 0.0);
+        corr.clear();
         for (size_t q = 0; q < pend.size(); ++q) {
           const Pending &P = pend[q];
           PhCtl &Cc = ctl[P.a];
@@ -1167,7 +1373,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           const bool paired = twin[k] >= 0 && !resolved[k];
           if (rel <= Cc.tol) finish(P.a, PH_STOP_CONVERGED, P.m, rel);
           else if (P.reason == PH_REASON_STALLED) finish(P.a, PH_STOP_STALLED, P.m, rel);
-          else if (stall_step(Cc, rel)) finish(P.a, PH_STOP_STALLED, P.m, rel);
+          else if (stall_step(P.a, Cc, P.m, rel)) finish(P.a, PH_STOP_STALLED, P.m, rel);
           else if (P.m >= Cc.max_it) finish(P.a, PH_STOP_MAX_ITERATIONS, P.m, rel);
           else resume(P.a, gnorm);
           if (inst[k].R.done) {
@@ -1179,8 +1385,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           } else if (paired && tracing && P.m < (uint32_t)probe[k]) {
             fprintf(stderr, "[race] inst %d (%s) evaluated at sweep %u before the probe\n", k,
                     is_twin[k] ? "plain" : "warm", P.m);
+          } else if (!paired && cyc_next[k] && P.m >= cyc_next[k]) {
+            corr.push_back(Corr{P.a, P.m, gnorm});
           }
         }
+        if (!corr.empty()) correct();
       }
       {
         // warm-start races whose two members both have a value: the smaller
@@ -1207,7 +1416,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
             const int a = slot[win];
             held[a] = 0;
             PhCtl &Cc = ctl[a];
-            Cc.sweep_limit = Cc.max_it;
+            if (win != k) cyc_next[k] = 0;
+            Cc.sweep_limit = run_limit(a);
             // a refinement that was due at the probe launch happens now, as it
             // would have without the probe
             Cc.halt = hold_refine[a] ? PH_HALT_REFINE : PH_RUN;

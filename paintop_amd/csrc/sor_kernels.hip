@@ -52,22 +52,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so neighbouring tiles (which share halo rows)
-// hit the same L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  const int per = (n + 7) >> 3;
-  const int xcd = b & 7, slot = b >> 3;
-  const int full = n - (per - 1) * 8;  // XCDs that get `per` tiles
-  int t;
-  if (xcd < full) t = xcd * per + slot;
-  else t = full * per + (xcd - full) * (per - 1) + slot;
-  return t;
-}
 
 // Stop rules for one finished sweep m (poisson.rs:202-229). Returns true when
 // the solve must halt at m.
-__device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
+__device__ bool finalize_sweep(PhCtl &C, const PhInst &I, uint32_t m, double res2) {
   const double rel = sqrt(res2) / C.r0;
   C.history[m - 1] = rel;
   C.sweeps_final = m;
@@ -77,7 +65,9 @@ __device__ bool finalize_sweep(PhCtl &C, uint32_t m, double res2) {
     C.halt_sweep = m;
     return true;
   }
-  if (rel < C.best * (1.0 - PH_STALL_IMPROVEMENT)) {
+  if (ph_in_ramp(I, m)) {
+    // stall guard on hold
+  } else if (rel < C.best * (1.0 - PH_STALL_IMPROVEMENT)) {
     C.best = rel;
     C.since = 0;
   } else {
@@ -114,10 +104,10 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *A) {
   const uint32_t in_buf = C.cur_buf;
   const uint64_t h0 = 2 * (uint64_t)S;  // first half-sweep of this launch
   bool stop = false;
-  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, S, C.prev_black + A[0]);
+  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, I, S, C.prev_black + A[0]);
   for (int j = 1; j < nsw && !stop; ++j) {
     const double wb = ph_omega_at(I, h0 + 2 * j - 1);  // black half-sweep of sweep S + j
-    stop = finalize_sweep(C, S + j, (1.0 - wb) * (1.0 - wb) * A[2 * j - 1] + A[2 * j]);
+    stop = finalize_sweep(C, I, S + j, (1.0 - wb) * (1.0 - wb) * A[2 * j - 1] + A[2 * j]);
   }
   const double wl = ph_omega_at(I, h0 + 2 * nsw - 1);
   C.prev_black = (1.0 - wl) * (1.0 - wl) * A[2 * nsw - 1];

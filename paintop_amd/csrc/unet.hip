@@ -545,6 +545,33 @@ extern "C" ph_status ph_unet_set_state(ph_unet *net, const float *params, const 
   });
 }
 
+extern "C" ph_status ph_unet_gather_device(const float *d_in_all, const float *d_target_all,
+                                           const float *d_weight_all, uint64_t n_samples,
+                                           const int32_t *index, const int32_t *mode,
+                                           int32_t batch, int32_t res, float *d_in,
+                                           float *d_target, float *d_weight, void *stream,
+                                           ph_error *err) {
+  return un_guarded(err, [&]() {
+    if (!d_in_all || !d_target_all || !d_weight_all || !index || !d_in || !d_target || !d_weight)
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null argument");
+    if (batch < 1 || batch > UN_GATHER_MAX || res < 1 || res > 4096)
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "gather: batch %d (at most %d), res %d", batch,
+                 UN_GATHER_MAX, res);
+    UnGather G{};
+    for (int j = 0; j < batch; ++j) {
+      if (index[j] < 0 || (uint64_t)index[j] >= n_samples)
+        throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "gather: index %d outside [0, %llu)",
+                   index[j], (unsigned long long)n_samples);
+      G.idx[j] = index[j];
+      G.mode[j] = mode ? (mode[j] & 7) : 0;
+    }
+    hipLaunchKernelGGL(k_gather_dihedral, dim3(res, batch), dim3(std::min(res, 256)), 0,
+                       (hipStream_t)stream, d_in_all, d_target_all, d_weight_all, G, res, d_in,
+                       d_target, d_weight);
+    HIP_OK(hipGetLastError());
+  });
+}
+
 extern "C" ph_status ph_warm_start_set_weights(const float *params, uint64_t n_params,
                                                ph_error *err) {
   std::lock_guard<std::mutex> lk(g_mu);

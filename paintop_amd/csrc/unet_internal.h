@@ -89,8 +89,16 @@ struct UnDwArgs {
   int dil;             // dilation of the 3x3 taps
 };
 
+constexpr int UN_GATHER_MAX = 256;
+struct UnGather {
+  int32_t idx[UN_GATHER_MAX];
+  int32_t mode[UN_GATHER_MAX];
+};
+
 template <int MB, int NB>
 __global__ void k_conv3(UnConvArgs A);
+__global__ void k_gather_dihedral(const float *xin, const float *yin, const float *win, UnGather G,
+                                  int res, float *xo, float *yo, float *wo);
 __global__ void k_conv3_dw(UnDwArgs A);
 __global__ void k_dw_reduce(const float *part, const float *part_db, int nchunk, int cout, int kp,
                             float *gw, float *gb);

@@ -528,6 +528,28 @@ __global__ void k_cast_w(const float *__restrict__ w, int cout, int cp, int kp,
   }
 }
 
+// Training batch gather: sample j of the batch is sample idx[j] of the
+// dataset under the dihedral map mode[j] (bit 2: transpose, bit 0: mirror x,
+// bit 1: mirror y) -- the Poisson problem is invariant under the 8 maps, and
+// every input and the target are scalar fields on the grid.
+__global__ void k_gather_dihedral(const float *__restrict__ xin, const float *__restrict__ yin,
+                                  const float *__restrict__ win, UnGather G, int res,
+                                  float *__restrict__ xo, float *__restrict__ yo,
+                                  float *__restrict__ wo) {
+  const int j = blockIdx.y, y = blockIdx.x;
+  const int src = G.idx[j], mode = G.mode[j];
+  for (int x = threadIdx.x; x < res; x += blockDim.x) {
+    int sy = (mode & 4) ? x : y, sx = (mode & 4) ? y : x;
+    if (mode & 1) sx = res - 1 - sx;
+    if (mode & 2) sy = res - 1 - sy;
+    const size_t q = ((size_t)src * res + sy) * res + sx;
+    const size_t p = ((size_t)j * res + y) * res + x;
+    *(float4 *)(xo + 4 * p) = *(const float4 *)(xin + 4 * q);
+    yo[p] = yin[q];
+    wo[p] = win[q];
+  }
+}
+
 // FP32 [n][res][res][4] network input -> bf16
 __global__ void k_to_bf16(const float *__restrict__ in, uint16_t *__restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

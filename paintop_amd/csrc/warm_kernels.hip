@@ -34,28 +34,42 @@ __device__ __forceinline__ bool warm_interior(const float *mask, int W, int H, i
 // order (lane-local rows, then the butterfly), so results are deterministic.
 // Ring cells are the non-interior cells with an interior 4-neighbour: the
 // Dirichlet data of the solve (poisson.rs:374-392 reads them as neighbours).
+// mode 0 (warm start): P = source, the rhs is lambda (P - anchor).
+// mode 1 (correction problem): P = 0 and the rhs is the 5-point Laplacian of
+// the source (poisson.rs:364-373): a blend with a zero anchor is the problem
+// Laplacian(u) = rhs, u = 0 on the ring, that a coarse correction solves.
 __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
                                                 const float *__restrict__ anc,
                                                 const float *__restrict__ mask,
                                                 const float *__restrict__ sol, int W, int H, int C,
                                                 int ch, double lambda, int x0, int y0, int f,
-                                                int res, float *__restrict__ in4,
+                                                int res, int mode, float *__restrict__ in4,
                                                 float *__restrict__ tgt, float *__restrict__ wt) {
   const int c = blockIdx.x;
   const int cx = c % res, cy = c / res;
   const int lane = threadIdx.x;
   double s_rhs = 0.0, s_ring = 0.0, n_int = 0.0, n_ring = 0.0, s_t = 0.0;
-  for (int dy = 0; dy < f; ++dy) {
+  // the block's f*f cells, row-major, strided over the lanes (every lane busy
+  // for any f)
+  for (int i = lane; i < f * f; i += 64) {
+    const int dy = i / f, dx = i - dy * f;
     const int iy = y0 + cy * f + dy;
-    if (iy < 0 || iy >= H) continue;
-    for (int dx = lane; dx < f; dx += 64) {
-      const int ix = x0 + cx * f + dx;
-      if (ix < 0 || ix >= W) continue;
+    const int ix = x0 + cx * f + dx;
+    if (iy < 0 || iy >= H || ix < 0 || ix >= W) continue;
+    {
       const size_t q = (size_t)iy * W + ix;
-      const double P = (double)src[q * C + ch], A = (double)anc[q * C + ch];
+      const double S = (double)src[q * C + ch], A = (double)anc[q * C + ch];
+      const double P = mode ? 0.0 : S;
       if (warm_interior(mask, W, H, ix, iy)) {
         n_int += 1.0;
-        s_rhs += lambda * (P - A);
+        if (mode) {
+          // interior cells are never on the image border: four in-image neighbours
+          const double l = src[(q - 1) * C + ch], r = src[(q + 1) * C + ch];
+          const double u = src[(q - W) * C + ch], d = src[(q + W) * C + ch];
+          s_rhs += fma(4.0, -S, ((l + r) + u) + d);
+        } else {
+          s_rhs += lambda * (P - A);
+        }
         if (sol) s_t += (double)sol[q * C + ch] - P;
       } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
                  warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
@@ -82,17 +96,73 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
   }
 }
 
-// x = P + bilinear(h) at the interior cells of an instance's plane (cell
-// centres: image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5,
-// clamped to the grid).
-__global__ void k_warm_apply(PhInst I, const float *__restrict__ src, const float *__restrict__ hc,
-                             int res, int x0, int y0, int f) {
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y;
-  if (px >= I.pw) return;
-  const size_t p = (size_t)py * I.pitch + px;
-  if (!I.cls[p]) return;
-  const int ix = I.ox + px, iy = I.oy + py;
+// Net input of a coarse correction, from an instance's plane: per block the
+// sum of g (the correction equation's rhs, Laplacian(e) = g) over interior
+// cells, zero boundary values, interior fraction and ring density, exactly as
+// k_coarsen lays them out. One wave per coarse cell.
+__global__ __launch_bounds__(64) void k_coarsen_plane(PhInst I, int x0, int y0, int f, int res,
+                                                      float *__restrict__ in4) {
+  const int c = blockIdx.x;
+  const int cx = c % res, cy = c / res;
+  const int lane = threadIdx.x;
+  double s_rhs = 0.0, n_int = 0.0, n_ring = 0.0;
+  for (int i = lane; i < f * f; i += 64) {
+    const int dy = i / f, dx = i - dy * f;
+    const int py = y0 + cy * f + dy - I.oy;
+    const int px = x0 + cx * f + dx - I.ox;
+    if (py < 1 || py >= I.ph - 1 || px < 1 || px >= I.pw - 1) continue;
+    {
+      const size_t p = (size_t)py * I.pitch + px;
+      if (I.cls[p]) {
+        n_int += 1.0;
+        s_rhs += (double)I.g[p];
+      } else if (I.cls[p - 1] || I.cls[p + 1] || I.cls[p - I.pitch] || I.cls[p + I.pitch]) {
+        n_ring += 1.0;
+      }
+    }
+  }
+  s_rhs = wave_sum_d(s_rhs);
+  n_int = wave_sum_d(n_int);
+  n_ring = wave_sum_d(n_ring);
+  if (lane == 0) {
+    in4[(size_t)c * 4 + 0] = (float)s_rhs;
+    in4[(size_t)c * 4 + 1] = 0.0f;
+    in4[(size_t)c * 4 + 2] = (float)(n_int / ((double)f * f));
+    in4[(size_t)c * 4 + 3] = (float)(n_ring / f);
+  }
+}
+
+// The problems are linear: each net input is scaled so that its rhs and
+// boundary channels have unit RMS over the grid (scale[b] = that RMS, 1 when
+// both are zero), and the prediction is scaled back. One block per input,
+// fixed-order reduction.
+__global__ __launch_bounds__(256) void k_in_norm(float *__restrict__ in4, int res,
+                                                 float *__restrict__ scale) {
+  const int b = blockIdx.x;
+  float *v = in4 + (size_t)b * res * res * 4;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < res * res; i += 256) {
+    const double a = v[4 * i], c = v[4 * i + 1];
+    s += a * a + c * c;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  const double rms = sqrt(red[0] / ((double)res * res));
+  const float sc = rms > 0.0 ? (float)rms : 1.0f;
+  for (int i = threadIdx.x; i < res * res; i += 256) {
+    v[4 * i] /= sc;
+    v[4 * i + 1] /= sc;
+  }
+  if (threadIdx.x == 0) scale[b] = sc;
+}
+
+__device__ __forceinline__ double warm_bilinear(const float *hc, int res, int x0, int y0, int f,
+                                                int ix, int iy) {
   const double inv = 1.0 / f;
   double u = ((ix - x0) + 0.5) * inv - 0.5, v = ((iy - y0) + 0.5) * inv - 0.5;
   u = fmin(fmax(u, 0.0), (double)(res - 1));
@@ -102,7 +172,81 @@ __global__ void k_warm_apply(PhInst I, const float *__restrict__ src, const floa
   const double a = u - i0, b = v - j0;
   const double h00 = hc[(size_t)j0 * res + i0], h01 = hc[(size_t)j0 * res + i1];
   const double h10 = hc[(size_t)j1 * res + i0], h11 = hc[(size_t)j1 * res + i1];
-  const double h = (1.0 - b) * ((1.0 - a) * h00 + a * h01) + b * ((1.0 - a) * h10 + a * h11);
+  return (1.0 - b) * ((1.0 - a) * h00 + a * h01) + b * ((1.0 - a) * h10 + a * h11);
+}
+
+// Step length of a coarse correction, on the coarse grid. For block means E of
+// a smooth error e, the block sums of Laplacian(e) are the net fluxes through
+// the block faces, sum over the 4 neighbours of (E_n - E_b) (A_c E): the
+// correction problem's coarse form is A_c E = G with G = the block sums of g
+// (channel 0 of the net input). The Galerkin step alpha = <G, H> / <A_c H, H>
+// for the prediction H is 1 for a prediction that solves it and scales a
+// wrong-sized one back; it is taken over bulk blocks only (every cell
+// interior, 4 neighbour blocks with interior cells), where A_c holds, and
+// clamped to [0, max_step] (0: no usable correction). Input and prediction
+// carry the same normalisation, so alpha is scale-free. One block of 1024,
+// fixed-order reduction.
+__global__ __launch_bounds__(1024) void k_corr_alpha(const float *__restrict__ in4,
+                                                     const float *__restrict__ hc, int res,
+                                                     double max_step, double *__restrict__ alpha) {
+  double s1 = 0.0, s2 = 0.0;
+  for (int c = threadIdx.x; c < res * res; c += 1024) {
+    const int cx = c % res, cy = c / res;
+    if (cx < 1 || cy < 1 || cx >= res - 1 || cy >= res - 1) continue;
+    if (!(in4[(size_t)c * 4 + 2] >= 1.0f)) continue;
+    if (!(in4[(size_t)(c - 1) * 4 + 2] > 0.0f && in4[(size_t)(c + 1) * 4 + 2] > 0.0f &&
+          in4[(size_t)(c - res) * 4 + 2] > 0.0f && in4[(size_t)(c + res) * 4 + 2] > 0.0f))
+      continue;
+    const double h = hc[c];
+    const double ah = (((double)hc[c - 1] + hc[c + 1]) + hc[c - res]) + hc[c + res] - 4.0 * h;
+    s1 = fma((double)in4[(size_t)c * 4], h, s1);
+    s2 = fma(ah, h, s2);
+  }
+  __shared__ double red[2][1024];
+  red[0][threadIdx.x] = s1;
+  red[1][threadIdx.x] = s2;
+  __syncthreads();
+  for (int k = 512; k >= 1; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + k];
+      red[1][threadIdx.x] += red[1][threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = red[0][0] / red[1][0];
+    alpha[0] = (red[1][0] < 0.0 && isfinite(a)) ? fmin(fmax(a, 0.0), max_step) : 0.0;
+  }
+}
+
+// x += sign * alpha * scale * bilinear(h) at the interior cells of an
+// instance's plane (sign -1 takes a correction back).
+__global__ void k_correct_apply(PhInst I, const float *__restrict__ hc,
+                                const float *__restrict__ scale,
+                                const double *__restrict__ alpha, int res, int x0, int y0, int f,
+                                double sign) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pw) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  if (!I.cls[p]) return;
+  const double a = sign * alpha[0];
+  if (a == 0.0) return;
+  I.x[p] += a * (double)scale[0] * warm_bilinear(hc, res, x0, y0, f, I.ox + px, I.oy + py);
+}
+
+// x = P + scale * bilinear(h) at the interior cells of an instance's plane (cell
+// centres: image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5,
+// clamped to the grid).
+__global__ void k_warm_apply(PhInst I, const float *__restrict__ src, const float *__restrict__ hc,
+                             const float *__restrict__ scale, int res, int x0, int y0, int f) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  const int py = blockIdx.y;
+  if (px >= I.pw) return;
+  const size_t p = (size_t)py * I.pitch + px;
+  if (!I.cls[p]) return;
+  const int ix = I.ox + px, iy = I.oy + py;
+  const double h = (double)scale[0] * warm_bilinear(hc, res, x0, y0, f, ix, iy);
   const size_t q = (size_t)iy * I.img_w + ix;
   I.x[p] = (double)src[q * I.channels + I.channel] + h;
 }
@@ -126,24 +270,25 @@ extern "C" ph_status ph_warm_coarsen_device(const float *d_source, const float *
                                             const float *d_mask, const float *d_solution,
                                             uint32_t width, uint32_t height, uint32_t channels,
                                             uint32_t channel, double lambda, int32_t x0,
-                                            int32_t y0, int32_t factor, int32_t res, float *d_in,
-                                            float *d_target, float *d_weight, void *stream,
-                                            ph_error *err) {
+                                            int32_t y0, int32_t factor, int32_t res, int32_t mode,
+                                            float *d_in, float *d_target, float *d_weight,
+                                            void *stream, ph_error *err) {
   std::lock_guard<std::mutex> lk(g_mu);
   clear_error(err);
   try {
     ensure_device();
     if (!d_source || !d_anchor || !d_mask || !d_in)
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
-    if (channels == 0 || channel >= channels || factor < 1 || res < 1 || res > 4096)
+    if (channels == 0 || channel >= channels || factor < 1 || res < 1 || res > 4096 ||
+        (mode != 0 && mode != 1))
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT",
-                 "coarsen: channel %u of %u, factor %d, resolution %d", channel, channels, factor,
-                 res);
+                 "coarsen: channel %u of %u, factor %d, resolution %d, mode %d", channel, channels,
+                 factor, res, (int)mode);
     if (d_solution && (!d_target || !d_weight))
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "coarsen: target/weight buffers required");
     hipLaunchKernelGGL(k_coarsen, dim3(res * res), dim3(64), 0, (hipStream_t)stream, d_source,
                        d_anchor, d_mask, d_solution, (int)width, (int)height, (int)channels,
-                       (int)channel, lambda, x0, y0, factor, res, d_in,
+                       (int)channel, lambda, x0, y0, factor, res, (int)mode, d_in,
                        d_solution ? d_target : nullptr, d_solution ? d_weight : nullptr);
     HIP_OK(hipGetLastError());
     return PH_OK;

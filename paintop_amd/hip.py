@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "ph_warm_start_set_weights",
     "ph_warm_coarsen_device",
     "ph_poisson_solve_start_device",
+    "ph_unet_gather_device",
 )
 
 
@@ -89,6 +90,8 @@ class PhReport(ctypes.Structure):
         ("warm_start_used", ctypes.c_uint32),
         ("warm_start_rel", ctypes.c_double),
         ("sweep_ms", ctypes.c_double),
+        ("coarse_corrections", ctypes.c_uint32),
+        ("reserved_", ctypes.c_uint32),
     ]
 
 
@@ -212,12 +215,16 @@ def lib():
     L.ph_unet_get_state.restype = ctypes.c_int
     L.ph_unet_set_state.argtypes = [vp, fp, fp, fp, ctypes.c_int32, pe]
     L.ph_unet_set_state.restype = ctypes.c_int
+    ip = ctypes.POINTER(ctypes.c_int32)
+    L.ph_unet_gather_device.argtypes = [vp, vp, vp, ctypes.c_uint64, ip, ip, ctypes.c_int32,
+                                        ctypes.c_int32, vp, vp, vp, vp, pe]
+    L.ph_unet_gather_device.restype = ctypes.c_int
     L.ph_warm_start_set_weights.argtypes = [fp, ctypes.c_uint64, pe]
     L.ph_warm_start_set_weights.restype = ctypes.c_int
     L.ph_warm_coarsen_device.argtypes = [vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                         ctypes.c_int32, vp, vp, vp, vp, pe]
+                                         ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp, pe]
     L.ph_warm_coarsen_device.restype = ctypes.c_int
     _lib = L
     return L
@@ -249,6 +256,7 @@ class SolveResult:
     warm_start_used: int = 0  # channels that kept the learned start
     warm_start_rel: float = 0.0
     sweep_ms: float = 0.0
+    coarse_corrections: int = 0
 
 
 def make_desc(width, height, channels, max_iterations, tolerance, lam=0.0, op=PH_OP_POISSON_BLEND,
@@ -278,6 +286,7 @@ def _result(rep: PhReport, hist: np.ndarray) -> SolveResult:
         warm_start_used=int(rep.warm_start_used),
         warm_start_rel=float(rep.warm_start_rel),
         sweep_ms=float(rep.sweep_ms),
+        coarse_corrections=int(rep.coarse_corrections),
     )
 
 
@@ -495,10 +504,28 @@ def warm_start_set_weights(params: np.ndarray | None) -> None:
 
 def warm_coarsen_device(d_source, d_anchor, d_mask, d_solution, width, height, channels, channel,
                         lam, x0, y0, factor, res, d_in, d_target=None, d_weight=None,
-                        stream: int = 0):
+                        stream: int = 0, mode: int = 0):
+    """Coarse net input (and target) of one channel; mode 0 = warm start,
+    mode 1 = coarse-correction problem (P = 0, rhs = Laplacian of the source)."""
     err = PhError()
     st = lib().ph_warm_coarsen_device(d_source, d_anchor, d_mask, d_solution or None, width,
-                                      height, channels, channel, lam, x0, y0, factor, res, d_in,
-                                      d_target or None, d_weight or None, stream or None,
+                                      height, channels, channel, lam, x0, y0, factor, res, mode,
+                                      d_in, d_target or None, d_weight or None, stream or None,
                                       ctypes.byref(err))
+    _check(st, err)
+
+
+def unet_gather_device(d_in_all: int, d_target_all: int, d_weight_all: int, n_samples: int,
+                       index, mode, res: int, d_in: int, d_target: int, d_weight: int,
+                       stream: int = 0):
+    """Gather a training batch (example index[j] under dihedral map mode[j])."""
+    idx = np.ascontiguousarray(index, dtype=np.int32)
+    md = None if mode is None else np.ascontiguousarray(mode, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    err = PhError()
+    st = lib().ph_unet_gather_device(d_in_all, d_target_all, d_weight_all, n_samples,
+                                     idx.ctypes.data_as(ip),
+                                     None if md is None else md.ctypes.data_as(ip), idx.size,
+                                     res, d_in, d_target, d_weight, stream or None,
+                                     ctypes.byref(err))
     _check(st, err)

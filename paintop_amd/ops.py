@@ -159,12 +159,19 @@ class _HipPoissonOp:
     kind = ""
     ph_op = hip.PH_OP_POISSON_BLEND
 
-    def __init__(self, omega: float = 0.0, half_sweeps: int = 0):
+    def __init__(self, omega: float = 0.0, half_sweeps: int = 0, warm_start: bool = False):
         # SOR factor and half-sweeps per launch: tuned when 0. These are backend
         # tuning knobs, not plan params (the reference fixes omega = 1.8 in its
-        # own backend, poisson_blend.rs:78).
+        # own backend, poisson_blend.rs:78). warm_start: the learned warm start
+        # (the shipped network weights, raced against the anchor start; the
+        # report keeps the reference's fields and denominator).
         self.omega = omega
         self.half_sweeps = half_sweeps
+        self.warm_start = warm_start
+        if warm_start:
+            from . import warmstart
+
+            warmstart.enable()
         self.last_result: hip.SolveResult | None = None
 
     @classmethod
@@ -197,7 +204,8 @@ class _HipPoissonOp:
         channels = out_desc.channels
         ext = out_desc.extent
         desc = hip.make_desc(ext.width, ext.height, channels, max_it, tol, lam, op=self.ph_op,
-                             omega=self.omega, half_sweeps=self.half_sweeps)
+                             omega=self.omega, half_sweeps=self.half_sweeps,
+                             flags=hip.PH_FLAG_WARM_START_NET if self.warm_start else 0)
         try:
             out, res = hip.poisson_solve(desc, a.samples, b.samples, m.samples)
         except hip.HipCallError as e:

@@ -143,10 +143,15 @@ def sample_spec(i: int) -> dict:
                 seed_s=2 * (SEED0 + i) + 1_000_001, seed_t=2 * (SEED0 + i) + 1_000_002)
 
 
-def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, log=print):
+def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, log=print,
+                   mode: int = 0):
     """Solve instances first .. first+n-1 with the plain solver and return the
     coarse (inputs [n,R,R,4], target [n,R,R], weight [n,R,R]) as host arrays,
-    plus the solver's sweep counts."""
+    plus the solver's sweep counts. mode 0: warm-start pairs (fbm source and
+    target). mode 1: coarse-correction pairs -- the blend of an fbm source into
+    a zero target solves Laplacian(u) = Laplacian(source), u = 0 on the ring,
+    which is the correction problem Laplacian(e) = g, e = 0 on the ring, with
+    g = the source's Laplacian."""
     from .device import DeviceArray, synchronize
 
     R2 = RES * RES
@@ -165,8 +170,11 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
             src, tgt, msk, out = (DeviceArray((s, s), np.float32) for _ in range(4))
             hip.fbm_device(src.ptr, s, s, sp["freq"], sp["seed_s"], 6, 2.0, 0.5, scale=0.5,
                            bias=0.5)
-            hip.fbm_device(tgt.ptr, s, s, sp["freq"], sp["seed_t"], 6, 2.0, 0.5, scale=0.5,
-                           bias=0.5)
+            if mode == 0:
+                hip.fbm_device(tgt.ptr, s, s, sp["freq"], sp["seed_t"], 6, 2.0, 0.5, scale=0.5,
+                               bias=0.5)
+            else:
+                tgt.zero()
             kind, geo = sp["shape"]
             if kind == "polygon":
                 hip.polygon_mask_device(msk.ptr, s, s, geo)
@@ -182,7 +190,7 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
             s = sp["size"]
             hip.warm_coarsen_device(src.ptr, tgt.ptr, msk.ptr, out.ptr, s, s, 1, 0, 0.0,
                                     sp["x0"], sp["y0"], sp["f"], RES, din.ptr + j * R2 * 16,
-                                    dt.ptr + j * R2 * 4, dw.ptr + j * R2 * 4)
+                                    dt.ptr + j * R2 * 4, dw.ptr + j * R2 * 4, mode=mode)
         synchronize()
         k = slice(c0, c0 + len(specs))
         X[k], Y[k], Wt[k] = din.numpy(), dt.numpy(), dw.numpy()
@@ -205,6 +213,20 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
     return X, Y, Wt, sweeps
 
 
+def normalise(X: np.ndarray, Y: np.ndarray) -> np.ndarray:
+    """Scale each example as the solver scales its net inputs (the problems are
+    linear): channels 0 and 1 and the target are divided by the RMS of
+    channels 0 and 1 over the grid (1 when both are zero). In place; returns
+    the scales."""
+    ms = np.einsum("nijc,nijc->n", X[..., :2].astype(np.float64),
+                   X[..., :2].astype(np.float64)) / (RES * RES)
+    sc = np.sqrt(ms).astype(np.float32)
+    sc[sc == 0] = 1.0
+    X[..., :2] /= sc[:, None, None, None]
+    Y /= sc[:, None, None]
+    return sc
+
+
 def loss_weight(X: np.ndarray, Wt: np.ndarray) -> np.ndarray:
     """Per-cell loss weight: the coarsening's weight (1 on blocks with interior
     cells, 0.5 on blocks holding only Dirichlet-ring cells, 0 elsewhere)."""
@@ -215,8 +237,10 @@ def loss_weight(X: np.ndarray, Wt: np.ndarray) -> np.ndarray:
 # training
 
 def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, seed=1,
-          val_frac=0.05, ckpt=None, log=print, time_budget=None, pyramid=1.0):
-    """AdamW on the GPU with a cosine schedule. Returns (params, history)."""
+          val_frac=0.05, ckpt=None, log=print, time_budget=None, pyramid=1.0, augment=True):
+    """AdamW on the GPU with a cosine schedule; each training example is drawn
+    under a random one of the 8 dihedral maps of the grid (augment). Returns
+    (params, history)."""
     from .device import DeviceArray
 
     n = X.shape[0]
@@ -229,15 +253,9 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
     bX = DeviceArray((batch, RES, RES, 4), np.float32)
     bY = DeviceArray((batch, RES, RES), np.float32)
     bW = DeviceArray((batch, RES, RES), np.float32)
-    from .device import runtime, D2D
-
-    rt = runtime()
-
-    def gather(idx):
-        for j, i in enumerate(idx):
-            rt.hipMemcpy(bX.ptr + j * R2 * 16, dX.ptr + int(i) * R2 * 16, R2 * 16, D2D)
-            rt.hipMemcpy(bY.ptr + j * R2 * 4, dY.ptr + int(i) * R2 * 4, R2 * 4, D2D)
-            rt.hipMemcpy(bW.ptr + j * R2 * 4, dW.ptr + int(i) * R2 * 4, R2 * 4, D2D)
+    def gather(idx, modes=None):
+        hip.unet_gather_device(dX.ptr, dY.ptr, dW.ptr, n, idx, modes, RES, bX.ptr, bY.ptr,
+                               bW.ptr)
 
     net = hip.UNet(batch, RES, seed=seed)
     step0 = 0
@@ -255,14 +273,31 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
     perm = rng.permutation(ntr)
     last = t0
 
-    def val_loss():
+    def val_loss(idx=None):
+        idx = np.arange(ntr, n) if idx is None else idx
         tot, cnt = 0.0, 0
-        for v0 in range(ntr, n - batch + 1, batch):
-            gather(range(v0, v0 + batch))
+        for v0 in range(0, len(idx) - batch + 1, batch):
+            gather(idx[v0:v0 + batch])
             tot += net.train_step(bX.ptr, bY.ptr, bW.ptr, batch, 0.0, update=False,
                                   pyramid=pyramid)
             cnt += 1
         return tot / max(cnt, 1)
+
+    # validation split by kind (a correction problem has no boundary values),
+    # each against the loss of a zero prediction
+    vidx = np.arange(ntr, n)
+    is_corr = ~np.any(X[ntr:, :, :, 1] != 0, axis=(1, 2))
+    kinds = [(nm, vidx[sel]) for nm, sel in (("warm", ~is_corr), ("corr", is_corr))
+             if sel.sum() >= batch]
+    zero = {nm: float(np.mean(np.sum(Wl[ix] * Y[ix] ** 2, axis=(1, 2)) / (RES * RES)))
+            for nm, ix in kinds}
+
+    def val_report():
+        parts = []
+        for nm, ix in kinds:
+            v = val_loss(ix)
+            parts.append(f"{nm} {v:.3e} ({v / max(zero[nm], 1e-30):.2e} of zero)")
+        return "  ".join(parts)
 
     step = step0
     while step < total:
@@ -270,7 +305,7 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
         if k == 0 and step != step0:
             perm = rng.permutation(ntr)
         idx = perm[k * batch:(k + 1) * batch]
-        gather(idx)
+        gather(idx, rng.integers(0, 8, size=batch) if augment else None)
         a = min(1.0, (step + 1) / warmup)
         cosf = 0.5 * (1 + math.cos(math.pi * step / max(total, 1)))
         cur = lr_min + (lr - lr_min) * cosf
@@ -284,6 +319,8 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
             hist.append((step, loss, vl))
             log(f"[warmstart] step {step}/{total} lr {cur * a:.2e} loss {loss:.3e} val {vl:.3e} "
                 f"{now - t0:.0f}s")
+            if len(kinds) > 1 and len(hist) % 4 == 1:
+                log(f"[warmstart]   val by kind: {val_report()}")
             if ckpt:
                 p, _, m, v, st = net.state()
                 np.savez(ckpt, params=p, adam_m=m, adam_v=v, step=np.int64(st))
@@ -292,10 +329,12 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
             break
     params = net.state()[0]
     final_val = val_loss()
+    by_kind = val_report() if kinds else ""
+    log(f"[warmstart] final val {final_val:.3e}  {by_kind}")
     net.close()
     for a in (dX, dY, dW, bX, bY, bW):
         a.free()
-    return params, dict(history=hist, val_loss=final_val, steps=step)
+    return params, dict(history=hist, val_loss=final_val, val_by_kind=by_kind, steps=step)
 
 
 def main(argv=None):
@@ -310,21 +349,37 @@ def main(argv=None):
     t.add_argument("--ckpt", default=None, help="checkpoint file (resume if present)")
     t.add_argument("--time-budget", type=float, default=None, help="seconds of training")
     t.add_argument("--pyramid", type=float, default=1.0, help="multi-scale loss weight")
+    t.add_argument("--no-augment", action="store_true", help="no dihedral augmentation")
+    t.add_argument("--correction-frac", type=float, default=0.5,
+                   help="fraction of the samples that are coarse-correction problems")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
         t0 = time.time()
-        X, Y, Wt, sweeps = generate_pairs(a.samples, log=log)
+        n1 = int(round(a.samples * a.correction_frac))
+        n0 = a.samples - n1
+        parts = [generate_pairs(n0, log=log, mode=0)] if n0 else []
+        if n1:
+            # correction problems on instances of their own (other masks and textures)
+            parts.append(generate_pairs(n1, first=n0, log=log, mode=1))
+        X, Y, Wt, sweeps = (np.concatenate(z) for z in zip(*parts))
+        # validation examples are the last ones: interleave the two kinds
+        order = np.random.default_rng(7).permutation(len(X))
+        X, Y, Wt, sweeps = X[order], Y[order], Wt[order], sweeps[order]
+        normalise(X, Y)
         gen_s = time.time() - t0
         Wl = loss_weight(X, Wt)
         params, info = train(X, Y, Wl, epochs=a.epochs, batch=a.batch, lr=a.lr, ckpt=a.ckpt,
-                             log=log, time_budget=a.time_budget, pyramid=a.pyramid)
+                             log=log, time_budget=a.time_budget, pyramid=a.pyramid,
+                             augment=not a.no_augment)
         os.makedirs(a.out, exist_ok=True)
         np.save(os.path.join(a.out, "warm_start_unet.npy"), params.astype(np.float32))
-        meta = dict(samples=a.samples, epochs=a.epochs, batch=a.batch, lr=a.lr,
-                    pyramid=a.pyramid,
+        meta = dict(samples=a.samples, correction_samples=n1, epochs=a.epochs, batch=a.batch,
+                    lr=a.lr, input_normalisation="rms of channels 0-1",
+                    pyramid=a.pyramid, augment=not a.no_augment,
                     dropped=int((Wt.reshape(len(Wt), -1).max(axis=1) == 0).sum()),
                     steps=info["steps"], val_loss=info["val_loss"],
+                    val_by_kind=info["val_by_kind"],
                     history=info["history"][-20:], pair_generation_s=round(gen_s, 1),
                     train_s=round(time.time() - t0 - gen_s, 1),
                     solver_sweeps_median=int(np.median(sweeps)),

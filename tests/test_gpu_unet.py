@@ -117,7 +117,8 @@ def test_training_fits_a_fixed_batch(gpu):
     net.close()
 
 
-def test_coarsen_matches_numpy(gpu):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_coarsen_matches_numpy(gpu, mode):
     from paintop_amd.device import DeviceArray
     from tests.helpers import classify
 
@@ -133,20 +134,29 @@ def test_coarsen_matches_numpy(gpu):
     dtg = DeviceArray((R, R), np.float32)
     dwt = DeviceArray((R, R), np.float32)
     gpu.warm_coarsen_device(ds.ptr, dt.ptr, dm.ptr, du.ptr, W, H, C, ch, lam, x0, y0, f, R,
-                            din.ptr, dtg.ptr, dwt.ptr)
+                            din.ptr, dtg.ptr, dwt.ptr, mode=mode)
     gi, gt, gw = din.numpy(), dtg.numpy(), dwt.numpy()
     inside = classify(m).astype(bool)
     pad = np.pad(inside, 1)
     near = pad[:-2, 1:-1] | pad[2:, 1:-1] | pad[1:-1, :-2] | pad[1:-1, 2:]
     ring = near & ~inside
-    P, A, S = (a[..., ch].astype(np.float64) for a in (s, t, u))
+    G, A, S = (a[..., ch].astype(np.float64) for a in (s, t, u))
+    if mode == 0:
+        P, rhs_cell = G, lam * (G - A)
+    else:
+        # correction problem: P = 0, rhs = the 5-point Laplacian of the source
+        # (interior cells are never on the image border)
+        P = np.zeros_like(G)
+        rhs_cell = np.zeros_like(G)
+        rhs_cell[1:-1, 1:-1] = (G[1:-1, :-2] + G[1:-1, 2:] + G[:-2, 1:-1] + G[2:, 1:-1]
+                                - 4.0 * G[1:-1, 1:-1])
     for cy in range(R):
         for cx in range(R):
             ys = slice(max(0, y0 + cy * f), max(0, min(H, y0 + cy * f + f)))
             xs = slice(max(0, x0 + cx * f), max(0, min(W, x0 + cx * f + f)))
             ins, rg = inside[ys, xs], ring[ys, xs]
             ni, nr = ins.sum(), rg.sum()
-            rhs = np.sum(np.where(ins, lam * (P[ys, xs] - A[ys, xs]), 0.0))
+            rhs = np.sum(np.where(ins, rhs_cell[ys, xs], 0.0))
             bnd = np.sum(np.where(rg, A[ys, xs] - P[ys, xs], 0.0)) / nr if nr else 0.0
             tg = np.sum(np.where(ins, S[ys, xs] - P[ys, xs], 0.0) +
                         np.where(rg, A[ys, xs] - P[ys, xs], 0.0))
@@ -281,3 +291,35 @@ def test_start_field_race_keeps_the_better_start(gpu):
     assert bad.warm_start_used == 0
     assert o_s.numpy().tobytes() == o_p.numpy().tobytes()
     assert bad.residual_history == rp.residual_history and bad.iterations == rp.iterations
+
+
+def test_gather_dihedral_matches_numpy(gpu):
+    from paintop_amd.device import DeviceArray
+
+    rng = np.random.default_rng(21)
+    n, R, B = 11, 16, 8
+    X = rng.random((n, R, R, 4)).astype(np.float32)
+    Y = rng.random((n, R, R)).astype(np.float32)
+    W = rng.random((n, R, R)).astype(np.float32)
+    dX, dY, dW = _dev(X), _dev(Y), _dev(W)
+    bX, bY, bW = (DeviceArray(s, np.float32) for s in ((B, R, R, 4), (B, R, R), (B, R, R)))
+    idx = rng.integers(0, n, B)
+    modes = np.arange(B)
+    gpu.unet_gather_device(dX.ptr, dY.ptr, dW.ptr, n, idx, modes, R, bX.ptr, bY.ptr, bW.ptr)
+
+    def mapped(a, m):  # out[y, x] = a[sy, sx]
+        yy, xx = np.mgrid[0:R, 0:R]
+        sy, sx = (xx, yy) if m & 4 else (yy, xx)
+        sx = R - 1 - sx if m & 1 else sx
+        sy = R - 1 - sy if m & 2 else sy
+        return a[sy, sx]
+
+    gx, gy, gw = bX.numpy(), bY.numpy(), bW.numpy()
+    for j in range(B):
+        assert np.array_equal(gy[j], mapped(Y[idx[j]], modes[j]))
+        assert np.array_equal(gw[j], mapped(W[idx[j]], modes[j]))
+        assert np.array_equal(gx[j], mapped(X[idx[j]], modes[j]))
+    # the 8 maps are distinct
+    assert len({mapped(np.arange(R * R).reshape(R, R), m).tobytes() for m in range(8)}) == 8
+    with pytest.raises(hip.HipCallError):
+        gpu.unet_gather_device(dX.ptr, dY.ptr, dW.ptr, n, [n], None, R, bX.ptr, bY.ptr, bW.ptr)

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(hip.EXPORTED_SYMBOLS)
     for n in names:
         assert hasattr(L, n), n
-    assert hip.abi_version() == 1
+    assert hip.abi_version() == 2
 
 
 def test_probe_forced_unavailable_is_policy_error():
