@@ -4,21 +4,26 @@ solver's ``PH_FLAG_WARM_START_NET`` path runs with.
 
 The warm start has no counterpart in the reference (paintop's solver starts
 from the anchor, crates/paintop-cpu/src/poisson.rs:169-170); the solve that
-follows it is the unchanged red-black SOR with the reference's relative
-residual ||r|| / ||r(anchor start)|| (poisson.rs:172) and stop rules.
+follows it is red-black SOR with the reference's relative residual
+||r|| / ||r(anchor start)|| (poisson.rs:172) and stop rules.
 
 What the network predicts: with P = the source image, the blend's solution is
 u = P + h where h is harmonic inside the mask with boundary values
 anchor - P on the Dirichlet ring. The network maps the f x f blocks of
-(rhs of the correction problem, boundary values on the Dirichlet ring,
-interior fraction, ring density) on a 128 x 128 grid around the mask to the
-block means of h over interior and ring cells; the solver starts from
-x = P + bilinear(h).
+(rhs, boundary values on the Dirichlet ring, interior fraction, ring density)
+on a 128 x 128 grid around the mask to the block means of the solution over
+interior and ring cells; the solver starts from x = P + bilinear(h). The same
+network then serves as the coarse level of the solve: after each cycle of
+sweeps it solves the error equation Laplacian(e) = g (rhs = block sums of the
+residual, no boundary values) and the solver adds a Galerkin-scaled
+bilinear(e). Training mixes both kinds of problem (``--correction-frac``);
+every example is normalised by the RMS of its rhs and boundary channels, as
+the solver normalises its inputs.
 
 Usage::
 
-    python -m paintop_amd.warmstart train --samples 20000 --epochs 24 \\
-        --out paintop_amd/weights
+    python -m paintop_amd.warmstart train --samples 60000 --epochs 30 \
+        --correction-frac 0.5 --out paintop_amd/weights
     from paintop_amd import warmstart; warmstart.enable()   # load the shipped weights
 """
 from __future__ import annotations
