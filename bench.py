@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-tight", action="store_true")
     p.add_argument("--plain-only", action="store_true",
                    help="time only the plain start (no learned warm start)")
+    p.add_argument("--learned", action="store_true",
+                   help="batch workload: run the learned warm start and corrections")
     p.add_argument("--workload", choices=["seamless", "screened", "batch"], default="seamless",
                    help="seamless: 4096^2 RGBA blend to 1e-6 (the headline); screened: "
                         "8192^2 Gray screened Poisson, lambda 0.01, to 1e-7; batch: 512 "
@@ -122,9 +124,14 @@ def bench_batch(args, info):
     cases = batch_case(512, seed0=500)
     outs = [DeviceArray((h, w), np.float32) for (w, h, *_r) in cases]
     items = [(w, h, 1, s.ptr, t.ptr, m.ptr, o.ptr) for (w, h, s, t, m), o in zip(cases, outs)]
+    flags = hip.PH_FLAG_TIMING
+    if args.learned:
+        from paintop_amd import warmstart
+        warmstart.enable()
+        flags |= hip.PH_FLAG_WARM_START_NET
     desc = hip.make_desc(0, 0, 1, args.max_iterations, args.tol, 0.0, hip.PH_OP_POISSON_BLEND,
                          omega=args.omega, half_sweeps=args.half_sweeps,
-                         flags=hip.PH_FLAG_TIMING, tile_rows=args.tile_rows)
+                         flags=flags, tile_rows=args.tile_rows)
     for _ in range(args.warmup):
         hip.poisson_solve_batch_device(desc, items, want_history=False)
     times, results = [], []
@@ -152,7 +159,11 @@ def bench_batch(args, info):
                                f"batched call", "megapixels": round(px / 1e6, 2),
                    "sweeps_min": int(its.min()), "sweeps_median": int(np.median(its)),
                    "sweeps_max": int(its.max()), "launches": res[0].launches,
-                   "all_converged": all(r.converged for r in res), "device": info["name"]},
+                   "all_converged": all(r.converged for r in res),
+                   "form": "learned warm start" if args.learned else "plain",
+                   "warm_start_kept": sum(r.warm_start_used for r in res),
+                   "coarse_corrections": sum(r.coarse_corrections for r in res),
+                   "device": info["name"]},
         "roofline": {"kernel": f"k_sor<{res[0].half_sweeps}>", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None},

@@ -335,6 +335,17 @@ static const double WARM_MAX_START_RATIO = 32.0;
 // ramp restarts. The cycle is a multiple of the ramp (PAINTOP_HIP_CYCLE_FACTOR,
 // default 2), on a launch boundary; PAINTOP_HIP_CORRECTIONS caps the count per
 // channel (0 = none).
+// Corrections pay only where sweeps are dear: each round costs a network
+// forward, a few plane passes and a host round trip, ~0.5-1 ms, against
+// sweeps of ~3 us per million interior cells. Instances with fewer interior
+// cells than this (PAINTOP_HIP_CORR_MIN_CELLS) keep the warm start alone;
+// 2048^2 and 4096^2 blends (1.2M / 4.9M per channel) gain 2.6x / 3.4x in time,
+// a batch of 512 blends of 32..1024 px lost 1.9x with them.
+static uint64_t warm_corr_min_cells() {
+  long long n = 1ll << 20;
+  if (const char *e = getenv("PAINTOP_HIP_CORR_MIN_CELLS")) n = atoll(e);
+  return (uint64_t)std::max(0LL, n);
+}
 static uint32_t warm_max_corrections() {
   long n = 16;
   if (const char *e = getenv("PAINTOP_HIP_CORRECTIONS")) n = atol(e);
@@ -939,6 +950,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     corr_alpha = g_pool.take<double>(net_list.size());
   }
   const uint32_t max_corr = warm_max_corrections();
+  const uint64_t corr_min_cells = warm_corr_min_cells();
   if (!cand.empty()) {
     const int nw = (int)cand.size();
     if (!net_list.empty()) {
@@ -1014,7 +1026,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       grow();
       launch_residual(kt);
       InstWork &Xw = inst[k];
-      const uint32_t n = warm_ramp_sweeps(w.omega, !w.in.start && max_corr > 0);
+      const bool cycles = !w.in.start && max_corr > 0 && (uint64_t)w.n_interior >= corr_min_cells;
+      const uint32_t n = warm_ramp_sweeps(w.omega, cycles);
       Xw.I.ramp_sweeps = n;
       Xw.I.ramp_log = n ? log(2.0 - w.omega) / n : 0.0;
       twin[k] = kt;
@@ -1029,7 +1042,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       const uint32_t pr = (warm_probe_sweeps(n) + half - 1) / half * half;
       probe[k] = probe[kt] = (int)std::min<uint32_t>(pr, d->max_iterations);
       act.push_back(kt);
-      if (!w.in.start && max_corr > 0) {
+      if (cycles) {
         // the first cycle ends past the probe, so the race is settled first
         cyc_len[k] = warm_cycle_sweeps(n, half);
         cyc_next[k] = std::max<uint32_t>(cyc_len[k], (uint32_t)probe[k] + half);
