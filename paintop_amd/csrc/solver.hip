@@ -333,7 +333,7 @@ static const double WARM_MAX_START_RATIO = 32.0;
 // problem Laplacian(e) = g with e = 0 on the ring; the network solves it on
 // the warm start's coarse grid and x += bilinear(e), after which the omega
 // ramp restarts. The cycle is a multiple of the ramp (PAINTOP_HIP_CYCLE_FACTOR,
-// default 2), on a launch boundary; PAINTOP_HIP_CORRECTIONS caps the count per
+// default 1.5), on a launch boundary; PAINTOP_HIP_CORRECTIONS caps the count per
 // channel (0 = none).
 // Corrections pay only where sweeps are dear: each round costs a network
 // forward, a few plane passes and a host round trip, ~0.5-1 ms, against
@@ -352,15 +352,20 @@ static uint32_t warm_max_corrections() {
   return (uint32_t)std::max(0L, std::min(n, 1000L));
 }
 static uint32_t warm_cycle_sweeps(uint32_t ramp, uint32_t half) {
-  double fct = 2.0;
+  double fct = 1.5;  // measured best at 2048^2 and 4096^2 (1..2 all within 20%)
   if (const char *e = getenv("PAINTOP_HIP_CYCLE_FACTOR")) fct = atof(e);
   if (!(fct > 0.0)) fct = 1.0;
   const uint32_t n = (uint32_t)std::min(1e5, std::ceil(fct * ramp));
   return std::max<uint32_t>(2 * half, (n + half - 1) / half * half);
 }
-// Corrections go on while each cycle ends at least this many times below the
-// end of the previous one (plain SOR gains about e per 1/(2 - omega) sweeps).
-static const double CORR_MIN_GAIN = 4.0;
+// Corrections go on while each cycle ends well below the end of the previous
+// one. Plain SOR gains about exp(c (2 - omega)) over c sweeps, ~1.45x over a
+// cycle of the default length; a cycle that gains less than CORR_MIN_GAIN
+// ends the corrections, and so do two in a row below CORR_GOOD_GAIN (a single
+// weak cycle, after a mediocre correction, still beats the plain rate that
+// ending would fall back to).
+static const double CORR_MIN_GAIN = 2.0;
+static const double CORR_GOOD_GAIN = 4.0;
 // Largest step of a correction (its Galerkin step on the coarse grid is
 // clamped to this).
 static const double CORR_MAX_STEP = 1.5;
@@ -914,7 +919,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   std::vector<char> is_twin, silent;
   // coarse-correction cycles of network instances: next cycle end (0 = none),
   // cycle length, corrections kept, residual at the previous cycle end
-  std::vector<uint32_t> cyc_next, cyc_len, cyc_count, cyc_undone;
+  std::vector<uint32_t> cyc_next, cyc_len, cyc_count, cyc_undone, cyc_weak;
   std::vector<double> cyc_prev;
   auto grow = [&]() {
     const size_t n = inst.size();
@@ -927,6 +932,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     cyc_len.resize(n, 0);
     cyc_count.resize(n, 0);
     cyc_undone.resize(n, 0);
+    cyc_weak.resize(n, 0);
     cyc_prev.resize(n, 0.0);
   };
   grow();
@@ -1193,7 +1199,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         const int k = act[c.a];
         const double rel = c.gnorm / inst[k].r0;
         // (after a correction taken back, the cycle ran uncorrected: no test)
-        const bool gain = cyc_count[k] == 0 || cyc_undone[k] || rel * CORR_MIN_GAIN <= cyc_prev[k];
+        bool gain = true;
+        if (cyc_count[k] > 0 && !cyc_undone[k]) {
+          const bool weak = rel * CORR_GOOD_GAIN > cyc_prev[k];
+          gain = rel * CORR_MIN_GAIN <= cyc_prev[k] && !(weak && cyc_weak[k]);
+          cyc_weak[k] = weak ? 1 : 0;
+        }
         cyc_prev[k] = rel;
         if (!gain || cyc_count[k] >= max_corr || !(c.gnorm > 0.0)) {
           cyc_next[k] = 0;
