@@ -218,6 +218,27 @@ struct UnNet {
     if (A.cout % 16 != 0 || (A.ca % 4) || (A.cb % 4) || (A.ca + A.cb != 4 && (A.ca + A.cb) % 16))
       throw fail(PH_CLASS_EXECUTION, "E_HIP_INTERNAL", "conv shape %d+%d -> %d", A.ca, A.cb,
                  A.cout);
+    // few pixels (the solver's batches of a handful of 128x128 grids below the
+    // top levels): split the taps over the block's waves instead
+    if (np <= UN_SPLITK_MAX_PIX && A.ca + A.cb >= 16) {
+      switch (MB) {
+#define UN_CONV_SK(M)                                                                    \
+  case M:                                                                                \
+    hipLaunchKernelGGL((k_conv3_sk<M>), dim3(cdiv(np, 16)), dim3(256), 0, st, A);        \
+    break;
+        UN_CONV_SK(1)
+        UN_CONV_SK(2)
+        UN_CONV_SK(3)
+        UN_CONV_SK(4)
+        UN_CONV_SK(6)
+        UN_CONV_SK(8)
+#undef UN_CONV_SK
+        default:
+          throw fail(PH_CLASS_EXECUTION, "E_HIP_INTERNAL", "conv: %d output channels", A.cout);
+      }
+      HIP_OK(hipGetLastError());
+      return;
+    }
     switch (MB) {
 #define UN_CONV(M, N)                                                                    \
   case M:                                                                                \

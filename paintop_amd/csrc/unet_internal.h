@@ -97,6 +97,10 @@ struct UnGather {
 
 template <int MB, int NB>
 __global__ void k_conv3(UnConvArgs A);
+template <int MB>
+__global__ void k_conv3_sk(UnConvArgs A);
+// Convolutions over at most this many pixels take the split-K kernel.
+constexpr int UN_SPLITK_MAX_PIX = 16384;
 __global__ void k_gather_dihedral(const float *xin, const float *yin, const float *win, UnGather G,
                                   int res, float *xo, float *yo, float *wo);
 __global__ void k_conv3_dw(UnDwArgs A);

@@ -172,6 +172,85 @@ __global__ __launch_bounds__(256) void k_conv3(UnConvArgs A) {
   }
 }
 
+// Split-K variant for convolutions over few pixels. In k_conv3 a wave's serial
+// chain is 9 taps x cin/16 chunks of load -> MFMA steps whatever the pixel
+// count, so a layer of a few thousand pixels runs on a few dozen waves, each
+// bound by load latency (~40 us for 64 channels). Here the 4 waves of a block
+// share one 16-pixel tile and split the taps (wave w: taps w, w+4, w+8); their
+// accumulators are added in wave order through LDS (deterministic), and wave w
+// finishes output blocks mb = w, w + 4. Requires cin % 16 == 0.
+template <int MB>
+__global__ __launch_bounds__(256) void k_conv3_sk(UnConvArgs A) {
+  __shared__ f32x4 red[4][MB][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hw = A.h * A.w;
+  const int npix = A.n * hw;
+  const int col = lane & 15, grp = lane >> 4;
+  const int cin = A.ca + A.cb;
+  const int h2 = A.h >> 1, w2 = A.w >> 1;
+  const int p = blockIdx.x * 16 + col;
+  const bool pv = p < npix;
+  const int pc = pv ? p : 0;
+  const int pimg = pc / hw;
+  const int rem = pc - pimg * hw;
+  const int py = rem / A.w, px = rem - py * A.w;
+
+  f32x4 acc[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t *wrow[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) wrow[mb] = A.wgt + (size_t)(mb * 16 + col) * A.kp + grp * 4;
+
+  for (int t = wave; t < 9; t += 4) {
+    const int dy = (t / 3 - 1) * A.dil, dx = (t % 3 - 1) * A.dil;
+    const int yy = py + dy, xx = px + dx;
+    const bool ok = pv && yy >= 0 && yy < A.h && xx >= 0 && xx < A.w;
+    const int ys = ok ? yy : 0, xs = ok ? xx : 0;
+    const uint16_t *sa = A.a + ((size_t)(pimg * A.h + ys) * A.w + xs) * A.ca + grp * 4;
+    const uint16_t *sb =
+        A.cb ? A.b + ((size_t)(pimg * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb + grp * 4 - A.ca
+             : nullptr;
+    const int kt = t * cin;
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+      bf16x4 bv = bf16x4{0, 0, 0, 0};
+      if (ok) bv = (c0 + grp * 4 < A.ca) ? ld4(sa + c0) : ld4(sb + c0);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) acc[mb] = mfma16(ld4(wrow[mb] + kt + c0), bv, acc[mb]);
+    }
+  }
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) red[wave][mb][lane] = acc[mb];
+  __syncthreads();
+  for (int mb = wave; mb < MB; mb += 4) {
+    const f32x4 v4 = ((red[0][mb][lane] + red[1][mb][lane]) + red[2][mb][lane]) + red[3][mb][lane];
+    if (!pv) continue;
+    const int co = mb * 16 + grp * 4;
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = v4[i];
+      if (A.bias) v[i] += A.bias[co + i];
+      if (A.relu) v[i] = fmaxf(v[i], 0.0f);
+    }
+    if (A.gate) {
+      const bf16x4 g = ld4(A.gate + (size_t)p * A.cout + co);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (!(bf2f(g[i]) > 0.0f)) v[i] = 0.0f;
+    }
+    st4(A.out + (size_t)p * A.cout + co,
+        bf16x4{(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])});
+  }
+}
+
+template __global__ void k_conv3_sk<1>(UnConvArgs);
+template __global__ void k_conv3_sk<2>(UnConvArgs);
+template __global__ void k_conv3_sk<3>(UnConvArgs);
+template __global__ void k_conv3_sk<4>(UnConvArgs);
+template __global__ void k_conv3_sk<6>(UnConvArgs);
+template __global__ void k_conv3_sk<8>(UnConvArgs);
+
 template __global__ void k_conv3<1, 4>(UnConvArgs);
 template __global__ void k_conv3<2, 4>(UnConvArgs);
 template __global__ void k_conv3<3, 4>(UnConvArgs);
