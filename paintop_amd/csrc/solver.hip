@@ -59,7 +59,8 @@ __global__ void k_coarsen(const float *, const float *, const float *, const flo
                           int, double, int, int, int, int, int, float *, float *, float *);
 __global__ void k_warm_apply(PhInst, const float *, const float *, const float *, int, int, int,
                              int);
-__global__ void k_coarsen_plane(PhInst, int, int, int, int, float *);
+__global__ void k_coarsen_plane(PhInst, int, int, int, int, int, float *);
+int coarsen_plane_cpb(int f);
 __global__ void k_in_norm(float *, int, float *);
 __global__ void k_correct_apply(PhInst, const float *, const float *, const double *, int, int,
                                 int, int, double);
@@ -1223,8 +1224,9 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         for (int j = 0; j < nb; ++j) {
           const int k = act[go[c0 + j].a];
           const Geo &g = wgeo[k];
-          hipLaunchKernelGGL(k_coarsen_plane, dim3(NR * NR), dim3(64), 0, st, inst[k].I, g.x0,
-                             g.y0, g.f, NR, net_in + NR2 * 4 * j);
+          const int cpb = coarsen_plane_cpb(g.f);
+          hipLaunchKernelGGL(k_coarsen_plane, dim3((NR + cpb - 1) / cpb, NR), dim3(256), 0, st,
+                             inst[k].I, g.x0, g.y0, g.f, NR, cpb, net_in + NR2 * 4 * j);
         }
         hipLaunchKernelGGL(k_in_norm, dim3(nb), dim3(256), 0, st, net_in, NR, net_scale + c0);
         HIP_OK(hipGetLastError());

@@ -11,6 +11,8 @@
 // x = P + bilinear(h) at interior cells.
 #include <math.h>
 
+#include <algorithm>
+
 #include "../../include/paintop_hip.h"
 #include "host_common.h"
 #include "ph_internal.h"
@@ -99,38 +101,110 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
 // Net input of a coarse correction, from an instance's plane: per block the
 // sum of g (the correction equation's rhs, Laplacian(e) = g) over interior
 // cells, zero boundary values, interior fraction and ring density, exactly as
-// k_coarsen lays them out. One wave per coarse cell.
-__global__ __launch_bounds__(64) void k_coarsen_plane(PhInst I, int x0, int y0, int f, int res,
-                                                      float *__restrict__ in4) {
-  const int c = blockIdx.x;
-  const int cx = c % res, cy = c / res;
-  const int lane = threadIdx.x;
-  double s_rhs = 0.0, n_int = 0.0, n_ring = 0.0;
-  for (int i = lane; i < f * f; i += 64) {
-    const int dy = i / f, dx = i - dy * f;
-    const int py = y0 + cy * f + dy - I.oy;
-    const int px = x0 + cx * f + dx - I.ox;
-    if (py < 1 || py >= I.ph - 1 || px < 1 || px >= I.pw - 1) continue;
-    {
-      const size_t p = (size_t)py * I.pitch + px;
-      if (I.cls[p]) {
-        n_int += 1.0;
-        s_rhs += (double)I.g[p];
-      } else if (I.cls[p - 1] || I.cls[p + 1] || I.cls[p - I.pitch] || I.cls[p + I.pitch]) {
-        n_ring += 1.0;
+// k_coarsen lays them out.
+// Block (bx, cy) takes coarse cells cx0 = bx*cpb .. cx0+cpb-1 of coarse row cy.
+// Each thread sums whole plane columns over the row's f plane rows (in row
+// order; adjacent threads read adjacent columns), the column sums go to LDS,
+// and one thread per coarse cell adds its f columns in column order, so the
+// result is deterministic. Columns go through LDS in chunks of
+// CP_MAX_COLS; a coarse cell wider than that (cpb = 1) adds chunk by chunk.
+constexpr int CP_MAX_COLS = 2048;
+__global__ __launch_bounds__(256) void k_coarsen_plane(PhInst I, int x0, int y0, int f, int res,
+                                                       int cpb, float *__restrict__ in4) {
+  __shared__ double s_rhs[CP_MAX_COLS];
+  __shared__ int s_int[CP_MAX_COLS], s_ring[CP_MAX_COLS];
+  const int cy = blockIdx.y;
+  const int cx0 = blockIdx.x * cpb;
+  const int ncc = min(cpb, res - cx0);
+  const int ncols = ncc * f;
+  const int pxb = x0 + cx0 * f - I.ox;  // plane column of the block's column 0
+  const int pyb = y0 + cy * f - I.oy;
+  const int ya = max(pyb, 1), yb = min(pyb + f, I.ph - 1);
+  double wr = 0.0, wi = 0.0, wn = 0.0;  // running sums of a wide cell (thread 0)
+  for (int j0 = 0; j0 < ncols; j0 += CP_MAX_COLS) {
+    const int nj = min(CP_MAX_COLS, ncols - j0);
+    // plane columns [c0, c1) of this chunk, inside the plane's non-padding
+    // columns, in groups of 4 aligned plane columns (one 4-byte load of cls
+    // and one 16-byte load of g per row); the rows above and below give the
+    // ring test, the bytes left and right of the group its edges
+    const int c0 = max(pxb + j0, 1), c1 = min(pxb + j0 + nj, I.pw - 1);
+    for (int j = threadIdx.x; j < nj; j += 256) {
+      s_rhs[j] = 0.0;
+      s_int[j] = 0;
+      s_ring[j] = 0;
+    }
+    __syncthreads();
+    const int ga = c0 & ~3;
+    for (int gx = ga + 4 * (int)threadIdx.x; gx < c1; gx += 4 * 256) {
+      double sr[4] = {0.0, 0.0, 0.0, 0.0};
+      int ni[4] = {0, 0, 0, 0}, nr[4] = {0, 0, 0, 0};
+      for (int py = ya; py < yb; ++py) {
+        const size_t p = (size_t)py * I.pitch + gx;
+        const uchar4 cc = *reinterpret_cast<const uchar4 *>(I.cls + p);
+        const uchar4 cu = *reinterpret_cast<const uchar4 *>(I.cls + p - I.pitch);
+        const uchar4 cd = *reinterpret_cast<const uchar4 *>(I.cls + p + I.pitch);
+        const unsigned char cl = I.cls[p - 1], cr = I.cls[p + 4];
+        const unsigned char c[6] = {cl, cc.x, cc.y, cc.z, cc.w, cr};
+        const unsigned char u[4] = {cu.x, cu.y, cu.z, cu.w}, d[4] = {cd.x, cd.y, cd.z, cd.w};
+        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (cc.x | cc.y | cc.z | cc.w) gv = *reinterpret_cast<const float4 *>(I.g + p);
+        const float gg[4] = {gv.x, gv.y, gv.z, gv.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (c[k + 1]) {
+            ni[k] += 1;
+            sr[k] += (double)gg[k];
+          } else if (c[k] | c[k + 2] | u[k] | d[k]) {
+            nr[k] += 1;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int px = gx + k;
+        if (px < c0 || px >= c1) continue;
+        const int j = px - pxb - j0;
+        s_rhs[j] = sr[k];
+        s_int[j] = ni[k];
+        s_ring[j] = nr[k];
       }
     }
+    __syncthreads();
+    if (f > CP_MAX_COLS) {
+      if (threadIdx.x == 0)
+        for (int j = 0; j < nj; ++j) {
+          wr += s_rhs[j];
+          wi += s_int[j];
+          wn += s_ring[j];
+        }
+    } else if ((int)threadIdx.x < ncc) {
+      const int c = threadIdx.x;
+      double sr = 0.0, ni = 0.0, nr = 0.0;
+      for (int j = c * f; j < c * f + f; ++j) {
+        sr += s_rhs[j];
+        ni += s_int[j];
+        nr += s_ring[j];
+      }
+      float *o = in4 + ((size_t)cy * res + cx0 + c) * 4;
+      o[0] = (float)sr;
+      o[1] = 0.0f;
+      o[2] = (float)(ni / ((double)f * f));
+      o[3] = (float)(nr / f);
+    }
+    __syncthreads();
   }
-  s_rhs = wave_sum_d(s_rhs);
-  n_int = wave_sum_d(n_int);
-  n_ring = wave_sum_d(n_ring);
-  if (lane == 0) {
-    in4[(size_t)c * 4 + 0] = (float)s_rhs;
-    in4[(size_t)c * 4 + 1] = 0.0f;
-    in4[(size_t)c * 4 + 2] = (float)(n_int / ((double)f * f));
-    in4[(size_t)c * 4 + 3] = (float)(n_ring / f);
+  if (f > CP_MAX_COLS && threadIdx.x == 0) {
+    float *o = in4 + ((size_t)cy * res + cx0) * 4;
+    o[0] = (float)wr;
+    o[1] = 0.0f;
+    o[2] = (float)(wi / ((double)f * f));
+    o[3] = (float)(wn / f);
   }
 }
+
+// Coarse cells per block of k_coarsen_plane: about one 4-column group per
+// thread (each thread's serial chain is then the f rows of its group).
+int coarsen_plane_cpb(int f) { return std::max(1, std::min(32, 1024 / f)); }
 
 // The problems are linear: each net input is scaled so that its rhs and
 // boundary channels have unit RMS over the grid (scale[b] = that RMS, 1 when
