@@ -115,6 +115,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return t;
 }
 
+// Slices of the two-stage reductions over a warm-start net input (input
+// normalisation, correction step).
+constexpr int PH_NET_SLICES = 16;
+
 struct PhTile {
   int32_t inst;
   int32_t px0; // first loaded plane column (valid columns start at px0 + K)

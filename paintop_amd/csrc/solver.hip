@@ -61,10 +61,12 @@ __global__ void k_warm_apply(PhInst, const float *, const float *, const float *
                              int);
 __global__ void k_coarsen_plane(PhInst, int, int, int, int, int, float *);
 int coarsen_plane_cpb(int f);
-__global__ void k_in_norm(float *, int, float *);
+__global__ void k_in_norm_part(const float *, int, double *);
+__global__ void k_in_norm_apply(float *, int, const double *, float *);
+__global__ void k_corr_alpha_part(const float *, const float *, int, double *);
 __global__ void k_correct_apply(PhInst, const float *, const float *, const double *, int, int,
                                 int, int, double);
-__global__ void k_corr_alpha(const float *, const float *, int, double, double *);
+__global__ void k_corr_alpha(const double *, int, double, double *);
 __global__ void k_start_apply(PhInst, const float *);
 bool warm_net_ready();
 int warm_net_res();
@@ -657,7 +659,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       size_t nch = 0;
       for (const ItemIn &it : items) nch += it.C;
       need += (R2 * 4 * warm_net_batch() + (R2 + 1) * nch) * sizeof(float) +
-              nch * sizeof(double) + 1280;
+              (nch + 2 * PH_NET_SLICES * (size_t)warm_net_batch()) * sizeof(double) + 1536;
     }
     g_pool.reset(need);
   }
@@ -954,7 +956,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int NR = warm_net_res(), NNB = warm_net_batch();
   const size_t NR2 = (size_t)NR * NR;
   float *net_in = nullptr, *net_h = nullptr, *net_scale = nullptr;
-  double *corr_alpha = nullptr;
+  double *corr_alpha = nullptr, *net_part = nullptr;
   if (!net_list.empty()) {
     net_in = g_pool.take<float>(NR2 * 4 * NNB);
     // one prediction and one input scale per network instance (the warm
@@ -962,6 +964,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     net_h = g_pool.take<float>(NR2 * net_list.size());
     net_scale = g_pool.take<float>(net_list.size());
     corr_alpha = g_pool.take<double>(net_list.size());
+    net_part = g_pool.take<double>((size_t)2 * PH_NET_SLICES * NNB);
   }
   const uint32_t max_corr = warm_max_corrections();
   const uint64_t corr_min_cells = warm_corr_min_cells();
@@ -984,7 +987,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                              nullptr, it.W, it.H, it.C, X.channel, d->lambda, g.x0, g.y0, g.f,
                              NR, 0, net_in + NR2 * 4 * j, nullptr, nullptr);
         }
-        hipLaunchKernelGGL(k_in_norm, dim3(nb), dim3(256), 0, st, net_in, NR, net_scale + j0);
+        hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
+                           net_part);
+        hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
+                           net_part, net_scale + j0);
         HIP_OK(hipGetLastError());
         warm_net_forward(net_in, net_h + NR2 * j0, nb, st);
       }
@@ -1228,12 +1234,16 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           hipLaunchKernelGGL(k_coarsen_plane, dim3((NR + cpb - 1) / cpb, NR), dim3(256), 0, st,
                              inst[k].I, g.x0, g.y0, g.f, NR, cpb, net_in + NR2 * 4 * j);
         }
-        hipLaunchKernelGGL(k_in_norm, dim3(nb), dim3(256), 0, st, net_in, NR, net_scale + c0);
+        hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
+                           net_part);
+        hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
+                           net_part, net_scale + c0);
         HIP_OK(hipGetLastError());
         warm_net_forward(net_in, net_h + NR2 * c0, nb, st);
-        for (int j = 0; j < nb; ++j)
-          hipLaunchKernelGGL(k_corr_alpha, dim3(1), dim3(1024), 0, st, net_in + NR2 * 4 * j,
-                             net_h + NR2 * (c0 + j), NR, CORR_MAX_STEP, corr_alpha + c0 + j);
+        hipLaunchKernelGGL(k_corr_alpha_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in,
+                           net_h + NR2 * c0, NR, net_part);
+        hipLaunchKernelGGL(k_corr_alpha, dim3((nb + 63) / 64), dim3(64), 0, st, net_part, nb,
+                           CORR_MAX_STEP, corr_alpha + c0);
         HIP_OK(hipGetLastError());
       }
       for (int j = 0; j < nc; ++j) {

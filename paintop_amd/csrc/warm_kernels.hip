@@ -208,16 +208,20 @@ int coarsen_plane_cpb(int f) { return std::max(1, std::min(32, 1024 / f)); }
 
 // The problems are linear: each net input is scaled so that its rhs and
 // boundary channels have unit RMS over the grid (scale[b] = that RMS, 1 when
-// both are zero), and the prediction is scaled back. One block per input,
-// fixed-order reduction.
-__global__ __launch_bounds__(256) void k_in_norm(float *__restrict__ in4, int res,
-                                                 float *__restrict__ scale) {
-  const int b = blockIdx.x;
-  float *v = in4 + (size_t)b * res * res * 4;
+// both are zero), and the prediction is scaled back. Two kernels over
+// (NORM_SLICES slices, inputs): per-slice sums of squares, then every slice
+// adds the slices' sums in order (deterministic) and divides its cells.
+constexpr int NORM_SLICES = PH_NET_SLICES;
+__global__ __launch_bounds__(256) void k_in_norm_part(const float *__restrict__ in4, int res,
+                                                      double *__restrict__ part) {
+  const int sl = blockIdx.x, b = blockIdx.y;
+  const int n = res * res, per = (n + NORM_SLICES - 1) / NORM_SLICES;
+  const int i0 = sl * per, i1 = min(n, i0 + per);
+  const float *v = in4 + (size_t)b * n * 4;
   double s = 0.0;
-  for (int i = threadIdx.x; i < res * res; i += 256) {
-    const double a = v[4 * i], c = v[4 * i + 1];
-    s += a * a + c * c;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double x = v[4 * i], y = v[4 * i + 1];
+    s += x * x + y * y;
   }
   __shared__ double red[256];
   red[threadIdx.x] = s;
@@ -226,13 +230,25 @@ __global__ __launch_bounds__(256) void k_in_norm(float *__restrict__ in4, int re
     if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  const double rms = sqrt(red[0] / ((double)res * res));
+  if (threadIdx.x == 0) part[(size_t)b * NORM_SLICES + sl] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_in_norm_apply(float *__restrict__ in4, int res,
+                                                       const double *__restrict__ part,
+                                                       float *__restrict__ scale) {
+  const int sl = blockIdx.x, b = blockIdx.y;
+  const int n = res * res, per = (n + NORM_SLICES - 1) / NORM_SLICES;
+  const int i0 = sl * per, i1 = min(n, i0 + per);
+  double t = 0.0;
+  for (int k = 0; k < NORM_SLICES; ++k) t += part[(size_t)b * NORM_SLICES + k];
+  const double rms = sqrt(t / ((double)n));
   const float sc = rms > 0.0 ? (float)rms : 1.0f;
-  for (int i = threadIdx.x; i < res * res; i += 256) {
+  float *v = in4 + (size_t)b * n * 4;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     v[4 * i] /= sc;
     v[4 * i + 1] /= sc;
   }
-  if (threadIdx.x == 0) scale[b] = sc;
+  if (sl == 0 && threadIdx.x == 0) scale[b] = sc;
 }
 
 __device__ __forceinline__ double warm_bilinear(const float *hc, int res, int x0, int y0, int f,
@@ -258,13 +274,19 @@ __device__ __forceinline__ double warm_bilinear(const float *hc, int res, int x0
 // wrong-sized one back; it is taken over bulk blocks only (every cell
 // interior, 4 neighbour blocks with interior cells), where A_c holds, and
 // clamped to [0, max_step] (0: no usable correction). Input and prediction
-// carry the same normalisation, so alpha is scale-free. One block of 1024,
-// fixed-order reduction.
-__global__ __launch_bounds__(1024) void k_corr_alpha(const float *__restrict__ in4,
-                                                     const float *__restrict__ hc, int res,
-                                                     double max_step, double *__restrict__ alpha) {
+// carry the same normalisation, so alpha is scale-free.
+// Two kernels over (NORM_SLICES slices, corrections): per-slice sums of
+// G H and (A_c H) H, then one lane per correction adds the slices in order.
+__global__ __launch_bounds__(256) void k_corr_alpha_part(const float *__restrict__ in4_all,
+                                                         const float *__restrict__ h_all,
+                                                         int res, double *__restrict__ part) {
+  const int sl = blockIdx.x, b = blockIdx.y;
+  const int n = res * res, per = (n + NORM_SLICES - 1) / NORM_SLICES;
+  const int i0 = sl * per, i1 = min(n, i0 + per);
+  const float *in4 = in4_all + (size_t)b * n * 4;
+  const float *hc = h_all + (size_t)b * n;
   double s1 = 0.0, s2 = 0.0;
-  for (int c = threadIdx.x; c < res * res; c += 1024) {
+  for (int c = i0 + threadIdx.x; c < i1; c += 256) {
     const int cx = c % res, cy = c / res;
     if (cx < 1 || cy < 1 || cx >= res - 1 || cy >= res - 1) continue;
     if (!(in4[(size_t)c * 4 + 2] >= 1.0f)) continue;
@@ -276,11 +298,11 @@ __global__ __launch_bounds__(1024) void k_corr_alpha(const float *__restrict__ i
     s1 = fma((double)in4[(size_t)c * 4], h, s1);
     s2 = fma(ah, h, s2);
   }
-  __shared__ double red[2][1024];
+  __shared__ double red[2][256];
   red[0][threadIdx.x] = s1;
   red[1][threadIdx.x] = s2;
   __syncthreads();
-  for (int k = 512; k >= 1; k >>= 1) {
+  for (int k = 128; k >= 1; k >>= 1) {
     if ((int)threadIdx.x < k) {
       red[0][threadIdx.x] += red[0][threadIdx.x + k];
       red[1][threadIdx.x] += red[1][threadIdx.x + k];
@@ -288,9 +310,22 @@ __global__ __launch_bounds__(1024) void k_corr_alpha(const float *__restrict__ i
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double a = red[0][0] / red[1][0];
-    alpha[0] = (red[1][0] < 0.0 && isfinite(a)) ? fmin(fmax(a, 0.0), max_step) : 0.0;
+    part[((size_t)b * NORM_SLICES + sl) * 2] = red[0][0];
+    part[((size_t)b * NORM_SLICES + sl) * 2 + 1] = red[1][0];
   }
+}
+
+__global__ void k_corr_alpha(const double *__restrict__ part, int n, double max_step,
+                             double *__restrict__ alpha) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int k = 0; k < NORM_SLICES; ++k) {
+    s1 += part[((size_t)b * NORM_SLICES + k) * 2];
+    s2 += part[((size_t)b * NORM_SLICES + k) * 2 + 1];
+  }
+  const double a = s1 / s2;
+  alpha[b] = (s2 < 0.0 && isfinite(a)) ? fmin(fmax(a, 0.0), max_step) : 0.0;
 }
 
 // x += sign * alpha * scale * bilinear(h) at the interior cells of an

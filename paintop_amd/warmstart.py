@@ -148,15 +148,29 @@ def sample_spec(i: int) -> dict:
                 seed_s=2 * (SEED0 + i) + 1_000_001, seed_t=2 * (SEED0 + i) + 1_000_002)
 
 
+# sweeps after which mode-2 pairs stop the learned solver (one value per chunk)
+STATE_SWEEPS = (16, 24, 32, 48, 64, 96, 128, 192)
+# mode-2 states closer to the solution than this relative residual are dropped:
+# e = u - x is then too small against the f32 rounding of u and x
+STATE_MIN_REL = 1e-4
+
+
 def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, log=print,
                    mode: int = 0):
     """Solve instances first .. first+n-1 with the plain solver and return the
     coarse (inputs [n,R,R,4], target [n,R,R], weight [n,R,R]) as host arrays,
-    plus the solver's sweep counts. mode 0: warm-start pairs (fbm source and
-    target). mode 1: coarse-correction pairs -- the blend of an fbm source into
-    a zero target solves Laplacian(u) = Laplacian(source), u = 0 on the ring,
-    which is the correction problem Laplacian(e) = g, e = 0 on the ring, with
-    g = the source's Laplacian."""
+    plus the solver's sweep counts.
+
+    mode 0: warm-start pairs (fbm source and target).
+    mode 1: coarse-correction pairs -- the blend of an fbm source into a zero
+    target solves Laplacian(u) = Laplacian(source), u = 0 on the ring, which is
+    the correction problem Laplacian(e) = g, e = 0 on the ring, with g = the
+    source's Laplacian.
+    mode 2: correction pairs from solver states -- the learned solver (warm
+    start and corrections, with the weights currently loaded) is stopped after
+    m sweeps (x); e = u - x against the converged u is the error a correction
+    at that point has to remove (e = 0 on the ring, Laplacian(e) = g there).
+    These are coarsened as a mode-1 problem with source e."""
     from .device import DeviceArray, synchronize
 
     R2 = RES * RES
@@ -175,7 +189,7 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
             src, tgt, msk, out = (DeviceArray((s, s), np.float32) for _ in range(4))
             hip.fbm_device(src.ptr, s, s, sp["freq"], sp["seed_s"], 6, 2.0, 0.5, scale=0.5,
                            bias=0.5)
-            if mode == 0:
+            if mode != 1:
                 hip.fbm_device(tgt.ptr, s, s, sp["freq"], sp["seed_t"], 6, 2.0, 0.5, scale=0.5,
                                bias=0.5)
             else:
@@ -188,6 +202,23 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
             bufs.append((src, tgt, msk, out))
             items.append((s, s, 1, src.ptr, tgt.ptr, msk.ptr, out.ptr))
         reps = hip.poisson_solve_batch_device(desc, items, want_history=False)
+        part_ok = [True] * len(specs)
+        if mode == 2:
+            # the learned solver's state after m sweeps, then e = u - x in the
+            # source buffer, a zero anchor, and e as the solution
+            m = STATE_SWEEPS[(first + c0) // chunk % len(STATE_SWEEPS)]
+            xs = [DeviceArray((sp["size"], sp["size"]), np.float32) for sp in specs]
+            d2 = hip.make_desc(1, 1, 1, m, tol, flags=hip.PH_FLAG_WARM_START_NET)
+            items2 = [(w_, h_, c_, s_, t_, m_, x.ptr)
+                      for (w_, h_, c_, s_, t_, m_, _o), x in zip(items, xs)]
+            reps2 = hip.poisson_solve_batch_device(d2, items2, want_history=False)
+            for j, (sp, (src, tgt, msk, out), x, r2) in enumerate(zip(specs, bufs, xs, reps2)):
+                e = out.numpy().astype(np.float64) - x.numpy().astype(np.float64)
+                src.copy_from(DeviceArray.from_numpy(e.astype(np.float32)))
+                tgt.zero()
+                out.copy_from(src)
+                part_ok[j] = (not r2.converged) and r2.final_residual >= STATE_MIN_REL
+                x.free()
         din = DeviceArray((len(specs), RES, RES, 4), np.float32)
         dt = DeviceArray((len(specs), RES, RES), np.float32)
         dw = DeviceArray((len(specs), RES, RES), np.float32)
@@ -195,7 +226,8 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
             s = sp["size"]
             hip.warm_coarsen_device(src.ptr, tgt.ptr, msk.ptr, out.ptr, s, s, 1, 0, 0.0,
                                     sp["x0"], sp["y0"], sp["f"], RES, din.ptr + j * R2 * 16,
-                                    dt.ptr + j * R2 * 4, dw.ptr + j * R2 * 4, mode=mode)
+                                    dt.ptr + j * R2 * 4, dw.ptr + j * R2 * 4,
+                                    mode=min(mode, 1))
         synchronize()
         k = slice(c0, c0 + len(specs))
         X[k], Y[k], Wt[k] = din.numpy(), dt.numpy(), dw.numpy()
@@ -208,7 +240,9 @@ def generate_pairs(n: int, first: int = 0, chunk: int = 96, tol: float = 1e-7, l
         for j, r in enumerate(reps):
             # a solve that ends (stalled) within 10x the tolerance is still a
             # clean target; anything else is dropped (zero loss weight)
-            if not (r.converged or r.final_residual <= 10 * tol):
+            if not part_ok[j]:
+                Wt[c0 + j] = 0.0  # a state too close to the solution (mode 2)
+            elif not (r.converged or r.final_residual <= 10 * tol):
                 Wt[c0 + j] = 0.0
                 dropped.append((first + c0 + j, r.stop_reason, r.final_residual))
                 log(f"[warmstart] instance {first + c0 + j} dropped: {r.stop_reason} at "
@@ -242,7 +276,8 @@ def loss_weight(X: np.ndarray, Wt: np.ndarray) -> np.ndarray:
 # training
 
 def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, seed=1,
-          val_frac=0.05, ckpt=None, log=print, time_budget=None, pyramid=1.0, augment=True):
+          val_frac=0.05, ckpt=None, log=print, time_budget=None, pyramid=1.0, augment=True,
+          init=None):
     """AdamW on the GPU with a cosine schedule; each training example is drawn
     under a random one of the 8 dihedral maps of the grid (augment). Returns
     (params, history)."""
@@ -262,7 +297,7 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
         hip.unet_gather_device(dX.ptr, dY.ptr, dW.ptr, n, idx, modes, RES, bX.ptr, bY.ptr,
                                bW.ptr)
 
-    net = hip.UNet(batch, RES, seed=seed)
+    net = hip.UNet(batch, RES, params=init, seed=seed)
     step0 = 0
     rng = np.random.default_rng(seed)
     if ckpt and os.path.exists(ckpt):
@@ -357,16 +392,30 @@ def main(argv=None):
     t.add_argument("--no-augment", action="store_true", help="no dihedral augmentation")
     t.add_argument("--correction-frac", type=float, default=0.5,
                    help="fraction of the samples that are coarse-correction problems")
+    t.add_argument("--state-frac", type=float, default=0.0,
+                   help="fraction of the samples that are correction problems from the "
+                        "learned solver's own states (needs --init or the shipped weights)")
+    t.add_argument("--init", default=None,
+                   help="start from these weights (.npy); also the solver's for --state-frac")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
         t0 = time.time()
         n1 = int(round(a.samples * a.correction_frac))
-        n0 = a.samples - n1
+        n2 = int(round(a.samples * a.state_frac))
+        n0 = a.samples - n1 - n2
+        init = load_weights(a.init) if a.init else None
         parts = [generate_pairs(n0, log=log, mode=0)] if n0 else []
         if n1:
             # correction problems on instances of their own (other masks and textures)
             parts.append(generate_pairs(n1, first=n0, log=log, mode=1))
+        if n2:
+            # the solver's own states, corrections on for these small crops too
+            hip.warm_start_set_weights(init if init is not None else load_weights())
+            os.environ["PAINTOP_HIP_CORR_MIN_CELLS"] = "0"
+            parts.append(generate_pairs(n2, first=n0 + n1, log=log, mode=2))
+            del os.environ["PAINTOP_HIP_CORR_MIN_CELLS"]
+            hip.warm_start_set_weights(None)
         X, Y, Wt, sweeps = (np.concatenate(z) for z in zip(*parts))
         # validation examples are the last ones: interleave the two kinds
         order = np.random.default_rng(7).permutation(len(X))
@@ -376,10 +425,12 @@ def main(argv=None):
         Wl = loss_weight(X, Wt)
         params, info = train(X, Y, Wl, epochs=a.epochs, batch=a.batch, lr=a.lr, ckpt=a.ckpt,
                              log=log, time_budget=a.time_budget, pyramid=a.pyramid,
-                             augment=not a.no_augment)
+                             augment=not a.no_augment, init=init)
         os.makedirs(a.out, exist_ok=True)
         np.save(os.path.join(a.out, "warm_start_unet.npy"), params.astype(np.float32))
-        meta = dict(samples=a.samples, correction_samples=n1, epochs=a.epochs, batch=a.batch,
+        meta = dict(samples=a.samples, correction_samples=n1, state_samples=n2,
+                    init=os.path.basename(a.init) if a.init else None, epochs=a.epochs,
+                    batch=a.batch,
                     lr=a.lr, input_normalisation="rms of channels 0-1",
                     pyramid=a.pyramid, augment=not a.no_augment,
                     dropped=int((Wt.reshape(len(Wt), -1).max(axis=1) == 0).sum()),
