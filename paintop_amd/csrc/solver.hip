@@ -625,14 +625,16 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int nitem = (int)items.size();
   const int K = d->half_sweeps ? (int)d->half_sweeps : 16;
   const int WV = 256 - 2 * K;
-  // Tile height: one wave streams hb + 2K rows per launch, so a launch lasts
-  // at least one wave's pass. Large calls fill the chip with 48-row tiles;
-  // below ~32M pixel-channels (a 2048^2 RGBA blend) 32-row tiles give twice
-  // the waves per SIMD and measured 15% faster (27.1 against 32.0 ms), while
-  // 4096^2 RGBA keeps 48 (its measured best).
+  // Tile height: one wave per tile streams hb + 2K rows per launch, and at
+  // 235 VGPRs two waves fit a SIMD (2048 on the chip), so a launch takes
+  // ceil(live tiles / 2048) rounds of an (hb + 2K)-row pass, and fewer, taller
+  // tiles hide latency worse. Measured plain solves (ms): 1024^2 RGBA 15.1 at
+  // 16 rows against 18.4 at 32; 2048^2 23.7 at 16, 27.0 at 32; 4096^2 66.7
+  // at 40, 71.8 at 48, 93-100 at 24-36 (a third round), 84-112 at 64-128;
+  // 8192^2 screened and the 512-blend batch equal at 40 and 48.
   uint64_t work_px = 0;
   for (const ItemIn &it : items) work_px += (uint64_t)it.W * it.H * it.C;
-  const int hb = d->tile_rows ? (int)d->tile_rows : (work_px >= (32ull << 20) ? 48 : 32);
+  const int hb = d->tile_rows ? (int)d->tile_rows : (work_px >= (32ull << 20) ? 40 : 16);
   const bool timing = (d->flags & PH_FLAG_TIMING) != 0;
   const bool tuned = d->omega <= 0.0;
   const bool want_warm =
