@@ -128,6 +128,19 @@ typedef struct ph_device_info {
   uint64_t hbm_bytes;
 } ph_device_info;
 
+/* Result of assert.no_change_outside_mask@1's scan (the assertion's metrics,
+ * crates/paintop-cpu/src/assert.rs:405-446 and :690-710). */
+#define PH_MAX_LOCATIONS 256 /* assert.rs:72 */
+typedef struct ph_outside_scan {
+  uint64_t changed_outside; /* outside pixels whose delta strictly exceeds the threshold */
+  double max_outside;       /* largest delta over outside pixels, 0 if none */
+  int64_t worst_x, worst_y; /* first pixel with the largest leaking delta; -1 if no leak */
+  double worst_delta;       /* its delta, 0 if no leak */
+  uint32_t n_locations;     /* min(changed_outside, PH_MAX_LOCATIONS) */
+  uint32_t reserved_;
+  int64_t locations[PH_MAX_LOCATIONS][2]; /* first leaking pixels, row-major order, (x, y) */
+} ph_outside_scan;
+
 /* Library ABI version (PH_ABI_VERSION). */
 int32_t ph_abi_version(void);
 
@@ -294,6 +307,25 @@ ph_status ph_unet_gather_device(const float *d_in_all, const float *d_target_all
 /* The weights the solver's warm start (PH_FLAG_WARM_START_NET) runs with;
  * params = NULL clears them. */
 ph_status ph_warm_start_set_weights(const float *params, uint64_t n_params, ph_error *err);
+
+/* assert.no_change_outside_mask@1 on device buffers: the scan of
+ * NoChangeOutsideMask::compute (crates/paintop-cpu/src/assert.rs:630-710,
+ * scan_outside :405-446) over `before`/`after` (width*height*channels f32,
+ * channel-interleaved) and the `allowed` coverage mask (width*height f32).
+ * A pixel is outside when its coverage is <= coverage_epsilon; it leaks when
+ * its largest channel |after - before| (sRGB-decoded first when decode_srgb,
+ * i.e. comparison_space "decoded-linear" on an sRGB image, assert.rs:677-679)
+ * strictly exceeds outside_threshold. Both parameters must be finite and
+ * >= 0 (E_ASSERT_PARAM, class Schema, as resolve_non_negative :157-176);
+ * channels == 0 gives the empty scan. passed = (changed_outside == 0); the
+ * severity and the report wrapping stay on the host. With decode_srgb the
+ * power is taken in double and rounded once (within one f32 ulp of powf). */
+ph_status ph_assert_no_change_outside_mask_device(const float *d_before, const float *d_after,
+                                                  const float *d_mask, uint32_t width,
+                                                  uint32_t height, uint32_t channels,
+                                                  int32_t decode_srgb, double outside_threshold,
+                                                  double coverage_epsilon, void *stream,
+                                                  ph_outside_scan *out, ph_error *err);
 
 /* Net input for one channel of an image (and, when d_solution is given, the
  * training target): the region [x0, x0 + res*f) x [y0, y0 + res*f) of the

@@ -11,6 +11,7 @@
  *   or_solve                 crates/paintop-cpu/src/poisson.rs:168-265 (+ STALL_* :75-85)
  *   or_blend_channels        crates/paintop-cpu/src/poisson_blend.rs:153-246
  *   or_fbm                   crates/paintop-cpu/src/noise.rs:72-160, 218-240, 660-694
+ *   or_outside_scan          crates/paintop-cpu/src/assert.rs:72-85, 178-201, 405-446
  *   or_rbsor_omp             no reference counterpart: the reference has no
  *                            cpu.optimized Poisson kernel, so this OpenMP
  *                            red-black SOR is the stand-in timed as the
@@ -84,6 +85,23 @@ int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double
                  const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
                  double omega, const double *omega_tab, int threads, uint32_t stall_window,
                  double *field, double *history, or_report *report);
+
+/* assert.no_change_outside_mask@1's scan (crates/paintop-cpu/src/assert.rs:405-446).
+ * Pixel indices are row-major; (x, y) = (i % width, i / width) (assert.rs:196-201). */
+#define OR_MAX_LOCATIONS 256 /* assert.rs:72 */
+typedef struct or_outside_result {
+  uint64_t changed_outside;
+  double max_outside;
+  int64_t worst_index; /* -1: no leak */
+  double worst_delta;
+  uint32_t n_locations;
+  uint32_t _pad;
+  int64_t locations[OR_MAX_LOCATIONS];
+} or_outside_result;
+
+void or_outside_scan(const float *before, const float *after, const float *mask, size_t width,
+                     size_t height, size_t channels, int decode, double outside_threshold,
+                     double coverage_epsilon, or_outside_result *out);
 
 #ifdef __cplusplus
 }

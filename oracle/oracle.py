@@ -7,7 +7,8 @@ product package never imports it.
 Each function names the reference code it restates (bobisme/paintop):
 ``solve`` -> crates/paintop-cpu/src/poisson.rs:168-265,
 ``blend_channels`` -> crates/paintop-cpu/src/poisson_blend.rs:168-246,
-``fbm`` -> crates/paintop-cpu/src/noise.rs:72-160, 218-240, 660-694.
+``fbm`` -> crates/paintop-cpu/src/noise.rs:72-160, 218-240, 660-694,
+``outside_scan`` -> crates/paintop-cpu/src/assert.rs:72-85, 178-201, 405-446.
 """
 from __future__ import annotations
 
@@ -34,6 +35,18 @@ class _Report(ctypes.Structure):
         ("final_residual", ctypes.c_double),
         ("history_len", ctypes.c_uint32),
         ("_pad2", ctypes.c_uint32),
+    ]
+
+
+class _OutsideResult(ctypes.Structure):
+    _fields_ = [
+        ("changed_outside", ctypes.c_uint64),
+        ("max_outside", ctypes.c_double),
+        ("worst_index", ctypes.c_int64),
+        ("worst_delta", ctypes.c_double),
+        ("n_locations", ctypes.c_uint32),
+        ("_pad", ctypes.c_uint32),
+        ("locations", ctypes.c_int64 * 256),
     ]
 
 
@@ -94,6 +107,8 @@ def lib():
                                    ctypes.c_double, ctypes.c_double, dp, ctypes.c_int,
                                    ctypes.c_uint32, dp, dp, ctypes.POINTER(_Report)]
         L.or_rbsor_omp.restype = ctypes.c_int
+        L.or_outside_scan.argtypes = [fp, fp, fp, sz, sz, sz, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.POINTER(_OutsideResult)]
         _lib = L
     return _lib
 
@@ -227,3 +242,24 @@ def fbm(width, height, frequency, seed, octaves, lacunarity=2.0, gain=0.5) -> np
     lib().or_fbm(width, height, float(frequency), seed & 0xFFFFFFFFFFFFFFFF, int(octaves),
                  float(lacunarity), float(gain), _p(out, ctypes.c_float))
     return out
+
+
+def outside_scan(before: np.ndarray, after: np.ndarray, mask: np.ndarray, decode: bool = False,
+                 outside_threshold: float = 0.0, coverage_epsilon: float = 0.0) -> dict:
+    """assert.no_change_outside_mask@1's scan (assert.rs:405-446). before/after
+    (H, W, C) or (H, W) f32, mask (H, W) f32. Returns the scan with (x, y)
+    coordinates (assert.rs:196-201)."""
+    b = np.ascontiguousarray(before, dtype=np.float32)
+    a = np.ascontiguousarray(after, dtype=np.float32)
+    m = np.ascontiguousarray(mask, dtype=np.float32)
+    h, w = m.shape
+    c = 1 if b.ndim == 2 else b.shape[2]
+    r = _OutsideResult()
+    fp = ctypes.c_float
+    lib().or_outside_scan(_p(b, fp), _p(a, fp), _p(m, fp), w, h, c, int(bool(decode)),
+                          float(outside_threshold), float(coverage_epsilon), ctypes.byref(r))
+    xy = lambda i: (int(i) % w, int(i) // w)  # noqa: E731
+    return dict(changed_outside=int(r.changed_outside), max_outside=float(r.max_outside),
+                worst_pixel=None if r.worst_index < 0 else xy(r.worst_index),
+                worst_delta=float(r.worst_delta),
+                locations=[xy(r.locations[k]) for k in range(r.n_locations)])

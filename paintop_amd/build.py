@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpaintop_hip.so")
 SOURCES = [
     "sor_kernels.hip", "fp64_kernels.hip", "gen_kernels.hip", "solver.hip",
-    "unet_kernels.hip", "unet.hip", "warm_kernels.hip",
+    "unet_kernels.hip", "unet.hip", "warm_kernels.hip", "assert_kernels.hip",
 ]
 
 HIPCC_FLAGS = [

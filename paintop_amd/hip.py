@@ -46,7 +46,24 @@ EXPORTED_SYMBOLS = (
     "ph_warm_coarsen_device",
     "ph_poisson_solve_start_device",
     "ph_unet_gather_device",
+    "ph_assert_no_change_outside_mask_device",
 )
+
+
+MAX_LOCATIONS = 256  # PH_MAX_LOCATIONS (assert.rs:72)
+
+
+class PhOutsideScan(ctypes.Structure):
+    _fields_ = [
+        ("changed_outside", ctypes.c_uint64),
+        ("max_outside", ctypes.c_double),
+        ("worst_x", ctypes.c_int64),
+        ("worst_y", ctypes.c_int64),
+        ("worst_delta", ctypes.c_double),
+        ("n_locations", ctypes.c_uint32),
+        ("reserved_", ctypes.c_uint32),
+        ("locations", (ctypes.c_int64 * 2) * MAX_LOCATIONS),
+    ]
 
 
 class HipLibraryMissing(RuntimeError):
@@ -190,6 +207,11 @@ def lib():
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, vp,
                                          ctypes.POINTER(PhError)]
     L.ph_ellipse_mask_device.restype = ctypes.c_int
+    L.ph_assert_no_change_outside_mask_device.argtypes = [
+        vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+        ctypes.c_double, ctypes.c_double, vp, ctypes.POINTER(PhOutsideScan),
+        ctypes.POINTER(PhError)]
+    L.ph_assert_no_change_outside_mask_device.restype = ctypes.c_int
     L.ph_gauss_noise_device.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, vp,
                                         ctypes.POINTER(PhError)]
     L.ph_gauss_noise_device.restype = ctypes.c_int
@@ -373,6 +395,27 @@ def polygon_mask_device(d_out: int, width, height, points, even_odd=False, strea
                                       len(pts) // 2, 1 if even_odd else 0, stream or None,
                                       ctypes.byref(err))
     _check(st, err)
+
+
+def assert_no_change_outside_mask_device(d_before: int, d_after: int, d_mask: int, width: int,
+                                         height: int, channels: int, decode_srgb: bool = False,
+                                         outside_threshold: float = 0.0,
+                                         coverage_epsilon: float = 0.0, stream: int = 0) -> dict:
+    """assert.no_change_outside_mask@1's scan on device buffers
+    (ph_assert_no_change_outside_mask_device). Returns the metrics with (x, y)
+    coordinates."""
+    r = PhOutsideScan()
+    err = PhError()
+    st = lib().ph_assert_no_change_outside_mask_device(
+        d_before or None, d_after or None, d_mask or None, width, height, channels,
+        int(bool(decode_srgb)), float(outside_threshold), float(coverage_epsilon),
+        stream or None, ctypes.byref(r), ctypes.byref(err))
+    _check(st, err)
+    return dict(changed_outside=int(r.changed_outside), max_outside=float(r.max_outside),
+                worst_pixel=None if r.worst_x < 0 else (int(r.worst_x), int(r.worst_y)),
+                worst_delta=float(r.worst_delta),
+                locations=[(int(r.locations[k][0]), int(r.locations[k][1]))
+                           for k in range(r.n_locations)])
 
 
 def ellipse_mask_device(d_out: int, width, height, cx, cy, rx, ry, stream: int = 0):

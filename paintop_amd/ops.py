@@ -18,7 +18,8 @@ import numpy as np
 
 from . import hip
 from .errors import E_GPU_UNAVAILABLE, ErrorClass, PaintopError
-from .values import ImageDescriptor, MaskDescriptor, Report, ResourceValue, SolverData
+from .values import (AssertionOutcome, ImageDescriptor, MaskDescriptor, Report, ResourceValue,
+                     SolverData)
 
 POISSON_BLEND_OP_ID = "repair.poisson_blend@1"
 SCREENED_POISSON_OP_ID = "repair.screened_poisson@1"
@@ -258,3 +259,117 @@ class ScreenedPoisson(_HipPoissonOp):
 def gpu_unavailable(reason: str) -> PaintopError:
     return PaintopError(ErrorClass.POLICY, E_GPU_UNAVAILABLE,
                         f"no compatible GPU adapter is available: {reason}")
+
+
+# ---------------------------------------------------------------------------
+# assert.no_change_outside_mask@1 (crates/paintop-cpu/src/assert.rs), the check
+# that runs after a blend: the scan on the GPU, the params, errors and verdict
+# as the reference op has them.
+
+NO_CHANGE_OUTSIDE_MASK_OP_ID = "assert.no_change_outside_mask@1"
+E_ASSERT_INPUT = "E_ASSERT_INPUT"  # assert.rs:59
+E_ASSERT_SHAPE = "E_ASSERT_SHAPE"  # assert.rs:63
+E_ASSERT_PARAM = "E_ASSERT_PARAM"  # assert.rs:66
+
+
+def _resolve_comparison_space(params: dict) -> str:
+    """ComparisonSpace::resolve (assert.rs:104-128): decoded-linear by default."""
+    if "comparison_space" not in params:
+        return "decoded-linear"
+    v = params["comparison_space"]
+    if v in ("decoded-linear", "encoded"):
+        return v
+    raise PaintopError(ErrorClass.SCHEMA, E_ASSERT_PARAM,
+                       "assert.no_change_outside_mask `comparison_space` must be "
+                       "`decoded-linear` or `encoded`")
+
+
+def _resolve_severity(params: dict) -> str:
+    """resolve_severity (assert.rs:132-153): error by default."""
+    if "severity" not in params:
+        return "error"
+    v = params["severity"]
+    if v in ("error", "warning", "metric"):
+        return v
+    raise PaintopError(ErrorClass.SCHEMA, E_ASSERT_PARAM,
+                       "assertion `severity` must be `error`, `warning`, or `metric`")
+
+
+def _resolve_non_negative(params: dict, name: str) -> float:
+    """resolve_non_negative (assert.rs:157-176): finite and >= 0, default 0."""
+    if name not in params:
+        return 0.0
+    v = params[name]
+    ok = isinstance(v, (int, float)) and not isinstance(v, bool)
+    if not ok or not math.isfinite(float(v)) or float(v) < 0.0:
+        raise PaintopError(ErrorClass.SCHEMA, E_ASSERT_PARAM,
+                           f"assert.no_change_outside_mask `{name}` must be a finite "
+                           f"non-negative number")
+    return float(v)
+
+
+class NoChangeOutsideMask:
+    """assert.no_change_outside_mask@1 with the scan on the GPU
+    (NoChangeOutsideMask::compute, assert.rs:630-710;
+    ph_assert_no_change_outside_mask_device). The inputs are host values, as
+    the executor hands them; ``scan_device`` takes device buffers directly."""
+
+    op_id = NO_CHANGE_OUTSIDE_MASK_OP_ID
+    op_name = NO_CHANGE_OUTSIDE_MASK_OP_ID
+
+    def _input(self, inputs, port):
+        if port not in inputs:
+            raise PaintopError(ErrorClass.REFERENCE, E_ASSERT_INPUT,
+                               f"{self.op_name} requires a `{port}` input value")
+        return inputs[port]
+
+    def compute(self, inputs: dict, params: dict) -> dict:
+        params = params or {}
+        space = _resolve_comparison_space(params)
+        severity = _resolve_severity(params)
+        threshold = _resolve_non_negative(params, "outside_threshold")
+        epsilon = _resolve_non_negative(params, "coverage_epsilon")
+        before, after, allowed = (self._input(inputs, p) for p in ("before", "after", "allowed"))
+        for port, v in (("before", before), ("after", after)):
+            if not isinstance(v.descriptor, ImageDescriptor):
+                raise PaintopError(ErrorClass.TYPE, E_ASSERT_INPUT,
+                                   f"{self.op_name} `{port}` input must be a image resource")
+        bd, ad = before.descriptor, after.descriptor
+        if bd.extent != ad.extent or before.channels != after.channels:
+            raise PaintopError(ErrorClass.SEMANTIC, E_ASSERT_SHAPE,
+                               f"{self.op_name}: the `before` and `after` images must share an "
+                               f"extent and channel count")
+        if not isinstance(allowed.descriptor, MaskDescriptor):
+            raise PaintopError(ErrorClass.TYPE, E_ASSERT_INPUT,
+                               f"{self.op_name} `allowed` input must be a mask resource")
+        if allowed.descriptor.extent != bd.extent:
+            raise PaintopError(ErrorClass.SEMANTIC, E_ASSERT_SHAPE,
+                               f"{self.op_name}: the `allowed` mask must share the images' extent")
+        decode = space == "decoded-linear" and bd.color == "Srgb"
+        from .device import DeviceArray
+
+        arrs = [DeviceArray.from_numpy(x.samples) for x in (before, after, allowed)]
+        try:
+            scan = self.scan_device(arrs[0].ptr, arrs[1].ptr, arrs[2].ptr, bd.extent.width,
+                                    bd.extent.height, before.channels, decode, threshold, epsilon)
+        finally:
+            for a in arrs:
+                a.free()
+        outcome = AssertionOutcome(
+            assertion=self.op_id, passed=scan["changed_outside"] == 0, severity=severity,
+            max_abs_delta_outside=scan["max_outside"],
+            changed_pixels_outside=scan["changed_outside"], worst_pixel=scan["worst_pixel"],
+            locations=scan["locations"])
+        report = Report(extent=bd.extent, channels=0, all_finite=True, solver=None,
+                        assertion=outcome)
+        return {"report": ResourceValue.report_value(report)}
+
+    @staticmethod
+    def scan_device(d_before, d_after, d_mask, width, height, channels, decode=False,
+                    outside_threshold=0.0, coverage_epsilon=0.0, stream=0) -> dict:
+        try:
+            return hip.assert_no_change_outside_mask_device(
+                d_before, d_after, d_mask, width, height, channels, decode, outside_threshold,
+                coverage_epsilon, stream)
+        except hip.HipCallError as e:
+            raise PaintopError(ErrorClass(e.error_class), e.code, e.message) from None

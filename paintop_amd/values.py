@@ -26,6 +26,7 @@ class Extent:
 class ImageDescriptor:
     extent: Extent
     layout: str = "Gray"  # Gray | GrayAlpha | Rgb | Rgba
+    color: str = "LinearSrgb"  # ColorEncoding: LinearSrgb | Srgb (resource.rs)
 
     @property
     def channels(self) -> int:
@@ -73,9 +74,28 @@ class SolverData:
 
 
 @dataclass
+class AssertionOutcome:
+    """paintop_ir::AssertionOutcome (resource.rs:982-1040), the fields
+    assert.no_change_outside_mask@1 fills (assert.rs:694-708)."""
+
+    assertion: str
+    passed: bool
+    severity: str  # error | warning | metric
+    max_abs_delta_outside: Optional[float] = None
+    changed_pixels_outside: Optional[int] = None
+    worst_pixel: Optional[tuple] = None
+    locations: list = field(default_factory=list)
+
+    def fails_run(self) -> bool:
+        """AssertionSeverity::fails_run (resource.rs:962-964): only an
+        `error` severity fails the run."""
+        return (not self.passed) and self.severity == "error"
+
+
+@dataclass
 class Report:
     """The op report (resource.rs:1066-1113) as the Poisson ops fill it
-    (poisson_blend.rs:581-593)."""
+    (poisson_blend.rs:581-593), or an assertion's verdict."""
 
     extent: Extent
     channels: int
@@ -83,6 +103,7 @@ class Report:
     solver: Optional[SolverData]
     channel_stats: list = field(default_factory=list)
     content_hash: str = ""
+    assertion: Optional[AssertionOutcome] = None
 
 
 class ResourceValue:
