@@ -49,7 +49,8 @@ def parse():
                    help="time only the plain start (no learned warm start)")
     p.add_argument("--learned", action="store_true",
                    help="batch workload: run the learned warm start and corrections")
-    p.add_argument("--workload", choices=["seamless", "screened", "batch"], default="seamless",
+    p.add_argument("--workload", choices=["seamless", "screened", "batch", "outside-scan"],
+                   default="seamless",
                    help="seamless: 4096^2 RGBA blend to 1e-6 (the headline); screened: "
                         "8192^2 Gray screened Poisson, lambda 0.01, to 1e-7; batch: 512 "
                         "uneven single-channel blends (sides 32..1024) in one call, to 1e-6")
@@ -114,6 +115,51 @@ def cpu_baselines(size, omega, gpu_sweeps, budget_s):
                                "at omega 1.8 it cannot reach 1e-6 on this workload within the "
                                "reference's 100000-iteration cap"),
     }
+
+
+def bench_outside_scan(args, info):
+    """assert.no_change_outside_mask@1's device scan after the blend: the
+    4096^2 RGBA target against the solved candidate with the star mask (nothing
+    leaks, so every pixel is read and no location is written). One step = one
+    scan; HBM-bound: (2 x channels + 1) x 4 B per pixel."""
+    from paintop_amd import hip
+    from paintop_amd.device import DeviceArray, synchronize
+    from paintop_amd.inputs import seamless_clone_case
+
+    n, c = args.size, 4
+    src, tgt, msk = seamless_clone_case(n, channels=c)
+    out = DeviceArray(tgt.shape, np.float32)
+    hip.poisson_solve_device(hip.make_desc(n, n, c, args.max_iterations, args.tol), src.ptr,
+                             tgt.ptr, msk.ptr, out.ptr)
+
+    def step():
+        return hip.assert_no_change_outside_mask_device(tgt.ptr, out.ptr, msk.ptr, n, n, c)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        r = step()  # returns after the result is on the host
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    alg = (2 * c + 1) * 4.0 * n * n
+    gbps = alg / (ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": round(ms / (n * n / 1e6), 5), "unit": "ms/MP", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (device-generated)",
+        "config": {"workload": f"assert.no_change_outside_mask scan, {n}x{n} RGBA target vs "
+                               f"solved candidate, star mask, encoded space",
+                   "changed_outside": r["changed_outside"], "device": info["name"]},
+        "roofline": {"kernel": "k_os_scan + k_os_total + k_os_locs (one scan, host round trip "
+                               "included)", "bound": "hbm", "achieved": round(gbps, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": None},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_batch(args, info):
@@ -185,6 +231,8 @@ def main():
     size = args.size
     if args.workload == "batch":
         return bench_batch(args, info)
+    if args.workload == "outside-scan":
+        return bench_outside_scan(args, info)
     if args.workload == "screened":
         from paintop_amd.inputs import screened_case
         if size == 4096:

@@ -120,6 +120,88 @@ __global__ __launch_bounds__(OS_THREADS) void k_os_scan(OsArgs A, uint64_t *__re
   }
 }
 
+// The same scan over groups of 4 consecutive pixels (C <= 4 channels, 16-byte
+// aligned buffers, n a multiple of 4): one 16-byte load of the mask, then the
+// group's 4*C samples of before and after as C 16-byte loads each, skipped
+// when all four pixels are inside. Thread t of block b takes groups
+// b*OS_PIX/4 + s*OS_THREADS + t, s < OS_STEPS/4 (the block's pixel range is
+// the generic kernel's).
+template <int C>
+__global__ __launch_bounds__(OS_THREADS) void k_os_scan_v(OsArgs A, uint64_t *__restrict__ cnt,
+                                                          double *__restrict__ mx,
+                                                          int64_t *__restrict__ ag) {
+  const uint64_t b = blockIdx.x;
+  uint64_t count = 0;
+  double m = 0.0;
+  int64_t arg = -1;
+  const float4 *mk4 = reinterpret_cast<const float4 *>(A.mask);
+  const float4 *bf4 = reinterpret_cast<const float4 *>(A.before);
+  const float4 *af4 = reinterpret_cast<const float4 *>(A.after);
+  for (int s = 0; s < OS_STEPS / 4; ++s) {
+    const uint64_t g = b * (OS_PIX / 4) + (uint64_t)s * OS_THREADS + threadIdx.x;
+    if (4 * g >= A.n) break;
+    const float4 mv = mk4[g];
+    const float cov[4] = {mv.x, mv.y, mv.z, mv.w};
+    bool out[4], any = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      out[j] = !((double)cov[j] > A.epsilon);
+      any |= out[j];
+    }
+    if (!any) continue;
+    float bv[4 * C], av[4 * C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const float4 x = bf4[g * C + q], y = af4[g * C + q];
+      bv[4 * q] = x.x; bv[4 * q + 1] = x.y; bv[4 * q + 2] = x.z; bv[4 * q + 3] = x.w;
+      av[4 * q] = y.x; av[4 * q + 1] = y.y; av[4 * q + 2] = y.z; av[4 * q + 3] = y.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!out[j]) continue;
+      double pd = 0.0;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        float x = av[j * C + c], y = bv[j * C + c];
+        if (A.decode) {
+          x = os_decode(x);
+          y = os_decode(y);
+        }
+        const double d = (double)fabsf(x - y);
+        if (d > pd) pd = d;
+      }
+      if (pd > A.threshold) count += 1;
+      if (arg < 0 || pd > m) {
+        m = pd;
+        arg = (int64_t)(4 * g + j);
+      }
+    }
+  }
+  __shared__ uint64_t s_cnt[OS_THREADS];
+  __shared__ double s_m[OS_THREADS];
+  __shared__ int64_t s_arg[OS_THREADS];
+  s_cnt[threadIdx.x] = count;
+  s_m[threadIdx.x] = m;
+  s_arg[threadIdx.x] = arg;
+  __syncthreads();
+  for (int h = OS_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + h];
+      double mm = s_m[threadIdx.x];
+      int64_t aa = s_arg[threadIdx.x];
+      os_better(mm, aa, s_m[threadIdx.x + h], s_arg[threadIdx.x + h]);
+      s_m[threadIdx.x] = mm;
+      s_arg[threadIdx.x] = aa;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    cnt[b] = s_cnt[0];
+    mx[b] = s_m[0];
+    ag[b] = s_arg[0];
+  }
+}
+
 // res: [0] total leaks (u64), [1] max (f64 bits), [2] first argmax (i64)
 __global__ __launch_bounds__(OS_THREADS) void k_os_total(const uint64_t *__restrict__ cnt,
                                                          const double *__restrict__ mx,
@@ -263,7 +345,18 @@ extern "C" ph_status ph_assert_no_change_outside_mask_device(
       throw fail(PH_CLASS_EXECUTION, "E_HIP_WORKSPACE", "scan workspace overrun");
     OsArgs A{d_before, d_after, d_mask, n, (int)channels, decode_srgb ? 1 : 0,
              outside_threshold, coverage_epsilon};
-    hipLaunchKernelGGL(k_os_scan, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, mx, ag);
+    const bool aligned = ((uintptr_t)d_before % 16 == 0) && ((uintptr_t)d_after % 16 == 0) &&
+                         ((uintptr_t)d_mask % 16 == 0) && (n % 4 == 0);
+    if (aligned && channels >= 1 && channels <= 4) {
+      switch (channels) {
+        case 1: hipLaunchKernelGGL(k_os_scan_v<1>, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, mx, ag); break;
+        case 2: hipLaunchKernelGGL(k_os_scan_v<2>, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, mx, ag); break;
+        case 3: hipLaunchKernelGGL(k_os_scan_v<3>, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, mx, ag); break;
+        default: hipLaunchKernelGGL(k_os_scan_v<4>, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, mx, ag); break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_os_scan, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, mx, ag);
+    }
     hipLaunchKernelGGL(k_os_total, dim3(1), dim3(OS_THREADS), 0, st, cnt, mx, ag, nblocks, off,
                        res);
     hipLaunchKernelGGL(k_os_locs, dim3(nblocks), dim3(OS_THREADS), 0, st, A, cnt, off, loc);
