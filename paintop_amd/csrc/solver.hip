@@ -338,6 +338,17 @@ static const double WARM_MAX_START_RATIO = 32.0;
 // ramp restarts. The cycle is a multiple of the ramp (PAINTOP_HIP_CYCLE_FACTOR,
 // default 1.5), on a launch boundary; PAINTOP_HIP_CORRECTIONS caps the count per
 // channel (0 = none).
+// The network's start pays only where sweeps are dear too: below this many
+// interior cells per channel (PAINTOP_HIP_WARM_MIN_CELLS) a channel takes the
+// plain start. Its race twin and network pass cost more than they saved on the
+// batch of 512 blends of 32..1024 px (137 against 120 ms), while a 1024^2
+// blend (0.3M cells per channel) gains 1.7x.
+static uint64_t warm_net_min_cells() {
+  long long n = 1ll << 18;
+  if (const char *e = getenv("PAINTOP_HIP_WARM_MIN_CELLS")) n = atoll(e);
+  return (uint64_t)std::max(0LL, n);
+}
+
 // Corrections pay only where sweeps are dear: each round costs a network
 // forward, a few plane passes and a host round trip, ~0.5-1 ms, against
 // sweeps of ~3 us per million interior cells. Instances with fewer interior
@@ -945,9 +956,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   // candidates: images with a caller's start field, and (with the network
   // flag) every other Poisson-blend instance
   std::vector<int> cand, net_list;
+  const uint64_t warm_min_cells = warm_net_min_cells();
   for (int k : act) {
     if (iw[inst[k].item].in.start) cand.push_back(k);
-    else if (want_warm && d->lambda == 0.0) {
+    else if (want_warm && d->lambda == 0.0 &&
+             (uint64_t)iw[inst[k].item].n_interior >= warm_min_cells) {
       cand.push_back(k);
       net_list.push_back(k);
     }

@@ -166,13 +166,15 @@ def test_coarsen_matches_numpy(gpu, mode):
             assert gw[cy, cx] == (1.0 if ni else (0.5 if nr else 0.0))
 
 
-def test_bad_weights_fall_back_to_the_plain_start(gpu):
+def test_bad_weights_fall_back_to_the_plain_start(gpu, monkeypatch):
     """Injected bad weights: the network's start is worse than the anchor
     start, so the solver keeps the plain start and the result is bit-identical
-    to the plain solve."""
+    to the plain solve. (These images are below the network start's size gate;
+    the gate is lowered so the network runs.)"""
     from paintop_amd.device import DeviceArray
     from paintop_amd.inputs import batch_case
 
+    monkeypatch.setenv("PAINTOP_HIP_WARM_MIN_CELLS", "0")
     cases = batch_case(3, seed0=950)
     bad = U.init_params(5) * 40.0
     gpu.warm_start_set_weights(bad)
@@ -195,6 +197,30 @@ def test_bad_weights_fall_back_to_the_plain_start(gpu):
             assert r3.warm_start_rel == 0.0 and r3.residual_history == r1.residual_history
     finally:
         gpu.warm_start_set_weights(None)
+
+
+def test_small_images_take_the_plain_start(gpu):
+    """Below the network start's size gate (2^18 interior cells per channel)
+    a channel takes the plain start: with the flag set and weights loaded the
+    result and history are the plain solve's, bit for bit, and the network is
+    not tried (warm_start_rel stays 0)."""
+    from paintop_amd import warmstart
+    from paintop_amd.device import DeviceArray
+
+    w = h = 256  # an ellipse of ~28k interior cells
+    s, t, m = _fbm_case(w, h, 1, 0.5, 960, "ellipse")
+    o1, o2 = DeviceArray((h, w), np.float32), DeviceArray((h, w), np.float32)
+    d = hip.make_desc(w, h, 1, 20000, 1e-6)
+    r1 = gpu.poisson_solve_device(d, s.ptr, t.ptr, m.ptr, o1.ptr)
+    warmstart.enable()
+    try:
+        d.flags = hip.PH_FLAG_WARM_START_NET
+        r2 = gpu.poisson_solve_device(d, s.ptr, t.ptr, m.ptr, o2.ptr)
+    finally:
+        gpu.warm_start_set_weights(None)
+    assert r2.warm_start_used == 0 and r2.warm_start_rel == 0.0
+    assert o1.numpy().tobytes() == o2.numpy().tobytes()
+    assert r1.residual_history == r2.residual_history
 
 
 def test_warm_start_without_weights_is_an_error(gpu):
