@@ -327,6 +327,21 @@ ph_status ph_assert_no_change_outside_mask_device(const float *d_before, const f
                                                   double coverage_epsilon, void *stream,
                                                   ph_outside_scan *out, ph_error *err);
 
+/* The coarse-correction input the solver builds from an instance's plane
+ * (no counterpart in the reference; exposed so the kernel can be checked on
+ * its own): per f x f block of the region [x0, x0 + res*f) x [y0, y0 + res*f)
+ * of image coordinates, in[..][0] = sum of g over interior cells,
+ * in[..][1] = 0, in[..][2] = interior cells / f^2, in[..][3] = ring cells / f,
+ * where interior = cls != 0 and ring = non-interior cells with an interior
+ * 4-neighbour. The plane is pitch x ph (g f32, cls u8 row-major; plane cell
+ * (px, py) is image cell (ox + px, oy + py)); rows 0 and ph - 1 and columns 0
+ * and pw - 1 are padding (never read as cells), pitch is a multiple of 4, g is
+ * 16-byte and cls 4-byte aligned. d_in is [res][res][4]. */
+ph_status ph_coarsen_plane_device(const float *d_g, const uint8_t *d_cls, int32_t pitch,
+                                  int32_t pw, int32_t ph, int32_t ox, int32_t oy, int32_t x0,
+                                  int32_t y0, int32_t factor, int32_t res, float *d_in,
+                                  void *stream, ph_error *err);
+
 /* Net input for one channel of an image (and, when d_solution is given, the
  * training target): the region [x0, x0 + res*f) x [y0, y0 + res*f) of the
  * image in f x f blocks. With interior = (mask > 0.5 and not on the image

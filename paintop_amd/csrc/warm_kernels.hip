@@ -406,3 +406,38 @@ extern "C" ph_status ph_warm_coarsen_device(const float *d_source, const float *
     return f.code == "E_GPU_UNAVAILABLE" ? PH_UNAVAILABLE : PH_ERROR;
   }
 }
+
+extern "C" ph_status ph_coarsen_plane_device(const float *d_g, const uint8_t *d_cls, int32_t pitch,
+                                             int32_t pw, int32_t ph, int32_t ox, int32_t oy,
+                                             int32_t x0, int32_t y0, int32_t factor,
+                                             int32_t res, float *d_in, void *stream,
+                                             ph_error *err) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  clear_error(err);
+  try {
+    ensure_device();
+    if (!d_g || !d_cls || !d_in) throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
+    if (pw < 3 || ph < 3 || pitch < pw || pitch % 4 != 0 || factor < 1 || res < 1 ||
+        res > 4096 || (uintptr_t)d_g % 16 != 0 || (uintptr_t)d_cls % 4 != 0)
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT",
+                 "coarsen_plane: plane %dx%d pitch %d, factor %d, resolution %d (pitch a "
+                 "multiple of 4, g 16-byte and cls 4-byte aligned)",
+                 pw, ph, pitch, factor, res);
+    PhInst I{};
+    I.g = const_cast<float *>(d_g);
+    I.cls = d_cls;
+    I.pitch = pitch;
+    I.pw = pw;
+    I.ph = ph;
+    I.ox = ox;
+    I.oy = oy;
+    const int cpb = coarsen_plane_cpb(factor);
+    hipLaunchKernelGGL(k_coarsen_plane, dim3((res + cpb - 1) / cpb, res), dim3(256), 0,
+                       (hipStream_t)stream, I, x0, y0, factor, res, cpb, d_in);
+    HIP_OK(hipGetLastError());
+    return PH_OK;
+  } catch (const Fail &f) {
+    put_error(err, f);
+    return f.code == "E_GPU_UNAVAILABLE" ? PH_UNAVAILABLE : PH_ERROR;
+  }
+}

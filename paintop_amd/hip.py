@@ -47,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "ph_poisson_solve_start_device",
     "ph_unet_gather_device",
     "ph_assert_no_change_outside_mask_device",
+    "ph_coarsen_plane_device",
 )
 
 
@@ -212,6 +213,9 @@ def lib():
         ctypes.c_double, ctypes.c_double, vp, ctypes.POINTER(PhOutsideScan),
         ctypes.POINTER(PhError)]
     L.ph_assert_no_change_outside_mask_device.restype = ctypes.c_int
+    L.ph_coarsen_plane_device.argtypes = ([vp, vp] + [ctypes.c_int32] * 9 +
+                                          [vp, vp, ctypes.POINTER(PhError)])
+    L.ph_coarsen_plane_device.restype = ctypes.c_int
     L.ph_gauss_noise_device.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, vp,
                                         ctypes.POINTER(PhError)]
     L.ph_gauss_noise_device.restype = ctypes.c_int
@@ -394,6 +398,15 @@ def polygon_mask_device(d_out: int, width, height, points, even_odd=False, strea
                                       pts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                       len(pts) // 2, 1 if even_odd else 0, stream or None,
                                       ctypes.byref(err))
+    _check(st, err)
+
+
+def coarsen_plane_device(d_g: int, d_cls: int, pitch: int, pw: int, ph: int, ox: int, oy: int,
+                         x0: int, y0: int, factor: int, res: int, d_in: int, stream: int = 0):
+    """The solver's coarse-correction input from a plane (ph_coarsen_plane_device)."""
+    err = PhError()
+    st = lib().ph_coarsen_plane_device(d_g, d_cls, pitch, pw, ph, ox, oy, x0, y0, factor, res,
+                                       d_in, stream or None, ctypes.byref(err))
     _check(st, err)
 
 

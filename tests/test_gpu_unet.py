@@ -166,6 +166,45 @@ def test_coarsen_matches_numpy(gpu, mode):
             assert gw[cy, cx] == (1.0 if ni else (0.5 if nr else 0.0))
 
 
+@pytest.mark.parametrize("pw,ph,pitch,f,res,x0,y0", [
+    (90, 70, 128, 5, 16, -7, -3),      # grid larger than the plane, negative offsets
+    (257, 200, 320, 30, 8, 3, 11),     # 8 cells per block
+    (300, 60, 320, 300, 1, -20, -5),   # f > 256: one coarse cell per block
+    (64, 64, 64, 1, 64, 0, 0),         # f = 1, pitch == pw
+])
+def test_coarsen_plane_matches_numpy(gpu, pw, ph, pitch, f, res, x0, y0):
+    """The correction input the solver builds from a plane (k_coarsen_plane
+    through ph_coarsen_plane_device) against numpy: per coarse cell the sum of g
+    over interior cells, the interior fraction and the ring density, with the
+    plane's padding rows and columns never counted."""
+    from paintop_amd.device import DeviceArray
+
+    rng = np.random.default_rng(pw * 31 + f)
+    ox, oy = 5, -2
+    cls = (rng.random((ph, pitch)) < 0.5).astype(np.uint8)
+    cls[0, :] = cls[-1, :] = 0
+    cls[:, 0] = 0
+    cls[:, pw - 1:] = 0
+    g = np.where(cls > 0, rng.standard_normal((ph, pitch)), np.nan).astype(np.float32)
+    dg, dc = DeviceArray.from_numpy(g), DeviceArray.from_numpy(cls)
+    din = DeviceArray((res, res, 4), np.float32)
+    gpu.coarsen_plane_device(dg.ptr, dc.ptr, pitch, pw, ph, ox, oy, x0, y0, f, res, din.ptr)
+    got = din.numpy()
+    pad = np.pad(cls, 1)
+    near = (pad[:-2, 1:-1] | pad[2:, 1:-1] | pad[1:-1, :-2] | pad[1:-1, 2:]) > 0
+    valid = np.zeros_like(cls, dtype=bool)
+    valid[1:ph - 1, 1:pw - 1] = True
+    inside = (cls > 0) & valid
+    ring = (cls == 0) & near & valid
+    g64 = np.where(inside, g.astype(np.float64), 0.0)
+    for cy in range(res):
+        for cx in range(res):
+            ys = slice(max(0, y0 + cy * f - oy), max(0, min(ph, y0 + cy * f + f - oy)))
+            xs = slice(max(0, x0 + cx * f - ox), max(0, min(pitch, x0 + cx * f + f - ox)))
+            exp = [g64[ys, xs].sum(), 0.0, inside[ys, xs].sum() / f ** 2, ring[ys, xs].sum() / f]
+            assert np.allclose(got[cy, cx], exp, rtol=1e-5, atol=1e-5), (cy, cx, got[cy, cx], exp)
+
+
 def test_bad_weights_fall_back_to_the_plain_start(gpu, monkeypatch):
     """Injected bad weights: the network's start is worse than the anchor
     start, so the solver keeps the plain start and the result is bit-identical
