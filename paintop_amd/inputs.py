@@ -192,3 +192,38 @@ def batch_case(n: int = 512, seed0: int = 500):
         out.append((w, h, src, tgt, msk))
     synchronize()
     return out
+
+
+def spots_case(size: int = 4096, n_spots: int = 48, channels: int = 4, seed: int = 601):
+    """Blemish-style blend: fbm source/target (as seamless_clone_case) and a mask
+    of n_spots disjoint elliptical spots (semi-axes 12..96 px) scattered over
+    the image. Returns DeviceArrays (source, target) and the host mask."""
+    from . import hip
+    from .device import DeviceArray, synchronize
+
+    src = DeviceArray((size, size, channels), np.float32)
+    tgt = DeviceArray((size, size, channels), np.float32)
+    for c in range(channels):
+        hip.fbm_device(src.ptr, size, size, 0.125, 101 + c, 6, 2.0, 0.5,
+                       channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5)
+        hip.fbm_device(tgt.ptr, size, size, 0.125, 201 + c, 6, 2.0, 0.5,
+                       channel_stride=channels, channel_offset=c, scale=0.5, bias=0.5)
+    rng = SplitMix64(seed)
+    msk = np.zeros((size, size), np.float32)
+    placed = []
+    tries = 0
+    while len(placed) < n_spots and tries < 20 * n_spots:
+        tries += 1
+        rx, ry = rng.uniform(12.0, 96.0), rng.uniform(12.0, 96.0)
+        cx, cy = rng.uniform(rx + 4, size - rx - 4), rng.uniform(ry + 4, size - ry - 4)
+        if any(abs(cx - px) < rx + prx + 6 and abs(cy - py) < ry + pry + 6
+               for px, py, prx, pry in placed):
+            continue
+        placed.append((cx, cy, rx, ry))
+        y0, y1 = int(cy - ry) - 1, int(cy + ry) + 2
+        x0, x1 = int(cx - rx) - 1, int(cx + rx) + 2
+        yy, xx = np.mgrid[y0:y1, x0:x1]
+        inside = ((xx + 0.5 - cx) / rx) ** 2 + ((yy + 0.5 - cy) / ry) ** 2 <= 1.0
+        msk[y0:y1, x0:x1][inside] = 1.0
+    synchronize()
+    return src, tgt, msk

@@ -160,7 +160,8 @@ class _HipPoissonOp:
     kind = ""
     ph_op = hip.PH_OP_POISSON_BLEND
 
-    def __init__(self, omega: float = 0.0, half_sweeps: int = 0, warm_start: bool = False):
+    def __init__(self, omega: float = 0.0, half_sweeps: int = 0, warm_start: bool = False,
+                 split_components: bool = False):
         # SOR factor and half-sweeps per launch: tuned when 0. These are backend
         # tuning knobs, not plan params (the reference fixes omega = 1.8 in its
         # own backend, poisson_blend.rs:78). warm_start: the learned warm start
@@ -169,6 +170,11 @@ class _HipPoissonOp:
         self.omega = omega
         self.half_sweeps = half_sweeps
         self.warm_start = warm_start
+        # split_components: solve each 4-connected component of the interior
+        # on its own bounding box, all in one batched call (components.py);
+        # the report is the worst component's
+        self.split_components = split_components
+        self.last_components: list = []
         if warm_start:
             from . import warmstart
 
@@ -208,7 +214,14 @@ class _HipPoissonOp:
                              omega=self.omega, half_sweeps=self.half_sweeps,
                              flags=hip.PH_FLAG_WARM_START_NET if self.warm_start else 0)
         try:
-            out, res = hip.poisson_solve(desc, a.samples, b.samples, m.samples)
+            res = None
+            if self.split_components:
+                from . import components
+
+                out, res, self.last_components = components.solve_components(
+                    desc, a.array(), b.array(), m.array()[..., 0])
+            if res is None:  # not split, or no interior cell at all
+                out, res = hip.poisson_solve(desc, a.samples, b.samples, m.samples)
         except hip.HipCallError as e:
             cls = ErrorClass(e.error_class)
             raise PaintopError(cls, e.code, e.message) from None

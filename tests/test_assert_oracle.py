@@ -133,3 +133,22 @@ def test_shape_and_input_errors():  # tests.rs:378-391
     with pytest.raises(PaintopError) as e:
         op.compute(ins, {})
     assert e.value.code == "E_ASSERT_INPUT" and e.value.error_class == ErrorClass.REFERENCE
+
+
+def test_mask_components_are_four_connected_interior_cells():
+    """components.py: interior = mask > 0.5 off the image border
+    (poisson.rs:340-355); diagonal contact does not join components; each box
+    includes the one-cell Dirichlet ring."""
+    from paintop_amd import components as C
+
+    m = np.zeros((12, 12), np.float32)
+    m[2:5, 2:5] = 1.0
+    m[6:9, 6:9] = 1.0
+    m[5, 5] = 1.0          # touches both squares only diagonally
+    m[0, :] = 1.0          # the border row is never interior
+    m[9:11, 1] = 0.5       # not > 0.5
+    labels, comps = C.components(m)
+    assert [(c.y0, c.y1, c.x0, c.x1, c.cells) for c in comps] == [
+        (1, 6, 1, 6, 9), (4, 7, 4, 7, 1), (5, 10, 5, 10, 9)]
+    assert labels[0].max() == 0 and labels[5, 5] == 2
+    assert (labels > 0).sum() == 19

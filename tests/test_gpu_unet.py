@@ -466,6 +466,40 @@ def test_corrections_in_a_batched_call(gpu):
         assert float(np.abs(a - b).max()) < 2e-4
 
 
+def test_blend_split_into_mask_components(gpu):
+    """components.py: the blend solved per 4-connected mask component in one
+    batched call equals the whole-mask solve (both at 1e-9: within 1e-6), keeps
+    the outside-mask samples, and every component meets the tolerance on its
+    own (so the whole mask meets it: ||r||^2 = sum ||r_k||^2). Through the op
+    (PoissonBlend(split_components=True)) the report is the worst component's."""
+    from paintop_amd import components as C
+    from paintop_amd import ops
+    from paintop_amd.device import DeviceArray
+    from paintop_amd.inputs import spots_case
+    from paintop_amd.values import image, mask
+
+    n = 1024
+    src, tgt, m = spots_case(n, 12, 3, seed=602)
+    S, T = src.numpy(), tgt.numpy()
+    dm = DeviceArray.from_numpy(m)
+    out = DeviceArray(tgt.shape, np.float32)
+    gpu.poisson_solve_device(hip.make_desc(n, n, 3, 100000, 1e-9), src.ptr, tgt.ptr, dm.ptr,
+                             out.ptr)
+    cand, worst, per = C.solve_components(hip.make_desc(0, 0, 1, 100000, 1e-9), S, T, m)
+    assert len(per) == 12 and all(r.converged and r.final_residual <= 1e-9 for _c, r in per)
+    assert worst.iterations == max(r.iterations for _c, r in per)
+    inside = C.interior_cells(m)
+    assert cand[~inside].tobytes() == T[~inside].tobytes()
+    assert float(np.abs(cand - out.numpy()).max()) < 1e-6
+    op = ops.PoissonBlend(split_components=True)
+    res = op.compute({"source": image(n, n, S.reshape(-1), "Rgb"),
+                      "target": image(n, n, T.reshape(-1), "Rgb"), "mask": mask(n, n, m)},
+                     {"tolerance": 1e-6, "max_iterations": 20000})
+    solver = res["report"].as_report().solver
+    assert solver.converged and len(op.last_components) == 12
+    assert solver.iterations == max(r.iterations for _c, r in op.last_components)
+
+
 def test_start_field_race_keeps_the_better_start(gpu):
     """ph_poisson_solve_start_device races the caller's start against the
     anchor start: a start at the solution wins and ends within a few dozen
