@@ -92,18 +92,23 @@ __global__ void k_rowx_init(int *__restrict__ rowx, int h) {
   }
 }
 
-// Count interior cells of each tile's valid core (one block per tile).
-__global__ void k_tile_occupancy(const uint8_t *__restrict__ cls, int w, int h, int ox, int oy,
-                                 int K, int wv, int hb, const int2 *__restrict__ origins,
-                                 int *__restrict__ counts) {
+// Interior-cell count of each tile's valid core, for every tile origin of
+// every image of a call in one launch (one block per origin; origin_item
+// gives its image). Images of a batch have few tiles each, so one launch per
+// image left the GPU idle between launches.
+__global__ void k_tile_occupancy_all(const PhItemGeo *__restrict__ items,
+                                     const int *__restrict__ origin_item,
+                                     const int2 *__restrict__ origins, int K, int wv, int hb,
+                                     int *__restrict__ counts) {
+  const PhItemGeo G = items[origin_item[blockIdx.x]];
   const int2 o = origins[blockIdx.x];
   int n = 0;
   for (int r = 0; r < hb; ++r) {
-    const int iy = oy + o.y + K + r;
-    if (iy < 0 || iy >= h) continue;
+    const int iy = G.oy + o.y + K + r;
+    if (iy < 0 || iy >= G.h) continue;
     for (int cidx = threadIdx.x; cidx < wv; cidx += blockDim.x) {
-      const int ix = ox + o.x + K + cidx;
-      if (ix >= 0 && ix < w) n += cls[(size_t)iy * w + ix];
+      const int ix = G.ox + o.x + K + cidx;
+      if (ix >= 0 && ix < G.w) n += G.cls[(size_t)iy * G.w + ix];
     }
   }
   __shared__ int red[256];
@@ -116,23 +121,24 @@ __global__ void k_tile_occupancy(const uint8_t *__restrict__ cls, int w, int h, 
   if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
 }
 
-// Interior-cell count of each f x f block of the region starting at image
-// cell (x0, y0): the coarse mask the SOR-factor tuner works on.
-__global__ void k_coarse_count(const uint8_t *__restrict__ cls, int w, int h, int x0, int y0,
-                               int f, int cw, int ch, int *__restrict__ counts) {
+// Interior-cell count of each f x f block of an image's region (the coarse
+// mask the SOR-factor tuner works on), for every image of a call in one
+// launch (blockIdx.y = image; images without a coarse grid have cw * ch = 0).
+__global__ void k_coarse_count_all(const PhItemGeo *__restrict__ items, int *__restrict__ counts) {
+  const PhItemGeo G = items[blockIdx.y];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cw * ch) return;
-  const int cx = c % cw, cy = c / cw;
+  if (c >= G.cw * G.ch) return;
+  const int cx = c % G.cw, cy = c / G.cw;
   int n = 0;
-  for (int dy = 0; dy < f; ++dy) {
-    const int iy = y0 + cy * f + dy;
-    if (iy < 0 || iy >= h) continue;
-    for (int dx = 0; dx < f; ++dx) {
-      const int ix = x0 + cx * f + dx;
-      if (ix >= 0 && ix < w) n += cls[(size_t)iy * w + ix];
+  for (int dy = 0; dy < G.cf; ++dy) {
+    const int iy = G.ry0 + cy * G.cf + dy;
+    if (iy < 0 || iy >= G.h) continue;
+    for (int dx = 0; dx < G.cf; ++dx) {
+      const int ix = G.rx0 + cx * G.cf + dx;
+      if (ix >= 0 && ix < G.w) n += G.cls[(size_t)iy * G.w + ix];
     }
   }
-  counts[c] = n;
+  counts[G.coarse_off + c] = n;
 }
 
 // Fill an instance's planes from the interleaved image channel.

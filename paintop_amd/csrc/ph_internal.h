@@ -119,6 +119,17 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // normalisation, correction step).
 constexpr int PH_NET_SLICES = 16;
 
+// Per-image geometry for the set-up kernels that serve every image of a call
+// in one launch (tile occupancy, tuner coarse masks).
+struct PhItemGeo {
+  const uint8_t *cls;   // image classification, w x h
+  int32_t w, h;
+  int32_t ox, oy;       // plane origin in image coordinates
+  int32_t rx0, ry0;     // region origin (interior box plus ring)
+  int32_t cf, cw, ch;   // tuner coarse grid: factor and size (0 x 0: none)
+  int64_t coarse_off;   // offset of the image's coarse counts
+};
+
 struct PhTile {
   int32_t inst;
   int32_t px0; // first loaded plane column (valid columns start at px0 + K)

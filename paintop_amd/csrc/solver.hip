@@ -47,9 +47,9 @@ __global__ void k_sor_control(const PhInst *, PhCtl *, const double *, int, int)
 __global__ void k_classify(const float *, int, int, uint8_t *, int *);
 __global__ void k_bbox(const int *, int, int *);
 __global__ void k_rowx_init(int *, int);
-__global__ void k_tile_occupancy(const uint8_t *, int, int, int, int, int, int, int, const int2 *,
-                                 int *);
-__global__ void k_coarse_count(const uint8_t *, int, int, int, int, int, int, int, int *);
+__global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 *, int, int, int,
+                                     int *);
+__global__ void k_coarse_count_all(const PhItemGeo *, int *);
 __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
 __global__ void k_apply_correction(PhInst, int);
 __global__ void k_residual(PhInst, double *);
@@ -623,7 +623,8 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t ph_up = (size_t)H + 2 * PH_PAD + 4 + hb + 2 * K;
   const size_t plane_up = pw_up * ph_up;
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
-  return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) + ntile_up * (sizeof(int2) + sizeof(int)) +
+  return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) +
+         ntile_up * (sizeof(int2) + 2 * sizeof(int)) + sizeof(PhItemGeo) +
          plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
          (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
          (size_t)C * ph_up * sizeof(double) +
@@ -755,22 +756,46 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     int2 *origins_d = g_pool.take<int2>(all_origins.size());
     int *counts_d = g_pool.take<int>(all_origins.size());
     int *coarse_d = coarse_total ? g_pool.take<int>(coarse_total) : nullptr;
+    // one launch each for every image: per-image geometry and the image of
+    // every tile origin
+    std::vector<PhItemGeo> geo(nitem);
+    std::vector<int> origin_item(all_origins.size());
+    int max_coarse = 0;
+    for (int i = 0; i < nitem; ++i) {
+      const ItemWork &w = iw[i];
+      PhItemGeo &G = geo[i];
+      memset(&G, 0, sizeof(G));
+      G.cls = w.cls_img;
+      G.w = w.in.W;
+      G.h = w.in.H;
+      G.ox = w.ox;
+      G.oy = w.oy;
+      G.rx0 = w.rx0;
+      G.ry0 = w.ry0;
+      if (w.any && tuned) {
+        G.cf = w.cf;
+        G.cw = w.cw;
+        G.ch = w.ch;
+        G.coarse_off = (int64_t)w.coarse_off;
+        max_coarse = std::max(max_coarse, w.cw * w.ch);
+      }
+      if (w.any)
+        for (size_t k = 0; k < w.origins.size(); ++k) origin_item[w.origin_off + k] = i;
+    }
+    PhItemGeo *geo_d = g_pool.take<PhItemGeo>(nitem);
+    int *oitem_d = g_pool.take<int>(all_origins.size());
     HIP_OK(hipMemcpyAsync(origins_d, all_origins.data(), all_origins.size() * sizeof(int2),
                           hipMemcpyHostToDevice, st));
-    for (int i = 0; i < nitem; ++i) {
-      ItemWork &w = iw[i];
-      if (!w.any) continue;
-      hipLaunchKernelGGL(k_tile_occupancy, dim3(w.origins.size()), dim3(256), 0, st, w.cls_img,
-                         w.in.W, w.in.H, w.ox, w.oy, K, WV, hb, origins_d + w.origin_off,
-                         counts_d + w.origin_off);
-      HIP_OK(hipGetLastError());
-      if (tuned) {
-        hipLaunchKernelGGL(k_coarse_count, dim3((w.cw * w.ch + 255) / 256), dim3(256), 0, st,
-                           w.cls_img, w.in.W, w.in.H, w.rx0, w.ry0, w.cf, w.cw, w.ch,
-                           coarse_d + w.coarse_off);
-        HIP_OK(hipGetLastError());
-      }
-    }
+    HIP_OK(hipMemcpyAsync(geo_d, geo.data(), nitem * sizeof(PhItemGeo), hipMemcpyHostToDevice,
+                          st));
+    HIP_OK(hipMemcpyAsync(oitem_d, origin_item.data(), origin_item.size() * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_tile_occupancy_all, dim3(all_origins.size()), dim3(256), 0, st, geo_d,
+                       oitem_d, origins_d, K, WV, hb, counts_d);
+    if (tuned && max_coarse > 0)
+      hipLaunchKernelGGL(k_coarse_count_all, dim3((max_coarse + 255) / 256, nitem), dim3(256), 0,
+                         st, geo_d, coarse_d);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
                           hipMemcpyDeviceToHost, st));
     if (coarse_total)
