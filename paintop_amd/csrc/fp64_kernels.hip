@@ -175,18 +175,6 @@ __global__ void k_assemble(PhInst I, const float *__restrict__ src, const float 
   I.g[p] = __builtin_nanf("");
 }
 
-// x += e at interior cells, e = 0 (the FP64 correction step).
-__global__ void k_apply_correction(PhInst I, int buf) {
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y;
-  if (px >= I.pw) return;
-  const size_t p = (size_t)py * I.pitch + px;
-  if (I.cls[p]) {
-    I.x[p] += (double)I.e[buf][p];
-    I.e[buf][p] = 0.0f;
-  }
-}
-
 // g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and the per-row
 // sums of r^2 in a fixed order; k_residual_norm adds the rows in order and
 // stores ||r|| in out[0] and ||r||^2 in out[1]. (A last-block reduction inside
@@ -251,6 +239,92 @@ __global__ __launch_bounds__(256) void k_residual_norm(const double *__restrict_
   if (threadIdx.x == 0) {
     out[0] = sqrt(red[0]);
     out[1] = red[0];
+  }
+}
+
+// The refinement of a list of instances in three launches instead of three
+// per instance (a batch of many small images refines hundreds of instances a
+// pass). rows[j] = (instance slot in insts, plane row, correction buffer).
+// x += e, e = 0 at the interior cells of each listed row.
+__global__ __launch_bounds__(256) void k_apply_correction_rows(const PhInst *__restrict__ insts,
+                                                               const int4 *__restrict__ rows) {
+  const int4 r = rows[blockIdx.x];
+  const PhInst &I = insts[r.x];
+  const size_t row = (size_t)r.y * I.pitch;
+  float *e = I.e[r.z];
+  for (int px = threadIdx.x; px < I.pw; px += 256) {
+    const size_t p = row + px;
+    if (I.cls[p]) {
+      I.x[p] += (double)e[p];
+      e[p] = 0.0f;
+    }
+  }
+}
+
+// k_residual for every listed row (same arithmetic, same pairs, same row sum
+// order); row_sums[j] is the sum of row j of the list.
+__global__ __launch_bounds__(256) void k_residual_rows(const PhInst *__restrict__ insts,
+                                                       const int4 *__restrict__ rows,
+                                                       double *__restrict__ row_sums) {
+  const int4 rr = rows[blockIdx.x];
+  const PhInst &I = insts[rr.x];
+  const int py = rr.y;
+  double s = 0.0;
+  if (py >= 1 && py < I.ph - 1) {
+    const size_t row = (size_t)py * I.pitch;
+    for (int px = 2 * threadIdx.x; px < I.pw; px += 2 * blockDim.x) {
+      const size_t p = row + px;
+      const uchar2 c = *reinterpret_cast<const uchar2 *>(I.cls + p);
+      if (!(c.x | c.y)) continue;
+      const double2 xc = *reinterpret_cast<const double2 *>(I.x + p);
+      const double2 xu = *reinterpret_cast<const double2 *>(I.x + p - I.pitch);
+      const double2 xd = *reinterpret_cast<const double2 *>(I.x + p + I.pitch);
+      const double2 b = *reinterpret_cast<const double2 *>(I.bp + p);
+      const double xl = I.x[p - 1], xr = I.x[p + 2];
+      const double n0 = ((xl + xc.y) + xu.x) + xd.x;
+      const double n1 = ((xc.x + xr) + xu.y) + xd.y;
+      const double r0 = fma(I.dg64, -xc.x, n0) - b.x;
+      const double r1 = fma(I.dg64, -xc.y, n1) - b.y;
+      float2 gv;
+      gv.x = c.x ? (float)(-r0) : __builtin_nanf("");
+      gv.y = c.y ? (float)(-r1) : __builtin_nanf("");
+      *reinterpret_cast<float2 *>(I.g + p) = gv;
+      if (c.x) s = fma(r0, r0, s);
+      if (c.y) s = fma(r1, r1, s);
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) row_sums[blockIdx.x] = red[0];
+}
+
+// ||r|| of each listed instance from its rows (list[i] = (first row in
+// row_sums, row count, output slot)), added exactly as k_residual_norm adds
+// an instance's rows, so an instance's norm does not depend on the list.
+__global__ __launch_bounds__(256) void k_residual_norm_list(const int4 *__restrict__ list,
+                                                            const double *__restrict__ row_sums,
+                                                            double *__restrict__ out) {
+  const int4 L = list[blockIdx.x];
+  const double *rs = row_sums + L.x;
+  const int nrows = L.y;
+  __shared__ double red[256];
+  const int per = (nrows + 255) / 256;
+  double v = 0.0;
+  for (int r = threadIdx.x * per; r < min(nrows, (int)(threadIdx.x + 1) * per); ++r) v += rs[r];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * (size_t)L.z] = sqrt(red[0]);
+    out[2 * (size_t)L.z + 1] = red[0];
   }
 }
 

@@ -51,9 +51,11 @@ __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 
                                      int *);
 __global__ void k_coarse_count_all(const PhItemGeo *, int *);
 __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
-__global__ void k_apply_correction(PhInst, int);
 __global__ void k_residual(PhInst, double *);
 __global__ void k_residual_norm(const double *, int, double *);
+__global__ void k_apply_correction_rows(const PhInst *, const int4 *);
+__global__ void k_residual_rows(const PhInst *, const int4 *, double *);
+__global__ void k_residual_norm_list(const int4 *, const double *, double *);
 __global__ void k_scatter(PhInst, float *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
                           int, double, int, int, int, int, int, float *, float *, float *);
@@ -625,6 +627,7 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
   return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) +
          ntile_up * (sizeof(int2) + 2 * sizeof(int)) + sizeof(PhItemGeo) +
+         (size_t)C * 2 * ph_up * (sizeof(int4) + sizeof(double)) + (size_t)C * 2 * sizeof(int4) +
          plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
          (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
          (size_t)C * ph_up * sizeof(double) +
@@ -1121,6 +1124,14 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     PhInst *inst_d = g_pool.take<PhInst>(nact);
     PhCtl *ctl_d = g_pool.take<PhCtl>(nact);
     double *partials_d = g_pool.take<double>(ntile_cap * K);
+    // refinement passes: one row table and one instance list for all pending
+    // instances (three launches a pass)
+    size_t row_cap = 0;
+    for (int a = 0; a < nact; ++a) row_cap += (size_t)inst[act[a]].I.ph;
+    int4 *ref_rows_d = g_pool.take<int4>(row_cap);
+    double *ref_sums_d = g_pool.take<double>(row_cap);
+    int4 *ref_list_d = g_pool.take<int4>(nact);
+    std::vector<int4> ref_rows, ref_list;
     std::vector<PhTile> tl;
     int ntiles = 0;
     auto build_tiles = [&](const std::vector<char> &running) {
@@ -1427,13 +1438,26 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         }
       }
       if (!pend.empty()) {
+        // x += e, e = 0 and the FP64 residual of every pending instance
+        ref_rows.clear();
+        ref_list.clear();
         for (const Pending &P : pend) {
-          const int k = act[P.a];
-          hipLaunchKernelGGL(k_apply_correction, inst[k].pgrd, dim3(256), 0, st, inst[k].I,
-                             (int)ctl[P.a].cur_buf);
-          HIP_OK(hipGetLastError());
-          launch_residual(k);
+          const PhInst &I = ai[P.a];
+          ref_list.push_back(make_int4((int)ref_rows.size(), I.ph, act[P.a], 0));
+          for (int py = 0; py < I.ph; ++py)
+            ref_rows.push_back(make_int4(P.a, py, (int)ctl[P.a].cur_buf, 0));
         }
+        HIP_OK(hipMemcpyAsync(ref_rows_d, ref_rows.data(), ref_rows.size() * sizeof(int4),
+                              hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(ref_list_d, ref_list.data(), ref_list.size() * sizeof(int4),
+                              hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_apply_correction_rows, dim3(ref_rows.size()), dim3(256), 0, st,
+                           inst_d, ref_rows_d);
+        hipLaunchKernelGGL(k_residual_rows, dim3(ref_rows.size()), dim3(256), 0, st, inst_d,
+                           ref_rows_d, ref_sums_d);
+        hipLaunchKernelGGL(k_residual_norm_list, dim3(ref_list.size()), dim3(256), 0, st,
+                           ref_list_d, ref_sums_d, resv_d);
+        HIP_OK(hipGetLastError());
         // every instance's slot, the warm-start race's plain twins included
         HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
                               hipMemcpyDeviceToHost, st));
