@@ -17,78 +17,89 @@ namespace ph {
 
 constexpr float MASK_THRESHOLD = 0.5f;  // poisson.rs:329
 
-// Classify every image pixel (one block per 256 pixels of a row). Each block
-// folds its interior columns into the row's [min, max] (few atomics per
-// row); k_bbox then reduces the rows.
-__global__ __launch_bounds__(256) void k_classify(const float *__restrict__ mask, int w, int h,
-                                                  uint8_t *__restrict__ cls,
-                                                  int *__restrict__ rowx) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  uint8_t in = 0;
-  if (x < w) {
-    const size_t idx = (size_t)y * w + x;
-    if (w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 && y < h - 1)
-      in = mask[idx] > MASK_THRESHOLD ? 1 : 0;
-    cls[idx] = in;
-  }
-  int xmin = in ? x : 0x7fffffff, xmax = in ? x : -1;
+// Block-wide min and max of one int pair per thread (256 threads); the
+// result is valid in thread 0.
+__device__ __forceinline__ int2 block_minmax(int lo, int hi) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
-    xmin = min(xmin, __shfl_xor(xmin, off, 64));
-    xmax = max(xmax, __shfl_xor(xmax, off, 64));
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
   }
-  __shared__ int smin[4], smax[4];
+  __shared__ int slo[4], shi[4];
   if ((threadIdx.x & 63) == 0) {
-    smin[threadIdx.x >> 6] = xmin;
-    smax[threadIdx.x >> 6] = xmax;
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    xmin = min(min(smin[0], smin[1]), min(smin[2], smin[3]));
-    xmax = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
-    if (xmax >= 0) {
-      atomicMin(&rowx[2 * y], xmin);
-      atomicMax(&rowx[2 * y + 1], xmax);
-    }
-  }
+  return make_int2(min(min(slo[0], slo[1]), min(slo[2], slo[3])),
+                   max(max(shi[0], shi[1]), max(shi[2], shi[3])));
 }
 
-// Bounding box of the interior cells from the per-row [min, max] (one block).
-__global__ __launch_bounds__(256) void k_bbox(const int *__restrict__ rowx, int h,
-                                              int *__restrict__ bbox) {
+// Set-up of every image of a call, one block per image row (spans: slot =
+// image, nrows = H). The candidate row starts as the target row, so every
+// non-interior sample keeps its bits; the row's pixels are classified; and
+// the row's interior column range goes to rowx[b] ((INT_MAX, -1) when the
+// row has none).
+__global__ __launch_bounds__(256) void k_classify_spans(const PhSpan *__restrict__ spans, int n,
+                                                        const PhItemIO *__restrict__ io,
+                                                        int2 *__restrict__ rowx) {
+  const int b = blockIdx.x;
+  const PhSpan S = spans[ph_span_find(spans, n, b)];
+  const PhItemIO &G = io[S.slot];
+  const int y = b - S.row0;
+  const int w = G.W, h = G.H;
+  const size_t rl = (size_t)w * G.C;
+  const float *t = G.tgt + (size_t)y * rl;
+  float *o = G.out + (size_t)y * rl;
+  if (((reinterpret_cast<uintptr_t>(G.tgt) | reinterpret_cast<uintptr_t>(G.out)) & 15) == 0 &&
+      (rl & 3) == 0) {
+    for (size_t q = threadIdx.x; q < rl / 4; q += 256)
+      reinterpret_cast<float4 *>(o)[q] = reinterpret_cast<const float4 *>(t)[q];
+  } else {
+    for (size_t q = threadIdx.x; q < rl; q += 256) o[q] = t[q];
+  }
+  const bool row_in = w >= 3 && h >= 3 && y >= 1 && y < h - 1;
+  int xmin = 0x7fffffff, xmax = -1;
+  for (int x = threadIdx.x; x < w; x += 256) {
+    const size_t idx = (size_t)y * w + x;
+    uint8_t in = 0;
+    if (row_in && x >= 1 && x < w - 1) in = G.mask[idx] > MASK_THRESHOLD ? 1 : 0;
+    G.cls_img[idx] = in;
+    if (in) {
+      xmin = min(xmin, x);
+      xmax = max(xmax, x);
+    }
+  }
+  const int2 r = block_minmax(xmin, xmax);
+  if (threadIdx.x == 0) rowx[b] = r;
+}
+
+// Bounding box of each image's interior cells from its rows' column ranges
+// (one block per image): bbox[4 * image + {0, 1, 2, 3}] = xmin, ymin, xmax,
+// ymax (xmax = -1: no interior cell).
+__global__ __launch_bounds__(256) void k_bbox_spans(const PhSpan *__restrict__ spans,
+                                                    const int2 *__restrict__ rowx,
+                                                    int *__restrict__ bbox) {
+  const PhSpan S = spans[blockIdx.x];
   int xmin = 0x7fffffff, xmax = -1, ymin = 0x7fffffff, ymax = -1;
-  for (int y = threadIdx.x; y < h; y += blockDim.x)
-    if (rowx[2 * y + 1] >= 0) {
-      xmin = min(xmin, rowx[2 * y]);
-      xmax = max(xmax, rowx[2 * y + 1]);
+  for (int y = threadIdx.x; y < S.nrows; y += 256) {
+    const int2 r = rowx[S.row0 + y];
+    if (r.y >= 0) {
+      xmin = min(xmin, r.x);
+      xmax = max(xmax, r.y);
       ymin = min(ymin, y);
       ymax = max(ymax, y);
     }
-  __shared__ int s[4][256];
-  s[0][threadIdx.x] = xmin;
-  s[1][threadIdx.x] = ymin;
-  s[2][threadIdx.x] = xmax;
-  s[3][threadIdx.x] = ymax;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) {
-      s[0][threadIdx.x] = min(s[0][threadIdx.x], s[0][threadIdx.x + k]);
-      s[1][threadIdx.x] = min(s[1][threadIdx.x], s[1][threadIdx.x + k]);
-      s[2][threadIdx.x] = max(s[2][threadIdx.x], s[2][threadIdx.x + k]);
-      s[3][threadIdx.x] = max(s[3][threadIdx.x], s[3][threadIdx.x + k]);
-    }
-    __syncthreads();
   }
-  if (threadIdx.x < 4) bbox[threadIdx.x] = s[threadIdx.x][0];
-}
-
-// Per-row [min, max] start empty.
-__global__ void k_rowx_init(int *__restrict__ rowx, int h) {
-  const int y = blockIdx.x * blockDim.x + threadIdx.x;
-  if (y < h) {
-    rowx[2 * y] = 0x7fffffff;
-    rowx[2 * y + 1] = -1;
+  const int2 bx = block_minmax(xmin, xmax);
+  __syncthreads();  // block_minmax's LDS is reused
+  const int2 by = block_minmax(ymin, ymax);
+  if (threadIdx.x == 0) {
+    int *o = bbox + 4 * (size_t)S.slot;
+    o[0] = bx.x;
+    o[1] = by.x;
+    o[2] = bx.y;
+    o[3] = by.y;
   }
 }
 
@@ -141,12 +152,11 @@ __global__ void k_coarse_count_all(const PhItemGeo *__restrict__ items, int *__r
   counts[G.coarse_off + c] = n;
 }
 
-// Fill an instance's planes from the interleaved image channel.
-__global__ void k_assemble(PhInst I, const float *__restrict__ src, const float *__restrict__ tgt,
-                           const uint8_t *__restrict__ cls_img, uint8_t *__restrict__ cls) {
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y;
-  if (px >= I.pitch) return;
+// Fill plane cell (px, py) of an instance from the interleaved image channel.
+__device__ __forceinline__ void assemble_cell(const PhInst &I, const float *__restrict__ src,
+                                              const float *__restrict__ tgt,
+                                              const uint8_t *__restrict__ cls_img,
+                                              uint8_t *__restrict__ cls, int px, int py) {
   const size_t p = (size_t)py * I.pitch + px;
   const int ix = I.ox + px, iy = I.oy + py;
   const int W = I.img_w, H = I.img_h, C = I.channels, ch = I.channel;
@@ -175,6 +185,26 @@ __global__ void k_assemble(PhInst I, const float *__restrict__ src, const float 
   I.g[p] = __builtin_nanf("");
 }
 
+// Fill an instance's planes (grid: pitch / 256 x plane rows).
+__global__ void k_assemble(PhInst I, const float *__restrict__ src, const float *__restrict__ tgt,
+                           const uint8_t *__restrict__ cls_img, uint8_t *__restrict__ cls) {
+  const int px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px < I.pitch) assemble_cell(I, src, tgt, cls_img, cls, px, blockIdx.y);
+}
+
+// Fill the planes of every listed instance, one block per plane row.
+__global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict__ insts,
+                                                        const PhSpan *__restrict__ spans, int n,
+                                                        const PhItemIO *__restrict__ io) {
+  const int b = blockIdx.x;
+  const PhSpan S = spans[ph_span_find(spans, n, b)];
+  const PhInst &I = insts[S.slot];
+  const PhItemIO &G = io[S.item];
+  const int py = b - S.row0;
+  for (int px = threadIdx.x; px < I.pitch; px += 256)
+    assemble_cell(I, G.src, G.tgt, G.cls_img, G.cls_plane, px, py);
+}
+
 // g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and the per-row
 // sums of r^2 in a fixed order; k_residual_norm adds the rows in order and
 // stores ||r|| in out[0] and ||r||^2 in out[1]. (A last-block reduction inside
@@ -185,8 +215,7 @@ __global__ void k_assemble(PhInst I, const float *__restrict__ src, const float 
 // and columns 0 and pw - 1 are padding (no interior cells), so every
 // neighbour read stays inside the plane. Non-interior cells of a pair get
 // g = NaN, the value they hold already.
-__global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__ row_sums) {
-  const int py = blockIdx.x;
+__device__ __forceinline__ double residual_row(const PhInst &I, int py) {
   double s = 0.0;
   if (py >= 1 && py < I.ph - 1) {
     const size_t row = (size_t)py * I.pitch;
@@ -211,47 +240,57 @@ __global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__
       if (c.y) s = fma(r1, r1, s);
     }
   }
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) row_sums[py] = red[0];
+  return s;
 }
 
-// ||r|| from the row sums of k_residual, added in a fixed order: thread t
-// takes a contiguous run of rows, then a tree over the threads. One block.
-__global__ __launch_bounds__(256) void k_residual_norm(const double *__restrict__ row_sums,
-                                                       int nrows, double *__restrict__ out) {
+// Tree sum over the 256 threads of a block, in a fixed order (thread 0).
+__device__ __forceinline__ double block_sum(double v) {
   __shared__ double red[256];
-  const int per = (nrows + 255) / 256;
-  double v = 0.0;
-  for (int r = threadIdx.x * per; r < min(nrows, (int)(threadIdx.x + 1) * per); ++r)
-    v += row_sums[r];
   red[threadIdx.x] = v;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
+  return red[0];
+}
+
+// Fixed-order sum of nrows row sums (thread 0): thread t takes a contiguous
+// run of rows, then the tree over the threads.
+__device__ __forceinline__ double rows_sum(const double *__restrict__ rs, int nrows) {
+  const int per = (nrows + 255) / 256;
+  double v = 0.0;
+  for (int r = threadIdx.x * per; r < min(nrows, (int)(threadIdx.x + 1) * per); ++r) v += rs[r];
+  return block_sum(v);
+}
+
+__global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__ row_sums) {
+  const double s = block_sum(residual_row(I, blockIdx.x));
+  if (threadIdx.x == 0) row_sums[blockIdx.x] = s;
+}
+
+// ||r|| from the row sums of k_residual, added in a fixed order; out[0] =
+// ||r||, out[1] = ||r||^2. One block.
+__global__ __launch_bounds__(256) void k_residual_norm(const double *__restrict__ row_sums,
+                                                       int nrows, double *__restrict__ out) {
+  const double s = rows_sum(row_sums, nrows);
   if (threadIdx.x == 0) {
-    out[0] = sqrt(red[0]);
-    out[1] = red[0];
+    out[0] = sqrt(s);
+    out[1] = s;
   }
 }
 
-// The refinement of a list of instances in three launches instead of three
-// per instance (a batch of many small images refines hundreds of instances a
-// pass). rows[j] = (instance slot in insts, plane row, correction buffer).
-// x += e, e = 0 at the interior cells of each listed row.
-__global__ __launch_bounds__(256) void k_apply_correction_rows(const PhInst *__restrict__ insts,
-                                                               const int4 *__restrict__ rows) {
-  const int4 r = rows[blockIdx.x];
-  const PhInst &I = insts[r.x];
-  const size_t row = (size_t)r.y * I.pitch;
-  float *e = I.e[r.z];
+// The refinement of many instances in three launches instead of three per
+// instance (a batch of small images refines hundreds of instances a pass).
+// x += e[buf], e[buf] = 0 at the interior cells of every listed row.
+__global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__restrict__ insts,
+                                                                const PhSpan *__restrict__ spans,
+                                                                int n) {
+  const int b = blockIdx.x;
+  const PhSpan S = spans[ph_span_find(spans, n, b)];
+  const PhInst &I = insts[S.slot];
+  const size_t row = (size_t)(b - S.row0) * I.pitch;
+  float *e = I.e[S.buf];
   for (int px = threadIdx.x; px < I.pw; px += 256) {
     const size_t p = row + px;
     if (I.cls[p]) {
@@ -261,71 +300,59 @@ __global__ __launch_bounds__(256) void k_apply_correction_rows(const PhInst *__r
   }
 }
 
-// k_residual for every listed row (same arithmetic, same pairs, same row sum
-// order); row_sums[j] is the sum of row j of the list.
-__global__ __launch_bounds__(256) void k_residual_rows(const PhInst *__restrict__ insts,
-                                                       const int4 *__restrict__ rows,
-                                                       double *__restrict__ row_sums) {
-  const int4 rr = rows[blockIdx.x];
-  const PhInst &I = insts[rr.x];
-  const int py = rr.y;
-  double s = 0.0;
-  if (py >= 1 && py < I.ph - 1) {
-    const size_t row = (size_t)py * I.pitch;
-    for (int px = 2 * threadIdx.x; px < I.pw; px += 2 * blockDim.x) {
-      const size_t p = row + px;
-      const uchar2 c = *reinterpret_cast<const uchar2 *>(I.cls + p);
-      if (!(c.x | c.y)) continue;
-      const double2 xc = *reinterpret_cast<const double2 *>(I.x + p);
-      const double2 xu = *reinterpret_cast<const double2 *>(I.x + p - I.pitch);
-      const double2 xd = *reinterpret_cast<const double2 *>(I.x + p + I.pitch);
-      const double2 b = *reinterpret_cast<const double2 *>(I.bp + p);
-      const double xl = I.x[p - 1], xr = I.x[p + 2];
-      const double n0 = ((xl + xc.y) + xu.x) + xd.x;
-      const double n1 = ((xc.x + xr) + xu.y) + xd.y;
-      const double r0 = fma(I.dg64, -xc.x, n0) - b.x;
-      const double r1 = fma(I.dg64, -xc.y, n1) - b.y;
-      float2 gv;
-      gv.x = c.x ? (float)(-r0) : __builtin_nanf("");
-      gv.y = c.y ? (float)(-r1) : __builtin_nanf("");
-      *reinterpret_cast<float2 *>(I.g + p) = gv;
-      if (c.x) s = fma(r0, r0, s);
-      if (c.y) s = fma(r1, r1, s);
-    }
-  }
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) row_sums[blockIdx.x] = red[0];
+// k_residual for every listed row (the same arithmetic and order);
+// row_sums[b] is the sum of block b's row.
+__global__ __launch_bounds__(256) void k_residual_spans(const PhInst *__restrict__ insts,
+                                                        const PhSpan *__restrict__ spans, int n,
+                                                        double *__restrict__ row_sums) {
+  const int b = blockIdx.x;
+  const PhSpan S = spans[ph_span_find(spans, n, b)];
+  const double s = block_sum(residual_row(insts[S.slot], b - S.row0));
+  if (threadIdx.x == 0) row_sums[b] = s;
 }
 
-// ||r|| of each listed instance from its rows (list[i] = (first row in
-// row_sums, row count, output slot)), added exactly as k_residual_norm adds
-// an instance's rows, so an instance's norm does not depend on the list.
-__global__ __launch_bounds__(256) void k_residual_norm_list(const int4 *__restrict__ list,
-                                                            const double *__restrict__ row_sums,
-                                                            double *__restrict__ out) {
-  const int4 L = list[blockIdx.x];
-  const double *rs = row_sums + L.x;
-  const int nrows = L.y;
-  __shared__ double red[256];
-  const int per = (nrows + 255) / 256;
-  double v = 0.0;
-  for (int r = threadIdx.x * per; r < min(nrows, (int)(threadIdx.x + 1) * per); ++r) v += rs[r];
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
+// ||r|| of each listed instance (one block per entry), added exactly as
+// k_residual_norm adds an instance's rows, so the norm does not depend on the
+// table: out[2 * out_slot] = ||r||, out[2 * out_slot + 1] = ||r||^2, and the
+// history entry, when given, receives ||r|| / r0.
+__global__ __launch_bounds__(256) void k_residual_norm_spans(const PhSpan *__restrict__ spans,
+                                                             const double *__restrict__ row_sums,
+                                                             double *__restrict__ out) {
+  const PhSpan S = spans[blockIdx.x];
+  const double s = rows_sum(row_sums + S.row0, S.nrows);
   if (threadIdx.x == 0) {
-    out[2 * (size_t)L.z] = sqrt(red[0]);
-    out[2 * (size_t)L.z + 1] = red[0];
+    const double r = sqrt(s);
+    out[2 * (size_t)S.out] = r;
+    out[2 * (size_t)S.out + 1] = s;
+    if (S.hist) *S.hist = r / S.r0;
   }
+}
+
+// Write the solved interior cells of every listed instance into its image's
+// candidate, one block per plane row.
+__global__ __launch_bounds__(256) void k_scatter_spans(const PhInst *__restrict__ insts,
+                                                       const PhSpan *__restrict__ spans, int n,
+                                                       const PhItemIO *__restrict__ io) {
+  const int b = blockIdx.x;
+  const PhSpan S = spans[ph_span_find(spans, n, b)];
+  const PhInst &I = insts[S.slot];
+  float *out = io[S.item].out;
+  const int py = b - S.row0;
+  for (int px = threadIdx.x; px < I.pw; px += 256) {
+    const size_t p = (size_t)py * I.pitch + px;
+    if (I.cls[p]) {
+      const size_t q = (size_t)(I.oy + py) * I.img_w + (I.ox + px);
+      out[q * I.channels + I.channel] = (float)I.x[p];
+    }
+  }
+}
+
+// Gather residual histories into one array for a single readback (one block
+// per history).
+__global__ __launch_bounds__(256) void k_gather_hist(const PhHistCopy *__restrict__ c,
+                                                     double *__restrict__ dst) {
+  const PhHistCopy C = c[blockIdx.x];
+  for (int q = threadIdx.x; q < C.n; q += 256) dst[C.off + q] = C.src[q];
 }
 
 // Write the solved interior cells of an instance into the interleaved

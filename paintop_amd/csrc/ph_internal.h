@@ -130,6 +130,47 @@ struct PhItemGeo {
   int64_t coarse_off;   // offset of the image's coarse counts
 };
 
+// One entry of a row-batched launch (many instances or images in one grid,
+// one block per row): block b with row0 <= b < row0 + nrows works on row
+// b - row0 of entry `slot`. row0 is the running sum of nrows over the table.
+struct PhSpan {
+  int32_t slot;   // instance (or image) index in the launch's descriptor array
+  int32_t row0;   // first block of the entry; also its first row-sum index
+  int32_t nrows;
+  int32_t buf;    // correction buffer (x += e[buf])
+  int32_t out;    // slot of the entry's norm in the output array
+  int32_t item;   // image index (assembly, write-back)
+  double *hist;   // when set: receives ||r|| / r0, the residual-history entry
+  double r0;
+};
+
+// Per-image buffers of the row-batched set-up and write-back kernels.
+struct PhItemIO {
+  const float *src, *tgt, *mask;
+  float *out;
+  uint8_t *cls_img;   // image classification, W x H
+  uint8_t *cls_plane; // plane classification (shared by the image's channels)
+  int32_t W, H, C, pad;
+};
+
+// One residual history to gather for the readback.
+struct PhHistCopy {
+  const double *src;
+  int64_t off;  // first element in the gathered array
+  int32_t n, pad;
+};
+
+// Entry of a span table that holds block b: the last entry with row0 <= b.
+__device__ __forceinline__ int ph_span_find(const PhSpan *__restrict__ s, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s[mid].row0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 struct PhTile {
   int32_t inst;
   int32_t px0; // first loaded plane column (valid columns start at px0 + K)

@@ -44,19 +44,21 @@ template <int K, int PF>
 __global__ void k_sor(const PhTile *, int, const PhInst *, const PhCtl *, double *, int);
 template <int K>
 __global__ void k_sor_control(const PhInst *, PhCtl *, const double *, int, int);
-__global__ void k_classify(const float *, int, int, uint8_t *, int *);
-__global__ void k_bbox(const int *, int, int *);
-__global__ void k_rowx_init(int *, int);
+__global__ void k_classify_spans(const PhSpan *, int, const PhItemIO *, int2 *);
+__global__ void k_bbox_spans(const PhSpan *, const int2 *, int *);
 __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 *, int, int, int,
                                      int *);
 __global__ void k_coarse_count_all(const PhItemGeo *, int *);
 __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
 __global__ void k_residual(PhInst, double *);
 __global__ void k_residual_norm(const double *, int, double *);
-__global__ void k_apply_correction_rows(const PhInst *, const int4 *);
-__global__ void k_residual_rows(const PhInst *, const int4 *, double *);
-__global__ void k_residual_norm_list(const int4 *, const double *, double *);
+__global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
+__global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
+__global__ void k_residual_spans(const PhInst *, const PhSpan *, int, double *);
+__global__ void k_residual_norm_spans(const PhSpan *, const double *, double *);
 __global__ void k_scatter(PhInst, float *);
+__global__ void k_scatter_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
+__global__ void k_gather_hist(const PhHistCopy *, double *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
                           int, double, int, int, int, int, int, float *, float *, float *);
 __global__ void k_warm_apply(PhInst, const float *, const float *, const float *, int, int, int,
@@ -618,6 +620,27 @@ static void tune_store(const TuneKey &key, double mu) {
 }
 
 // Upper bound of one image's workspace, so the pool never regrows mid-call.
+// Host side of a row-batched launch: entries with the running row count.
+struct SpanTable {
+  std::vector<PhSpan> h;
+  int rows = 0;
+  void clear() {
+    h.clear();
+    rows = 0;
+  }
+  void add(int slot, int nrows, int item, int buf = 0, int out = 0, double *hist = nullptr,
+           double r0 = 0.0) {
+    h.push_back(PhSpan{slot, rows, nrows, buf, out, item, hist, r0});
+    rows += nrows;
+  }
+  // the host vector is pageable staging: keep it unchanged until the stream
+  // has been synchronised
+  void upload(PhSpan *d, hipStream_t st) const {
+    if (!h.empty())
+      HIP_OK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(PhSpan), hipMemcpyHostToDevice, st));
+  }
+};
+
 static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int hb) {
   const int WV = 256 - 2 * K;
   const size_t npx = (size_t)W * H;
@@ -627,7 +650,8 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
   return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) +
          ntile_up * (sizeof(int2) + 2 * sizeof(int)) + sizeof(PhItemGeo) +
-         (size_t)C * 2 * ph_up * (sizeof(int4) + sizeof(double)) + (size_t)C * 2 * sizeof(int4) +
+         (size_t)C * 3 * ph_up * sizeof(double) + (size_t)(4 * C + 2) * sizeof(PhSpan) +
+         sizeof(PhItemIO) + sizeof(PhHistCopy) + (size_t)(d->max_iterations + 2) * sizeof(double) +
          plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
          (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
          (size_t)C * ph_up * sizeof(double) +
@@ -666,7 +690,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     HIP_OK(hipEventRecord(ev0, st));
   }
   {
-    size_t need = 64 * 1024 + (size_t)nitem * 64;
+    size_t need = 128 * 1024 + (size_t)nitem * 64;
     for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
     // the plain twins of the warm-start race
     if (race_any)
@@ -698,26 +722,41 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     rep->omega_used = d->omega;
   }
 
-  // --- 1. classification and bounding boxes (one readback for all images)
+  // --- 1. classification and bounding boxes of every image: one launch per
+  // step over all image rows, one readback
   int *bbox_d = g_pool.take<int>(4 * (size_t)nitem);
   std::vector<int> bbox_h(4 * (size_t)nitem, -1);
+  std::vector<PhItemIO> io(nitem);
+  SpanTable img_rows;
   for (int i = 0; i < nitem; ++i) {
     const ItemIn &it = iw[i].in;
-    const size_t npx = (size_t)it.W * it.H;
-    if (npx == 0) continue;
-    // the candidate starts as the target: every non-interior sample keeps its bits
-    HIP_OK(hipMemcpyAsync(it.out, it.tgt, npx * it.C * sizeof(float), hipMemcpyDeviceToDevice,
-                          st));
-    iw[i].cls_img = g_pool.take<uint8_t>(npx);
-    int *rowx = g_pool.take<int>(2 * (size_t)it.H);
-    hipLaunchKernelGGL(k_rowx_init, dim3((it.H + 255) / 256), dim3(256), 0, st, rowx, it.H);
-    hipLaunchKernelGGL(k_classify, dim3((it.W + 255) / 256, it.H), dim3(256), 0, st, it.mask,
-                       it.W, it.H, iw[i].cls_img, rowx);
-    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(256), 0, st, rowx, it.H, bbox_d + 4 * i);
-    HIP_OK(hipGetLastError());
+    PhItemIO &G = io[i];
+    memset(&G, 0, sizeof(G));
+    G.src = it.src;
+    G.tgt = it.tgt;
+    G.mask = it.mask;
+    G.out = it.out;
+    G.W = it.W;
+    G.H = it.H;
+    G.C = it.C;
+    if ((size_t)it.W * it.H == 0) continue;
+    iw[i].cls_img = G.cls_img = g_pool.take<uint8_t>((size_t)it.W * it.H);
+    img_rows.add(i, it.H, i);
   }
-  HIP_OK(hipMemcpyAsync(bbox_h.data(), bbox_d, bbox_h.size() * sizeof(int), hipMemcpyDeviceToHost,
-                        st));
+  PhItemIO *io_d = g_pool.take<PhItemIO>(nitem);
+  PhSpan *img_rows_d = g_pool.take<PhSpan>(nitem);
+  if (img_rows.rows > 0) {
+    int2 *rowx_d = g_pool.take<int2>((size_t)img_rows.rows);
+    HIP_OK(hipMemcpyAsync(io_d, io.data(), nitem * sizeof(PhItemIO), hipMemcpyHostToDevice, st));
+    img_rows.upload(img_rows_d, st);
+    hipLaunchKernelGGL(k_classify_spans, dim3(img_rows.rows), dim3(256), 0, st, img_rows_d,
+                       (int)img_rows.h.size(), io_d, rowx_d);
+    hipLaunchKernelGGL(k_bbox_spans, dim3(img_rows.h.size()), dim3(256), 0, st, img_rows_d, rowx_d,
+                       bbox_d);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(bbox_h.data(), bbox_d, bbox_h.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+  }
   HIP_OK(hipStreamSynchronize(st));
   for (int i = 0; i < nitem; ++i)
     if ((size_t)iw[i].in.W * iw[i].in.H == 0) bbox_h[4 * i + 2] = -1;
@@ -904,9 +943,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       X.pgrd = dim3((w.pitch + 255) / 256, w.phh);
       X.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
       X.rowsum = g_pool.take<double>(w.phh);
-      hipLaunchKernelGGL(k_assemble, X.pgrd, dim3(256), 0, st, I, w.in.src, w.in.tgt, w.cls_img,
-                         w.cls_plane);
-      HIP_OK(hipGetLastError());
       inst.push_back(X);
     }
   }
@@ -921,8 +957,30 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                        resv_d + 2 * k);
     HIP_OK(hipGetLastError());
   };
-  for (int k = 0; k < ninst; ++k) launch_residual(k);
+  // assembly and the anchor-start residual of every instance: one launch per
+  // step over all plane rows (inst_all_d is the instances' descriptors)
+  PhInst *inst_all_d = g_pool.take<PhInst>(std::max(ninst, 1));
+  PhSpan *inst_rows_d = g_pool.take<PhSpan>(std::max(ninst, 1));
+  SpanTable inst_rows;
+  std::vector<PhInst> inst_all(ninst);
+  for (int k = 0; k < ninst; ++k) {
+    inst_all[k] = inst[k].I;
+    inst_rows.add(k, inst[k].I.ph, inst[k].item, 0, k);
+  }
   if (ninst) {
+    for (int i = 0; i < nitem; ++i) io[i].cls_plane = iw[i].cls_plane;
+    double *inst_sums_d = g_pool.take<double>((size_t)inst_rows.rows);
+    HIP_OK(hipMemcpyAsync(io_d, io.data(), nitem * sizeof(PhItemIO), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(inst_all_d, inst_all.data(), ninst * sizeof(PhInst),
+                          hipMemcpyHostToDevice, st));
+    inst_rows.upload(inst_rows_d, st);
+    hipLaunchKernelGGL(k_assemble_spans, dim3(inst_rows.rows), dim3(256), 0, st, inst_all_d,
+                       inst_rows_d, ninst, io_d);
+    hipLaunchKernelGGL(k_residual_spans, dim3(inst_rows.rows), dim3(256), 0, st, inst_all_d,
+                       inst_rows_d, ninst, inst_sums_d);
+    hipLaunchKernelGGL(k_residual_norm_spans, dim3(ninst), dim3(256), 0, st, inst_rows_d,
+                       inst_sums_d, resv_d);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1124,14 +1182,13 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     PhInst *inst_d = g_pool.take<PhInst>(nact);
     PhCtl *ctl_d = g_pool.take<PhCtl>(nact);
     double *partials_d = g_pool.take<double>(ntile_cap * K);
-    // refinement passes: one row table and one instance list for all pending
-    // instances (three launches a pass)
+    // refinement passes: one span table for all pending instances (three
+    // launches a pass)
     size_t row_cap = 0;
     for (int a = 0; a < nact; ++a) row_cap += (size_t)inst[act[a]].I.ph;
-    int4 *ref_rows_d = g_pool.take<int4>(row_cap);
     double *ref_sums_d = g_pool.take<double>(row_cap);
-    int4 *ref_list_d = g_pool.take<int4>(nact);
-    std::vector<int4> ref_rows, ref_list;
+    PhSpan *ref_spans_d = g_pool.take<PhSpan>(nact);
+    SpanTable ref;
     std::vector<PhTile> tl;
     int ntiles = 0;
     auto build_tiles = [&](const std::vector<char> &running) {
@@ -1233,7 +1290,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       int reason;
     };
     std::vector<Pending> pend;
-    std::vector<double> hist_stage;
     struct Trend {
       uint32_t m = 0;
       double rel = 0.0;
@@ -1438,25 +1494,21 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         }
       }
       if (!pend.empty()) {
-        // x += e, e = 0 and the FP64 residual of every pending instance
-        ref_rows.clear();
-        ref_list.clear();
+        // x += e, e = 0 and the FP64 residual of every pending instance; the
+        // norm kernel also writes each one's history entry ||r|| / r0 of sweep m
+        ref.clear();
         for (const Pending &P : pend) {
-          const PhInst &I = ai[P.a];
-          ref_list.push_back(make_int4((int)ref_rows.size(), I.ph, act[P.a], 0));
-          for (int py = 0; py < I.ph; ++py)
-            ref_rows.push_back(make_int4(P.a, py, (int)ctl[P.a].cur_buf, 0));
+          const PhCtl &Cc = ctl[P.a];
+          ref.add(P.a, ai[P.a].ph, 0, (int)Cc.cur_buf, act[P.a], Cc.history + (P.m - 1), Cc.r0);
         }
-        HIP_OK(hipMemcpyAsync(ref_rows_d, ref_rows.data(), ref_rows.size() * sizeof(int4),
-                              hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(ref_list_d, ref_list.data(), ref_list.size() * sizeof(int4),
-                              hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_apply_correction_rows, dim3(ref_rows.size()), dim3(256), 0, st,
-                           inst_d, ref_rows_d);
-        hipLaunchKernelGGL(k_residual_rows, dim3(ref_rows.size()), dim3(256), 0, st, inst_d,
-                           ref_rows_d, ref_sums_d);
-        hipLaunchKernelGGL(k_residual_norm_list, dim3(ref_list.size()), dim3(256), 0, st,
-                           ref_list_d, ref_sums_d, resv_d);
+        ref.upload(ref_spans_d, st);
+        const int nref = (int)ref.h.size();
+        hipLaunchKernelGGL(k_apply_correction_spans, dim3(ref.rows), dim3(256), 0, st, inst_d,
+                           ref_spans_d, nref);
+        hipLaunchKernelGGL(k_residual_spans, dim3(ref.rows), dim3(256), 0, st, inst_d, ref_spans_d,
+                           nref, ref_sums_d);
+        hipLaunchKernelGGL(k_residual_norm_spans, dim3(nref), dim3(256), 0, st, ref_spans_d,
+                           ref_sums_d, resv_d);
         HIP_OK(hipGetLastError());
         // every instance's slot, the warm-start race's plain twins included
         HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
@@ -1464,16 +1516,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         HIP_OK(hipStreamSynchronize(st));
         total_refine += (uint32_t)pend.size();
         // pass 2: the reference's rules on the exact FP64 residual of sweep m
-        hist_stage.assign(pend.size(), 0.0);
         corr.clear();
         for (size_t q = 0; q < pend.size(); ++q) {
           const Pending &P = pend[q];
           PhCtl &Cc = ctl[P.a];
           const double gnorm = resv[2 * act[P.a]];
-          const double rel = gnorm / Cc.r0;
-          hist_stage[q] = rel;
-          HIP_OK(hipMemcpyAsync(Cc.history + (P.m - 1), &hist_stage[q], sizeof(double),
-                                hipMemcpyHostToDevice, st));
+          const double rel = gnorm / Cc.r0;  // the value k_residual_norm_spans wrote
           Cc.sweeps_final = P.m;
           Cc.sweeps_done = P.m;
           const int k = act[P.a];
@@ -1586,7 +1634,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       if (guard > 4.0 * MAX_ITERATIONS_LIMIT)
         throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "solve loop did not terminate");
     }
-    HIP_OK(hipStreamSynchronize(st));  // hist_stage is host staging
+    HIP_OK(hipStreamSynchronize(st));  // corr_stage is host staging
     if (timing) {
       hipEventDestroy(eb0);
       hipEventDestroy(eb1);
@@ -1609,12 +1657,25 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         X.hist = T.hist;
       }
     }
+    // write-back of every solved instance in one launch (a plain twin's
+    // planes stand in for its image's instance where the twin won)
+    SpanTable out_rows;
+    std::vector<PhInst> out_inst;
     for (int a = 0; a < nact; ++a) {
       const int k = act[a];
       if (k >= ninst || !inst[k].R.active) continue;
-      const InstWork &X = inst[k];
-      hipLaunchKernelGGL(k_scatter, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.out);
+      out_rows.add((int)out_inst.size(), inst[k].I.ph, inst[k].item);
+      out_inst.push_back(inst[k].I);
+    }
+    if (!out_inst.empty()) {
+      const int nout = (int)out_inst.size();
+      HIP_OK(hipMemcpyAsync(inst_all_d, out_inst.data(), nout * sizeof(PhInst),
+                            hipMemcpyHostToDevice, st));
+      out_rows.upload(inst_rows_d, st);
+      hipLaunchKernelGGL(k_scatter_spans, dim3(out_rows.rows), dim3(256), 0, st, inst_all_d,
+                         inst_rows_d, nout, io_d);
       HIP_OK(hipGetLastError());
+      HIP_OK(hipStreamSynchronize(st));  // out_rows and out_inst are host staging
     }
     for (int i = 0; i < nitem; ++i) {
       iw[i].in.rep->launches = total_launches;
@@ -1637,7 +1698,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     }
 
   // --- worst channel of each image (poisson_blend.rs:240-246: more
-  // iterations, then larger final residual; ties keep the earlier channel)
+  // iterations, then larger final residual; ties keep the earlier channel);
+  // the reported histories are gathered for one readback
+  std::vector<PhHistCopy> hcopy;
+  std::vector<double *> hdst;
+  int64_t htotal = 0;
   for (int i = 0; i < nitem; ++i) {
     ItemWork &w = iw[i];
     ph_report *rep = w.in.rep;
@@ -1670,13 +1735,25 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       if (Wr->nonfinite) {
         for (uint32_t q = 0; q < Wr->iterations; ++q) rep->residual_history[q] = NAN;
       } else {
-        HIP_OK(hipMemcpyAsync(rep->residual_history, whist, Wr->iterations * sizeof(double),
-                              hipMemcpyDeviceToHost, st));
+        hcopy.push_back(PhHistCopy{whist, htotal, (int32_t)Wr->iterations, 0});
+        hdst.push_back(rep->residual_history);
+        htotal += Wr->iterations;
       }
     } else if (Wr->iterations > 0 && rep->residual_history) {
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "history capacity %u < %u iterations",
                  rep->history_capacity, Wr->iterations);
     }
+  }
+  std::vector<double> hbuf((size_t)htotal);
+  if (!hcopy.empty()) {
+    PhHistCopy *hcopy_d = g_pool.take<PhHistCopy>(hcopy.size());
+    double *hbuf_d = g_pool.take<double>((size_t)htotal);
+    HIP_OK(hipMemcpyAsync(hcopy_d, hcopy.data(), hcopy.size() * sizeof(PhHistCopy),
+                          hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gather_hist, dim3(hcopy.size()), dim3(256), 0, st, hcopy_d, hbuf_d);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(hbuf.data(), hbuf_d, (size_t)htotal * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
   }
   if (timing) {
     HIP_OK(hipEventRecord(ev1, st));
@@ -1688,6 +1765,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     hipEventDestroy(ev1);
   }
   HIP_OK(hipStreamSynchronize(st));
+  for (size_t q = 0; q < hcopy.size(); ++q)
+    memcpy(hdst[q], hbuf.data() + hcopy[q].off, (size_t)hcopy[q].n * sizeof(double));
 }
 
 static void solve_device(const ph_solve_desc *d, const float *src, const float *tgt,
