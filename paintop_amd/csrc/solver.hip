@@ -29,6 +29,7 @@
 #include <cmath>
 #include <functional>
 #include <atomic>
+#include <chrono>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -704,6 +705,16 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     }
     g_pool.reset(need);
   }
+  // host time per phase (printed with PAINTOP_HIP_TRACE=1)
+  const bool phase_trace = trace_on();
+  auto phase_t0 = std::chrono::steady_clock::now();
+  auto phase = [&](const char *name) {
+    if (!phase_trace) return;
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "[phase] %-12s %8.3f ms\n", name,
+            std::chrono::duration<double, std::milli>(t - phase_t0).count());
+    phase_t0 = t;
+  };
   std::vector<ItemWork> iw(nitem);
   for (int i = 0; i < nitem; ++i) {
     iw[i].in = items[i];
@@ -761,6 +772,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   for (int i = 0; i < nitem; ++i)
     if ((size_t)iw[i].in.W * iw[i].in.H == 0) bbox_h[4 * i + 2] = -1;
 
+  phase("classify");
   // --- 2. planes, tile occupancy, coarse masks for the tuner (one readback)
   std::vector<int2> all_origins;
   size_t coarse_total = 0;
@@ -846,6 +858,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     HIP_OK(hipStreamSynchronize(st));
   }
 
+  phase("occupancy");
   // --- 3. live tiles, SOR factor and stall window per image
   for (int i = 0; i < nitem; ++i) {
     ItemWork &w = iw[i];
@@ -906,6 +919,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     w.in.rep->omega_used = w.omega;
   }
 
+  phase("tuner");
   // --- 4. per-channel planes, assembly and r0 (one readback)
   std::vector<InstWork> inst;
   for (int i = 0; i < nitem; ++i) {
@@ -986,6 +1000,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     HIP_OK(hipStreamSynchronize(st));
   }
 
+  phase("assembly");
   // --- 5. instances that need sweeps
   std::vector<int> act;
   for (int k = 0; k < ninst; ++k) {
@@ -1171,6 +1186,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     }
   }
 
+  phase("warm");
   const int nact = (int)act.size();
   if (nact > 0) {
     // the tile list of the running instances (rebuilt as instances finish)
@@ -1683,6 +1699,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       iw[i].in.rep->sweep_ms = sweep_ms;
     }
   }
+  phase("sweeps");
   // non-finite channels: interior samples become NaN (the reference's
   // residuals and field are NaN there)
   for (int k = 0; k < ninst; ++k)
@@ -1767,6 +1784,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   HIP_OK(hipStreamSynchronize(st));
   for (size_t q = 0; q < hcopy.size(); ++q)
     memcpy(hdst[q], hbuf.data() + hcopy[q].off, (size_t)hcopy[q].n * sizeof(double));
+  phase("reports");
 }
 
 static void solve_device(const ph_solve_desc *d, const float *src, const float *tgt,

@@ -293,14 +293,14 @@ def make_desc(width, height, channels, max_iterations, tolerance, lam=0.0, op=PH
                        tile_rows=tile_rows)
 
 
-def _result(rep: PhReport, hist: np.ndarray) -> SolveResult:
+def _result(rep: PhReport, hist: np.ndarray | None) -> SolveResult:
     return SolveResult(
         iterations=int(rep.iterations),
         converged=bool(rep.converged),
         stop_reason=STOP_NAMES.get(int(rep.stop_reason), "unknown"),
         tolerance=float(rep.tolerance),
         final_residual=float(rep.final_residual),
-        residual_history=[float(v) for v in hist[: rep.history_len]],
+        residual_history=[] if hist is None else hist[: rep.history_len].tolist(),
         worst_channel=int(rep.worst_channel),
         refinements=int(rep.refinements),
         launches=int(rep.launches),
@@ -455,12 +455,11 @@ def poisson_solve_batch_device(desc: PhSolveDesc, items, stream: int = 0,
         arr[i] = PhBatchItem(w, h, c, 0, s_, t_, m_, o_)
     reps = (PhReport * max(n, 1))()
     cap = int(desc.max_iterations)
-    hists = []
-    for i in range(n):
-        hist = np.zeros(max(cap, 1), dtype=np.float64)
-        hists.append(hist)
-        if want_history:
-            reps[i].residual_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    hists = [None] * n  # no history buffers unless asked for (max_iterations doubles each)
+    if want_history:
+        for i in range(n):
+            hists[i] = np.zeros(max(cap, 1), dtype=np.float64)
+            reps[i].residual_history = hists[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             reps[i].history_capacity = cap
     err = PhError()
     st = lib().ph_poisson_solve_batch_device(ctypes.byref(desc), arr, n, stream or None, reps,
