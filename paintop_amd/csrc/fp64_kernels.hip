@@ -205,11 +205,11 @@ __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict
     assemble_cell(I, G.src, G.tgt, G.cls_img, G.cls_plane, px, py);
 }
 
-// g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and the per-row
-// sums of r^2 in a fixed order; k_residual_norm adds the rows in order and
-// stores ||r|| in out[0] and ||r||^2 in out[1]. (A last-block reduction inside
-// this kernel would cost one device-scope atomic per row block, which
-// serialise across the XCDs and took longer than the rows themselves.)
+// g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and this
+// thread's part of the row's sum of r^2 (the block adds the parts in a fixed
+// order; a separate launch adds the rows, since a last-block reduction in the
+// same kernel costs one device-scope atomic per row block, which serialise
+// across the XCDs and took longer than the rows themselves).
 // One block per plane row. Each thread takes cell pairs (px, px + 1), px even,
 // so the loads of a pair are independent 16-byte vectors; rows 0 and ph - 1
 // and columns 0 and pw - 1 are padding (no interior cells), so every
@@ -264,24 +264,8 @@ __device__ __forceinline__ double rows_sum(const double *__restrict__ rs, int nr
   return block_sum(v);
 }
 
-__global__ __launch_bounds__(256) void k_residual(PhInst I, double *__restrict__ row_sums) {
-  const double s = block_sum(residual_row(I, blockIdx.x));
-  if (threadIdx.x == 0) row_sums[blockIdx.x] = s;
-}
-
-// ||r|| from the row sums of k_residual, added in a fixed order; out[0] =
-// ||r||, out[1] = ||r||^2. One block.
-__global__ __launch_bounds__(256) void k_residual_norm(const double *__restrict__ row_sums,
-                                                       int nrows, double *__restrict__ out) {
-  const double s = rows_sum(row_sums, nrows);
-  if (threadIdx.x == 0) {
-    out[0] = sqrt(s);
-    out[1] = s;
-  }
-}
-
-// The refinement of many instances in three launches instead of three per
-// instance (a batch of small images refines hundreds of instances a pass).
+// The refinement of many instances in three launches, not three per instance
+// (a batch of small images refines hundreds of instances a pass).
 // x += e[buf], e[buf] = 0 at the interior cells of every listed row.
 __global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__restrict__ insts,
                                                                 const PhSpan *__restrict__ spans,
@@ -300,7 +284,7 @@ __global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__
   }
 }
 
-// k_residual for every listed row (the same arithmetic and order);
+// g = -r(x) and the row's sum of r^2 (residual_row) for every listed row;
 // row_sums[b] is the sum of block b's row.
 __global__ __launch_bounds__(256) void k_residual_spans(const PhInst *__restrict__ insts,
                                                         const PhSpan *__restrict__ spans, int n,
@@ -311,10 +295,10 @@ __global__ __launch_bounds__(256) void k_residual_spans(const PhInst *__restrict
   if (threadIdx.x == 0) row_sums[b] = s;
 }
 
-// ||r|| of each listed instance (one block per entry), added exactly as
-// k_residual_norm adds an instance's rows, so the norm does not depend on the
-// table: out[2 * out_slot] = ||r||, out[2 * out_slot + 1] = ||r||^2, and the
-// history entry, when given, receives ||r|| / r0.
+// ||r|| of each listed instance (one block per entry) from its row sums in a
+// fixed order (rows_sum), so the norm does not depend on the table or the
+// other entries: out[2 * out_slot] = ||r||, out[2 * out_slot + 1] = ||r||^2,
+// and the history entry, when given, receives ||r|| / r0.
 __global__ __launch_bounds__(256) void k_residual_norm_spans(const PhSpan *__restrict__ spans,
                                                              const double *__restrict__ row_sums,
                                                              double *__restrict__ out) {
@@ -345,6 +329,12 @@ __global__ __launch_bounds__(256) void k_scatter_spans(const PhInst *__restrict_
       out[q * I.channels + I.channel] = (float)I.x[p];
     }
   }
+}
+
+// Write n values to their device addresses (one launch for many small updates).
+__global__ void k_put_values(const PhValue *__restrict__ v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) *v[i].dst = v[i].val;
 }
 
 // Gather residual histories into one array for a single readback (one block

@@ -160,6 +160,23 @@ struct PhHistCopy {
   int32_t n, pad;
 };
 
+// One network instance of a batched warm start or correction round: entry j
+// of the round's list (the same order as its span table).
+struct PhCorrArgs {
+  int32_t slot;       // instance index in the descriptor array
+  int32_t x0, y0, f;  // coarse-grid framing (image cell of coarse cell (0, 0), cell size)
+  int32_t cpb;        // coarse cells per block of the input coarsening
+  int32_t net;        // network batch index: prediction h + res*res*net, scale[net], alpha[net]
+  int32_t item, pad;  // image index (the warm start reads the image)
+  double sign;        // +1 applies the correction, -1 takes it back
+};
+
+// A value for one device address (batched small updates).
+struct PhValue {
+  double *dst;
+  double val;
+};
+
 // Entry of a span table that holds block b: the last entry with row0 <= b.
 __device__ __forceinline__ int ph_span_find(const PhSpan *__restrict__ s, int n, int b) {
   int lo = 0, hi = n - 1;

@@ -51,8 +51,6 @@ __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 
                                      int *);
 __global__ void k_coarse_count_all(const PhItemGeo *, int *);
 __global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
-__global__ void k_residual(PhInst, double *);
-__global__ void k_residual_norm(const double *, int, double *);
 __global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
 __global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
 __global__ void k_residual_spans(const PhInst *, const PhSpan *, int, double *);
@@ -62,15 +60,19 @@ __global__ void k_scatter_spans(const PhInst *, const PhSpan *, int, const PhIte
 __global__ void k_gather_hist(const PhHistCopy *, double *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
                           int, double, int, int, int, int, int, float *, float *, float *);
-__global__ void k_warm_apply(PhInst, const float *, const float *, const float *, int, int, int,
-                             int);
+__global__ void k_coarsen_list(const PhInst *, const PhCorrArgs *, const PhItemIO *, double, int,
+                               float *);
+__global__ void k_warm_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
+                                   const PhItemIO *, const float *, const float *, int);
 __global__ void k_coarsen_plane(PhInst, int, int, int, int, int, float *);
+__global__ void k_coarsen_plane_list(const PhInst *, const PhCorrArgs *, int, float *);
+__global__ void k_correct_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
+                                      const float *, const float *, const double *, int);
+__global__ void k_put_values(const PhValue *, int);
 int coarsen_plane_cpb(int f);
 __global__ void k_in_norm_part(const float *, int, double *);
 __global__ void k_in_norm_apply(float *, int, const double *, float *);
 __global__ void k_corr_alpha_part(const float *, const float *, int, double *);
-__global__ void k_correct_apply(PhInst, const float *, const float *, const double *, int, int,
-                                int, int, double);
 __global__ void k_corr_alpha(const double *, int, double, double *);
 __global__ void k_start_apply(PhInst, const float *);
 bool warm_net_ready();
@@ -343,27 +345,25 @@ static const double WARM_MAX_START_RATIO = 32.0;
 // ramp restarts. The cycle is a multiple of the ramp (PAINTOP_HIP_CYCLE_FACTOR,
 // default 1.5), on a launch boundary; PAINTOP_HIP_CORRECTIONS caps the count per
 // channel (0 = none).
-// The network's start pays only where sweeps are dear too: below this many
-// interior cells per channel (PAINTOP_HIP_WARM_MIN_CELLS) a channel takes the
-// plain start. Its race twin and network pass cost more than they saved on the
-// batch of 512 blends of 32..1024 px (137 against 120 ms), while a 1024^2
-// blend (0.3M cells per channel) gains 1.7x.
-static uint64_t warm_net_min_cells() {
-  long long n = 1ll << 18;
-  if (const char *e = getenv("PAINTOP_HIP_WARM_MIN_CELLS")) n = atoll(e);
-  return (uint64_t)std::max(0LL, n);
-}
-
-// Corrections pay only where sweeps are dear: each round costs a network
-// forward, a few plane passes and a host round trip, ~0.5-1 ms, against
-// sweeps of ~3 us per million interior cells. Instances with fewer interior
-// cells than this (PAINTOP_HIP_CORR_MIN_CELLS) keep the warm start alone;
-// 2048^2 and 4096^2 blends (1.2M / 4.9M per channel) gain 2.6x / 3.4x in time,
-// a batch of 512 blends of 32..1024 px lost 1.9x with them.
-static uint64_t warm_corr_min_cells() {
-  long long n = 1ll << 20;
-  if (const char *e = getenv("PAINTOP_HIP_CORR_MIN_CELLS")) n = atoll(e);
-  return (uint64_t)std::max(0LL, n);
+// The network's start and its corrections pay only where sweeps are dear: each
+// costs a race twin or a network round (a forward, a few plane passes and a
+// host round trip) whatever the instance's size, while a small instance's
+// sweeps cost little, and nothing at all in a batch, where they ride in the
+// launches of the larger instances. So a channel takes the learned path when
+// its interior cells reach max(2^14, the call's largest instance / 8), or
+// PAINTOP_HIP_WARM_MIN_CELLS / PAINTOP_HIP_CORR_MIN_CELLS (start / corrections)
+// when set. Measured on MI355X (ms, learned against plain):
+//   4-channel star blends: 256^2 (20k cells per channel) 3.0 vs 4.7 with the
+//   gate at 2^14; 512^2 3.4 vs 4.9; 1024^2 4.9 vs 15.0 (8.6 with the start
+//   alone);
+//   the batch of 512 blends of 32..1024 px (largest 0.6M cells): fixed gates
+//   of 2^12 / 2^14 / 2^15 / 2^16 gave 49.5 / 43.4 / 40.0 / 38.5 against 55.0
+//   plain (2^18 for the start and 2^20 for the corrections: 59.7).
+static const uint64_t WARM_GATE_FLOOR = 1ull << 14;
+static const uint64_t WARM_GATE_DIVISOR = 8;
+static uint64_t warm_gate(const char *env, uint64_t largest) {
+  if (const char *e = getenv(env)) return (uint64_t)std::max(0LL, atoll(e));
+  return std::max(WARM_GATE_FLOOR, largest / WARM_GATE_DIVISOR);
 }
 static uint32_t warm_max_corrections() {
   long n = 16;
@@ -578,7 +578,6 @@ struct InstWork {
   double r0 = 0.0;
   ChanResult R;
   double *hist = nullptr;
-  double *rowsum = nullptr;
   dim3 pgrd;
 };
 
@@ -651,7 +650,8 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
   return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) +
          ntile_up * (sizeof(int2) + 2 * sizeof(int)) + sizeof(PhItemGeo) +
-         (size_t)C * 3 * ph_up * sizeof(double) + (size_t)(4 * C + 2) * sizeof(PhSpan) +
+         (size_t)C * 5 * ph_up * sizeof(double) + (size_t)(10 * C + 2) * sizeof(PhSpan) +
+         (size_t)C * (2 * sizeof(PhInst) + 4 * sizeof(PhCorrArgs) + sizeof(PhValue)) +
          sizeof(PhItemIO) + sizeof(PhHistCopy) + (size_t)(d->max_iterations + 2) * sizeof(double) +
          plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
          (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
@@ -956,7 +956,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       I.lambda = d->lambda;
       X.pgrd = dim3((w.pitch + 255) / 256, w.phh);
       X.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
-      X.rowsum = g_pool.take<double>(w.phh);
       inst.push_back(X);
     }
   }
@@ -964,13 +963,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int ncap = std::max(race_any ? 2 * ninst : ninst, 1);
   double *resv_d = g_pool.take<double>(2 * (size_t)ncap);
   std::vector<double> resv(2 * (size_t)ncap);
-  auto launch_residual = [&](int k) {
-    const InstWork &X = inst[k];
-    hipLaunchKernelGGL(k_residual, dim3(X.I.ph), dim3(256), 0, st, X.I, X.rowsum);
-    hipLaunchKernelGGL(k_residual_norm, dim3(1), dim3(256), 0, st, X.rowsum, X.I.ph,
-                       resv_d + 2 * k);
-    HIP_OK(hipGetLastError());
-  };
   // assembly and the anchor-start residual of every instance: one launch per
   // step over all plane rows (inst_all_d is the instances' descriptors)
   PhInst *inst_all_d = g_pool.take<PhInst>(std::max(ninst, 1));
@@ -1057,7 +1049,9 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   // candidates: images with a caller's start field, and (with the network
   // flag) every other Poisson-blend instance
   std::vector<int> cand, net_list;
-  const uint64_t warm_min_cells = warm_net_min_cells();
+  uint64_t largest = 0;
+  for (int k : act) largest = std::max(largest, (uint64_t)iw[inst[k].item].n_interior);
+  const uint64_t warm_min_cells = warm_gate("PAINTOP_HIP_WARM_MIN_CELLS", largest);
   for (int k : act) {
     if (iw[inst[k].item].in.start) cand.push_back(k);
     else if (want_warm && d->lambda == 0.0 &&
@@ -1083,7 +1077,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     net_part = g_pool.take<double>((size_t)2 * PH_NET_SLICES * NNB);
   }
   const uint32_t max_corr = warm_max_corrections();
-  const uint64_t corr_min_cells = warm_corr_min_cells();
+  const uint64_t corr_min_cells = warm_gate("PAINTOP_HIP_CORR_MIN_CELLS", largest);
+  // host staging of the warm start (kept until the stream is synchronised)
+  std::vector<PhCorrArgs> warm_args;
+  std::vector<PhInst> re_inst;
+  SpanTable warm_rows, cand_rows, re_rows;
   if (!cand.empty()) {
     const int nw = (int)cand.size();
     if (!net_list.empty()) {
@@ -1093,16 +1091,25 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         const int f = std::max(1, (std::max(w.rw, w.rh) + NR - 1) / NR);
         wgeo[k] = Geo{w.rx0 + w.rw / 2 - (NR * f) / 2, w.ry0 + w.rh / 2 - (NR * f) / 2, f};
       }
+      // inputs, network and application for every network instance: one
+      // coarsening launch per network batch, one application launch
+      warm_args.resize(nn);
+      warm_rows.clear();
+      for (int j = 0; j < nn; ++j) {
+        const int k = net_list[j];
+        const Geo &g = wgeo[k];
+        warm_args[j] = PhCorrArgs{k, g.x0, g.y0, g.f, 0, j, inst[k].item, 0, 1.0};
+        warm_rows.add(k, inst[k].I.ph, inst[k].item);
+      }
+      PhCorrArgs *warm_args_d = g_pool.take<PhCorrArgs>(nn);
+      PhSpan *warm_spans_d = g_pool.take<PhSpan>(nn);
+      HIP_OK(hipMemcpyAsync(warm_args_d, warm_args.data(), nn * sizeof(PhCorrArgs),
+                            hipMemcpyHostToDevice, st));
+      warm_rows.upload(warm_spans_d, st);
       for (int j0 = 0; j0 < nn; j0 += NNB) {
         const int nb = std::min(NNB, nn - j0);
-        for (int j = 0; j < nb; ++j) {
-          const InstWork &X = inst[net_list[j0 + j]];
-          const ItemIn &it = iw[X.item].in;
-          const Geo &g = wgeo[net_list[j0 + j]];
-          hipLaunchKernelGGL(k_coarsen, dim3(NR * NR), dim3(64), 0, st, it.src, it.tgt, it.mask,
-                             nullptr, it.W, it.H, it.C, X.channel, d->lambda, g.x0, g.y0, g.f,
-                             NR, 0, net_in + NR2 * 4 * j, nullptr, nullptr);
-        }
+        hipLaunchKernelGGL(k_coarsen_list, dim3(NR * NR, nb), dim3(64), 0, st, inst_all_d,
+                           warm_args_d + j0, io_d, d->lambda, NR, net_in);
         hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
                            net_part);
         hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
@@ -1110,22 +1117,28 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         HIP_OK(hipGetLastError());
         warm_net_forward(net_in, net_h + NR2 * j0, nb, st);
       }
-      for (int j = 0; j < nn; ++j) {
-        const InstWork &X = inst[net_list[j]];
-        const Geo &g = wgeo[net_list[j]];
-        hipLaunchKernelGGL(k_warm_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.src,
-                           net_h + NR2 * j, net_scale + j, NR, g.x0, g.y0, g.f);
-        HIP_OK(hipGetLastError());
-      }
+      hipLaunchKernelGGL(k_warm_apply_spans, dim3(warm_rows.rows), dim3(256), 0, st, inst_all_d,
+                         warm_spans_d, nn, warm_args_d, io_d, net_h, net_scale, NR);
+      HIP_OK(hipGetLastError());
     }
+    // the caller's start fields, then the starts' FP64 residuals in three launches
+    cand_rows.clear();
     for (int k : cand) {
       const InstWork &X = inst[k];
       if (iw[X.item].in.start) {
         hipLaunchKernelGGL(k_start_apply, X.pgrd, dim3(256), 0, st, X.I, iw[X.item].in.start);
         HIP_OK(hipGetLastError());
       }
-      launch_residual(k);
+      cand_rows.add(k, X.I.ph, X.item, 0, k);
     }
+    PhSpan *cand_spans_d = g_pool.take<PhSpan>(nw);
+    double *cand_sums_d = g_pool.take<double>((size_t)cand_rows.rows);
+    cand_rows.upload(cand_spans_d, st);
+    hipLaunchKernelGGL(k_residual_spans, dim3(cand_rows.rows), dim3(256), 0, st, inst_all_d,
+                       cand_spans_d, nw, cand_sums_d);
+    hipLaunchKernelGGL(k_residual_norm_spans, dim3(nw), dim3(256), 0, st, cand_spans_d,
+                       cand_sums_d, resv_d);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1138,10 +1151,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       rep->warm_start_rel = std::max(rep->warm_start_rel, std::isfinite(rw) ? rw / r0 : INFINITY);
       if (!(std::isfinite(rw) && rw <= WARM_MAX_START_RATIO * r0)) {
         // no race for a start this far off: back to the anchor start at once
-        hipLaunchKernelGGL(k_assemble, inst[k].pgrd, dim3(256), 0, st, inst[k].I, w.in.src,
-                           w.in.tgt, w.cls_img, w.cls_plane);
-        HIP_OK(hipGetLastError());
-        launch_residual(k);
+        re_rows.add((int)re_inst.size(), inst[k].I.ph, inst[k].item, 0, k);
+        re_inst.push_back(inst[k].I);
         continue;
       }
       // the plain twin: its own planes, the anchor start
@@ -1153,14 +1164,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       T.I.x = g_pool.take<double>(plane);
       T.I.bp = g_pool.take<double>(plane);
       T.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
-      T.rowsum = g_pool.take<double>(w.phh);
-      hipLaunchKernelGGL(k_assemble, T.pgrd, dim3(256), 0, st, T.I, w.in.src, w.in.tgt,
-                         w.cls_img, w.cls_plane);
-      HIP_OK(hipGetLastError());
       inst.push_back(T);
       const int kt = (int)inst.size() - 1;
       grow();
-      launch_residual(kt);
+      re_rows.add((int)re_inst.size(), T.I.ph, T.item, 0, kt);
+      re_inst.push_back(T.I);
       InstWork &Xw = inst[k];
       const bool cycles = !w.in.start && max_corr > 0 && (uint64_t)w.n_interior >= corr_min_cells;
       const uint32_t n = warm_ramp_sweeps(w.omega, cycles);
@@ -1184,6 +1192,23 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         cyc_next[k] = std::max<uint32_t>(cyc_len[k], (uint32_t)probe[k] + half);
       }
     }
+    // the anchor start again for the rejected starts, and the plain twins:
+    // their assembly and FP64 residual in three launches
+    if (!re_inst.empty()) {
+      const int nr = (int)re_inst.size();
+      PhInst *re_d = g_pool.take<PhInst>(nr);
+      PhSpan *re_spans_d = g_pool.take<PhSpan>(nr);
+      double *re_sums_d = g_pool.take<double>((size_t)re_rows.rows);
+      HIP_OK(hipMemcpyAsync(re_d, re_inst.data(), nr * sizeof(PhInst), hipMemcpyHostToDevice, st));
+      re_rows.upload(re_spans_d, st);
+      hipLaunchKernelGGL(k_assemble_spans, dim3(re_rows.rows), dim3(256), 0, st, re_d, re_spans_d,
+                         nr, io_d);
+      hipLaunchKernelGGL(k_residual_spans, dim3(re_rows.rows), dim3(256), 0, st, re_d, re_spans_d,
+                         nr, re_sums_d);
+      hipLaunchKernelGGL(k_residual_norm_spans, dim3(nr), dim3(256), 0, st, re_spans_d, re_sums_d,
+                         resv_d);
+      HIP_OK(hipGetLastError());
+    }
   }
 
   phase("warm");
@@ -1204,6 +1229,18 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     for (int a = 0; a < nact; ++a) row_cap += (size_t)inst[act[a]].I.ph;
     double *ref_sums_d = g_pool.take<double>(row_cap);
     PhSpan *ref_spans_d = g_pool.take<PhSpan>(nact);
+    // correction rounds: arguments and span tables of the corrections and
+    // of the take-backs, the history entries of the kept ones
+    PhCorrArgs *corr_args_d = nullptr, *undo_args_d = nullptr;
+    PhSpan *corr_spans_d = nullptr, *undo_spans_d = nullptr;
+    PhValue *corr_vals_d = nullptr;
+    if (!net_list.empty()) {
+      corr_args_d = g_pool.take<PhCorrArgs>(nact);
+      undo_args_d = g_pool.take<PhCorrArgs>(nact);
+      corr_spans_d = g_pool.take<PhSpan>(nact);
+      undo_spans_d = g_pool.take<PhSpan>(nact);
+      corr_vals_d = g_pool.take<PhValue>(nact);
+    }
     SpanTable ref;
     std::vector<PhTile> tl;
     int ntiles = 0;
@@ -1321,7 +1358,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       double gnorm;
     };
     std::vector<Corr> corr;
-    std::vector<double> corr_stage;
+    // host staging of the correction rounds (kept until the stream is synchronised)
+    std::vector<PhCorrArgs> corr_args, undo_args;
+    std::vector<PhValue> corr_vals;
+    std::vector<PhInst> ai_stage;
+    SpanTable corr_rows, undo_rows;
     auto correct = [&]() {
       // cycles that stopped paying for themselves end here, without a correction
       std::vector<Corr> go;
@@ -1348,15 +1389,24 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       }
       if (go.empty()) return;
       const int nc = (int)go.size();
+      corr_args.resize(nc);
+      corr_rows.clear();
+      for (int j = 0; j < nc; ++j) {
+        const int a = go[j].a, k = act[a];
+        const Geo &g = wgeo[k];
+        corr_args[j] = PhCorrArgs{a, g.x0, g.y0, g.f, coarsen_plane_cpb(g.f), j, inst[k].item, 0, 1.0};
+        corr_rows.add(a, ai[a].ph, 0, 0, k);
+      }
+      HIP_OK(hipMemcpyAsync(corr_args_d, corr_args.data(), nc * sizeof(PhCorrArgs),
+                            hipMemcpyHostToDevice, st));
+      corr_rows.upload(corr_spans_d, st);
       for (int c0 = 0; c0 < nc; c0 += NNB) {
         const int nb = std::min(NNB, nc - c0);
-        for (int j = 0; j < nb; ++j) {
-          const int k = act[go[c0 + j].a];
-          const Geo &g = wgeo[k];
-          const int cpb = coarsen_plane_cpb(g.f);
-          hipLaunchKernelGGL(k_coarsen_plane, dim3((NR + cpb - 1) / cpb, NR), dim3(256), 0, st,
-                             inst[k].I, g.x0, g.y0, g.f, NR, cpb, net_in + NR2 * 4 * j);
-        }
+        int gx = 1;
+        for (int j = c0; j < c0 + nb; ++j)
+          gx = std::max(gx, (NR + corr_args[j].cpb - 1) / corr_args[j].cpb);
+        hipLaunchKernelGGL(k_coarsen_plane_list, dim3(gx, NR, nb), dim3(256), 0, st, inst_d,
+                           corr_args_d + c0, NR, net_in);
         hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
                            net_part);
         hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
@@ -1369,15 +1419,14 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                            CORR_MAX_STEP, corr_alpha + c0);
         HIP_OK(hipGetLastError());
       }
-      for (int j = 0; j < nc; ++j) {
-        const int k = act[go[j].a];
-        const InstWork &X = inst[k];
-        const Geo &g = wgeo[k];
-        hipLaunchKernelGGL(k_correct_apply, X.pgrd, dim3(256), 0, st, X.I, net_h + NR2 * j,
-                           net_scale + j, corr_alpha + j, NR, g.x0, g.y0, g.f, 1.0);
-        HIP_OK(hipGetLastError());
-        launch_residual(k);
-      }
+      // every correction and its FP64 residual in three launches
+      hipLaunchKernelGGL(k_correct_apply_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
+                         corr_spans_d, nc, corr_args_d, net_h, net_scale, corr_alpha, NR);
+      hipLaunchKernelGGL(k_residual_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
+                         corr_spans_d, nc, ref_sums_d);
+      hipLaunchKernelGGL(k_residual_norm_spans, dim3(nc), dim3(256), 0, st, corr_spans_d,
+                         ref_sums_d, resv_d);
+      HIP_OK(hipGetLastError());
       std::vector<double> alpha(nc);
       HIP_OK(hipMemcpyAsync(alpha.data(), corr_alpha, nc * sizeof(double), hipMemcpyDeviceToHost,
                             st));
@@ -1386,27 +1435,37 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       HIP_OK(hipStreamSynchronize(st));
       std::vector<char> undo(nc, 0);
       std::vector<double> jump(nc, 0.0);
-      bool any_undo = false;
+      undo_args.clear();
+      undo_rows.clear();
       for (int j = 0; j < nc; ++j) {
-        const int k = act[go[j].a];
+        const int a = go[j].a, k = act[a];
         jump[j] = resv[2 * k] / go[j].gnorm;
         if (alpha[j] != 0.0 && !(std::isfinite(jump[j]) && jump[j] <= CORR_MAX_JUMP)) {
-          const InstWork &X = inst[k];
-          const Geo &g = wgeo[k];
-          hipLaunchKernelGGL(k_correct_apply, X.pgrd, dim3(256), 0, st, X.I, net_h + NR2 * j,
-                             net_scale + j, corr_alpha + j, NR, g.x0, g.y0, g.f, -1.0);
-          HIP_OK(hipGetLastError());
-          launch_residual(k);
+          PhCorrArgs A = corr_args[j];
+          A.sign = -1.0;
+          undo_args.push_back(A);
+          undo_rows.add(a, ai[a].ph, 0, 0, k);
           undo[j] = 1;
-          any_undo = true;
         }
       }
-      if (any_undo) {
+      if (!undo_args.empty()) {
+        const int nu = (int)undo_args.size();
+        HIP_OK(hipMemcpyAsync(undo_args_d, undo_args.data(), nu * sizeof(PhCorrArgs),
+                              hipMemcpyHostToDevice, st));
+        undo_rows.upload(undo_spans_d, st);
+        hipLaunchKernelGGL(k_correct_apply_spans, dim3(undo_rows.rows), dim3(256), 0, st, inst_d,
+                           undo_spans_d, nu, undo_args_d, net_h, net_scale, corr_alpha, NR);
+        hipLaunchKernelGGL(k_residual_spans, dim3(undo_rows.rows), dim3(256), 0, st, inst_d,
+                           undo_spans_d, nu, ref_sums_d);
+        hipLaunchKernelGGL(k_residual_norm_spans, dim3(nu), dim3(256), 0, st, undo_spans_d,
+                           ref_sums_d, resv_d);
+        HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
                               hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
       }
-      corr_stage.assign(nc, 0.0);
+      corr_vals.clear();
+      bool ramp_moved = false;
       for (int j = 0; j < nc; ++j) {
         const int a = go[j].a, k = act[a];
         const uint32_t m = go[j].m;
@@ -1430,10 +1489,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           // the omega ramp restarts at the corrected state
           ai[a].ramp_from = m;
           inst[k].I.ramp_from = m;
-          HIP_OK(hipMemcpyAsync(inst_d + a, &ai[a], sizeof(PhInst), hipMemcpyHostToDevice, st));
-          corr_stage[j] = rel;
-          HIP_OK(hipMemcpyAsync(Cc.history + (m - 1), &corr_stage[j], sizeof(double),
-                                hipMemcpyHostToDevice, st));
+          ramp_moved = true;
+          corr_vals.push_back(PhValue{Cc.history + (m - 1), rel});
           // the residual dips and then rises through the restarted ramp: the
           // stall guard starts again after it, the decay estimate from here
           Cc.best = INFINITY;
@@ -1447,6 +1504,19 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         } else {
           resume(a, gnorm);
         }
+      }
+      // the restarted ramps and the kept corrections' history entries
+      if (ramp_moved) {
+        ai_stage = ai;
+        HIP_OK(hipMemcpyAsync(inst_d, ai_stage.data(), nact * sizeof(PhInst),
+                              hipMemcpyHostToDevice, st));
+      }
+      if (!corr_vals.empty()) {
+        HIP_OK(hipMemcpyAsync(corr_vals_d, corr_vals.data(), corr_vals.size() * sizeof(PhValue),
+                              hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_put_values, dim3((corr_vals.size() + 255) / 256), dim3(256), 0, st,
+                           corr_vals_d, (int)corr_vals.size());
+        HIP_OK(hipGetLastError());
       }
     };
 
@@ -1650,7 +1720,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       if (guard > 4.0 * MAX_ITERATIONS_LIMIT)
         throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "solve loop did not terminate");
     }
-    HIP_OK(hipStreamSynchronize(st));  // corr_stage is host staging
+    HIP_OK(hipStreamSynchronize(st));  // the correction rounds' host staging
     if (timing) {
       hipEventDestroy(eb0);
       hipEventDestroy(eb1);

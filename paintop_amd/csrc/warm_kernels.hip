@@ -40,14 +40,14 @@ __device__ __forceinline__ bool warm_interior(const float *mask, int W, int H, i
 // mode 1 (correction problem): P = 0 and the rhs is the 5-point Laplacian of
 // the source (poisson.rs:364-373): a blend with a zero anchor is the problem
 // Laplacian(u) = rhs, u = 0 on the ring, that a coarse correction solves.
-__global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
-                                                const float *__restrict__ anc,
-                                                const float *__restrict__ mask,
-                                                const float *__restrict__ sol, int W, int H, int C,
-                                                int ch, double lambda, int x0, int y0, int f,
-                                                int res, int mode, float *__restrict__ in4,
-                                                float *__restrict__ tgt, float *__restrict__ wt) {
-  const int c = blockIdx.x;
+__device__ __forceinline__ void coarsen_cell(const float *__restrict__ src,
+                                             const float *__restrict__ anc,
+                                             const float *__restrict__ mask,
+                                             const float *__restrict__ sol, int W, int H, int C,
+                                             int ch, double lambda, int x0, int y0, int f,
+                                             int res, int mode, float *__restrict__ in4,
+                                             float *__restrict__ tgt, float *__restrict__ wt,
+                                             int c) {
   const int cx = c % res, cy = c / res;
   const int lane = threadIdx.x;
   double s_rhs = 0.0, s_ring = 0.0, n_int = 0.0, n_ring = 0.0, s_t = 0.0;
@@ -98,6 +98,31 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
   }
 }
 
+__global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
+                                                const float *__restrict__ anc,
+                                                const float *__restrict__ mask,
+                                                const float *__restrict__ sol, int W, int H, int C,
+                                                int ch, double lambda, int x0, int y0, int f,
+                                                int res, int mode, float *__restrict__ in4,
+                                                float *__restrict__ tgt, float *__restrict__ wt) {
+  coarsen_cell(src, anc, mask, sol, W, H, C, ch, lambda, x0, y0, f, res, mode, in4, tgt, wt,
+               blockIdx.x);
+}
+
+// The warm-start inputs (mode 0) of every listed network instance in one
+// launch: blockIdx.y = entry j, whose input goes to in4 + res*res*4*j.
+__global__ __launch_bounds__(64) void k_coarsen_list(const PhInst *__restrict__ insts,
+                                                     const PhCorrArgs *__restrict__ args,
+                                                     const PhItemIO *__restrict__ io,
+                                                     double lambda, int res,
+                                                     float *__restrict__ in4) {
+  const PhCorrArgs A = args[blockIdx.y];
+  const PhItemIO &G = io[A.item];
+  coarsen_cell(G.src, G.tgt, G.mask, nullptr, G.W, G.H, G.C, insts[A.slot].channel, lambda, A.x0,
+               A.y0, A.f, res, 0, in4 + (size_t)res * res * 4 * blockIdx.y, nullptr, nullptr,
+               blockIdx.x);
+}
+
 // Net input of a coarse correction, from an instance's plane: per block the
 // sum of g (the correction equation's rhs, Laplacian(e) = g) over interior
 // cells, zero boundary values, interior fraction and ring density, exactly as
@@ -109,12 +134,12 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
 // result is deterministic. Columns go through LDS in chunks of
 // CP_MAX_COLS; a coarse cell wider than that (cpb = 1) adds chunk by chunk.
 constexpr int CP_MAX_COLS = 2048;
-__global__ __launch_bounds__(256) void k_coarsen_plane(PhInst I, int x0, int y0, int f, int res,
-                                                       int cpb, float *__restrict__ in4) {
+__device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int y0, int f,
+                                                    int res, int cpb, float *__restrict__ in4,
+                                                    int bx, int cy) {
   __shared__ double s_rhs[CP_MAX_COLS];
   __shared__ int s_int[CP_MAX_COLS], s_ring[CP_MAX_COLS];
-  const int cy = blockIdx.y;
-  const int cx0 = blockIdx.x * cpb;
+  const int cx0 = bx * cpb;
   const int ncc = min(cpb, res - cx0);
   const int ncols = ncc * f;
   const int pxb = x0 + cx0 * f - I.ox;  // plane column of the block's column 0
@@ -200,6 +225,23 @@ __global__ __launch_bounds__(256) void k_coarsen_plane(PhInst I, int x0, int y0,
     o[2] = (float)(wi / ((double)f * f));
     o[3] = (float)(wn / f);
   }
+}
+
+__global__ __launch_bounds__(256) void k_coarsen_plane(PhInst I, int x0, int y0, int f, int res,
+                                                       int cpb, float *__restrict__ in4) {
+  coarsen_plane_block(I, x0, y0, f, res, cpb, in4, blockIdx.x, blockIdx.y);
+}
+
+// k_coarsen_plane for every correction of a round in one launch: blockIdx.z =
+// entry j, whose input goes to in4 + res*res*4*j. gridDim.x covers the entry
+// with the most blocks per coarse row; blocks past an entry's row end exit.
+__global__ __launch_bounds__(256) void k_coarsen_plane_list(const PhInst *__restrict__ insts,
+                                                            const PhCorrArgs *__restrict__ args,
+                                                            int res, float *__restrict__ in4) {
+  const PhCorrArgs A = args[blockIdx.z];
+  if ((int)blockIdx.x * A.cpb >= res) return;
+  coarsen_plane_block(insts[A.slot], A.x0, A.y0, A.f, res, A.cpb,
+                      in4 + (size_t)res * res * 4 * blockIdx.z, blockIdx.x, blockIdx.y);
 }
 
 // Coarse cells per block of k_coarsen_plane: about one 4-column group per
@@ -328,36 +370,62 @@ __global__ void k_corr_alpha(const double *__restrict__ part, int n, double max_
   alpha[b] = (s2 < 0.0 && isfinite(a)) ? fmin(fmax(a, 0.0), max_step) : 0.0;
 }
 
-// x += sign * alpha * scale * bilinear(h) at the interior cells of an
-// instance's plane (sign -1 takes a correction back).
-__global__ void k_correct_apply(PhInst I, const float *__restrict__ hc,
-                                const float *__restrict__ scale,
-                                const double *__restrict__ alpha, int res, int x0, int y0, int f,
-                                double sign) {
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y;
-  if (px >= I.pw) return;
-  const size_t p = (size_t)py * I.pitch + px;
-  if (!I.cls[p]) return;
-  const double a = sign * alpha[0];
+// x += sign * alpha * scale * bilinear(h) at the interior cells of every
+// correction of a round, one block per plane row (span entry i takes args[i];
+// sign -1 takes a correction back).
+__global__ __launch_bounds__(256) void k_correct_apply_spans(const PhInst *__restrict__ insts,
+                                                             const PhSpan *__restrict__ spans,
+                                                             int n,
+                                                             const PhCorrArgs *__restrict__ args,
+                                                             const float *__restrict__ hc,
+                                                             const float *__restrict__ scale,
+                                                             const double *__restrict__ alpha,
+                                                             int res) {
+  const int b = blockIdx.x;
+  const int i = ph_span_find(spans, n, b);
+  const PhSpan S = spans[i];
+  const PhCorrArgs A = args[i];
+  const PhInst &I = insts[S.slot];
+  const double a = A.sign * alpha[A.net];
   if (a == 0.0) return;
-  I.x[p] += a * (double)scale[0] * warm_bilinear(hc, res, x0, y0, f, I.ox + px, I.oy + py);
+  const double sc = (double)scale[A.net];
+  const float *h = hc + (size_t)res * res * A.net;
+  const int py = b - S.row0;
+  for (int px = threadIdx.x; px < I.pw; px += 256) {
+    const size_t p = (size_t)py * I.pitch + px;
+    if (I.cls[p]) I.x[p] += a * sc * warm_bilinear(h, res, A.x0, A.y0, A.f, I.ox + px, I.oy + py);
+  }
 }
 
-// x = P + scale * bilinear(h) at the interior cells of an instance's plane (cell
-// centres: image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5,
-// clamped to the grid).
-__global__ void k_warm_apply(PhInst I, const float *__restrict__ src, const float *__restrict__ hc,
-                             const float *__restrict__ scale, int res, int x0, int y0, int f) {
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  const int py = blockIdx.y;
-  if (px >= I.pw) return;
-  const size_t p = (size_t)py * I.pitch + px;
-  if (!I.cls[p]) return;
-  const int ix = I.ox + px, iy = I.oy + py;
-  const double h = (double)scale[0] * warm_bilinear(hc, res, x0, y0, f, ix, iy);
-  const size_t q = (size_t)iy * I.img_w + ix;
-  I.x[p] = (double)src[q * I.channels + I.channel] + h;
+// x = P + scale * bilinear(h) at the interior cells of every listed network
+// instance, one block per plane row (span entry i takes args[i]; cell centres:
+// image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5, clamped
+// to the grid).
+__global__ __launch_bounds__(256) void k_warm_apply_spans(const PhInst *__restrict__ insts,
+                                                          const PhSpan *__restrict__ spans, int n,
+                                                          const PhCorrArgs *__restrict__ args,
+                                                          const PhItemIO *__restrict__ io,
+                                                          const float *__restrict__ hc,
+                                                          const float *__restrict__ scale,
+                                                          int res) {
+  const int b = blockIdx.x;
+  const int i = ph_span_find(spans, n, b);
+  const PhSpan S = spans[i];
+  const PhCorrArgs A = args[i];
+  const PhInst &I = insts[S.slot];
+  const float *src = io[A.item].src;
+  const float *h = hc + (size_t)res * res * A.net;
+  const double sc = (double)scale[A.net];
+  const int py = b - S.row0;
+  const int iy = I.oy + py;
+  for (int px = threadIdx.x; px < I.pw; px += 256) {
+    const size_t p = (size_t)py * I.pitch + px;
+    if (!I.cls[p]) continue;
+    const int ix = I.ox + px;
+    const double hv = sc * warm_bilinear(h, res, A.x0, A.y0, A.f, ix, iy);
+    const size_t q = (size_t)iy * I.img_w + ix;
+    I.x[p] = (double)src[q * I.channels + I.channel] + hv;
+  }
 }
 
 // x = the caller's start field at the interior cells of an instance's plane.

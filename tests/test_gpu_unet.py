@@ -239,14 +239,14 @@ def test_bad_weights_fall_back_to_the_plain_start(gpu, monkeypatch):
 
 
 def test_small_images_take_the_plain_start(gpu):
-    """Below the network start's size gate (2^18 interior cells per channel)
-    a channel takes the plain start: with the flag set and weights loaded the
-    result and history are the plain solve's, bit for bit, and the network is
-    not tried (warm_start_rel stays 0)."""
+    """Below the learned path's size gate (2^14 interior cells per channel
+    for a lone image) a channel takes the plain start: with the flag set and
+    weights loaded the result and history are the plain solve's, bit for bit,
+    and the network is not tried (warm_start_rel stays 0)."""
     from paintop_amd import warmstart
     from paintop_amd.device import DeviceArray
 
-    w = h = 256  # an ellipse of ~28k interior cells
+    w = h = 96  # an ellipse of ~3.5k interior cells
     s, t, m = _fbm_case(w, h, 1, 0.5, 960, "ellipse")
     o1, o2 = DeviceArray((h, w), np.float32), DeviceArray((h, w), np.float32)
     d = hip.make_desc(w, h, 1, 20000, 1e-6)
