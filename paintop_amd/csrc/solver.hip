@@ -1608,6 +1608,9 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           PhCtl &Cc = ctl[P.a];
           const double gnorm = resv[2 * act[P.a]];
           const double rel = gnorm / Cc.r0;  // the value k_residual_norm_spans wrote
+          if (tracing)
+            fprintf(stderr, "[refine] inst %d sweep %u: rel %.3e (halt %u, est %.3e)\n", act[P.a],
+                    P.m, rel, Cc.halt, Cc.last_rel);
           Cc.sweeps_final = P.m;
           Cc.sweeps_done = P.m;
           const int k = act[P.a];
