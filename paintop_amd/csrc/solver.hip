@@ -174,6 +174,34 @@ struct Pool {
 };
 static Pool g_pool;
 
+// Pinned host memory for the readbacks of the sweep loop (control blocks,
+// residual norms): a small copy to pageable memory goes through a staging
+// buffer and waits on it; to pinned memory it is one DMA. Host code must not
+// change a pinned buffer while a copy from it is queued.
+struct PinnedPool {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void reset(size_t need) {
+    if (need > cap) {
+      if (base) HIP_OK(hipHostFree(base));
+      base = nullptr;
+      cap = 0;
+      HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&base), need, hipHostMallocDefault));
+      cap = need;
+    }
+    used = 0;
+  }
+  template <class T>
+  T *take(size_t n) {
+    size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
+    if (used + bytes > cap) throw fail(PH_CLASS_EXECUTION, "E_HIP_WORKSPACE", "pinned staging overrun");
+    T *p = reinterpret_cast<T *>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+static PinnedPool g_pinned;
+
 // ---------------------------------------------------------------------------
 // parameter validation (the same ranges and codes as resolve_controls,
 // poisson_blend.rs:102-139 and screened_poisson.rs:66-112)
@@ -704,6 +732,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
               (nch + 2 * PH_NET_SLICES * (size_t)warm_net_batch()) * sizeof(double) + 1536;
     }
     g_pool.reset(need);
+    size_t nch = 0;
+    for (const ItemIn &it : items) nch += it.C;
+    g_pinned.reset(4096 + 2 * std::max<size_t>(nch, 1) * (2 * sizeof(double) + sizeof(PhCtl)) +
+                   2 * 256);
   }
   // host time per phase (printed with PAINTOP_HIP_TRACE=1)
   const bool phase_trace = trace_on();
@@ -962,7 +994,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int ninst = (int)inst.size();  // the images' instances (twins come after)
   const int ncap = std::max(race_any ? 2 * ninst : ninst, 1);
   double *resv_d = g_pool.take<double>(2 * (size_t)ncap);
-  std::vector<double> resv(2 * (size_t)ncap);
+  double *resv = g_pinned.take<double>(2 * (size_t)ncap);
   // assembly and the anchor-start residual of every instance: one launch per
   // step over all plane rows (inst_all_d is the instances' descriptors)
   PhInst *inst_all_d = g_pool.take<PhInst>(std::max(ninst, 1));
@@ -987,7 +1019,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     hipLaunchKernelGGL(k_residual_norm_spans, dim3(ninst), dim3(256), 0, st, inst_rows_d,
                        inst_sums_d, resv_d);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
+    HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
   }
@@ -1139,7 +1171,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     hipLaunchKernelGGL(k_residual_norm_spans, dim3(nw), dim3(256), 0, st, cand_spans_d,
                        cand_sums_d, resv_d);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
+    HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * ninst * sizeof(double), hipMemcpyDeviceToHost,
                           st));
     HIP_OK(hipStreamSynchronize(st));
     for (int j = 0; j < nw; ++j) {
@@ -1262,7 +1294,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     std::vector<char> running(nact, 1);
     build_tiles(running);
 
-    std::vector<PhCtl> ctl(nact);
+    PhCtl *ctl = g_pinned.take<PhCtl>(nact);
     for (int a = 0; a < nact; ++a) {
       const InstWork &X = inst[act[a]];
       const ItemWork &w = iw[X.item];
@@ -1283,7 +1315,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       Cc.refine_every = REFINE_EVERY;
       Cc.last_refine = 0;
     }
-    HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(ctl_d, ctl, nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
 
     auto finish = [&](int a, int stop, uint32_t m, double rel) {
       ChanResult &R = inst[act[a]].R;
@@ -1430,7 +1462,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       std::vector<double> alpha(nc);
       HIP_OK(hipMemcpyAsync(alpha.data(), corr_alpha, nc * sizeof(double), hipMemcpyDeviceToHost,
                             st));
-      HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
+      HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * inst.size() * sizeof(double),
                             hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
       std::vector<char> undo(nc, 0);
@@ -1460,7 +1492,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         hipLaunchKernelGGL(k_residual_norm_spans, dim3(nu), dim3(256), 0, st, undo_spans_d,
                            ref_sums_d, resv_d);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
+        HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * inst.size() * sizeof(double),
                               hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
       }
@@ -1525,7 +1557,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       launch_sor_batch(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb, batch);
       total_launches += (uint32_t)batch;
       if (timing) HIP_OK(hipEventRecord(eb1, st));
-      HIP_OK(hipMemcpyAsync(ctl.data(), ctl_d, nact * sizeof(PhCtl), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipMemcpyAsync(ctl, ctl_d, nact * sizeof(PhCtl), hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
       if (timing) {
         float bms = 0.f;
@@ -1597,7 +1629,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                            ref_sums_d, resv_d);
         HIP_OK(hipGetLastError());
         // every instance's slot, the warm-start race's plain twins included
-        HIP_OK(hipMemcpyAsync(resv.data(), resv_d, 2 * inst.size() * sizeof(double),
+        HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * inst.size() * sizeof(double),
                               hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         total_refine += (uint32_t)pend.size();
@@ -1669,7 +1701,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           if (warm_wins) iw[inst[k].item].in.rep->warm_start_used += 1;
         }
       }
-      HIP_OK(hipMemcpyAsync(ctl_d, ctl.data(), nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(ctl_d, ctl, nact * sizeof(PhCtl), hipMemcpyHostToDevice, st));
       if (remaining == 0) break;
       // drop the tiles of finished instances once they are most of the list
       {
