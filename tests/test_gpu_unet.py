@@ -466,6 +466,58 @@ def test_corrections_in_a_batched_call(gpu):
         assert float(np.abs(a - b).max()) < 2e-4
 
 
+def test_batch_gate_is_relative_to_the_largest_image(gpu):
+    """In one batched call with the learned path, an image far smaller than the
+    call's largest (below max(2^14, largest / 8) interior cells per channel)
+    takes the plain start: its output, history and report equal the plain
+    batched call's, bit for bit, while the large image keeps its warm start and
+    corrections. The same small image alone passes the gate (2^14) and takes
+    the learned path."""
+    from paintop_amd import warmstart
+    from paintop_amd.device import DeviceArray
+
+    # interior cells per channel: ~473k, ~24.6k, ~25.9k (gate in the batch: ~59k)
+    dims = [(1024, 1024, 3), (256, 224, 1), (224, 256, 2)]
+    cases = [_fbm_case(1024, 1024, 3, 0.125, 830, "polygon"),
+             _fbm_case(256, 224, 1, 0.3, 840, "ellipse"),
+             _fbm_case(224, 256, 2, 0.3, 850, "polygon")]
+    d = hip.make_desc(0, 0, 1, 100000, 1e-6)
+
+    def batch(flags):
+        outs = [DeviceArray(t_.shape, np.float32) for (_s, t_, _m) in cases]
+        items = [(w, h, c, s_.ptr, t_.ptr, m_.ptr, o.ptr)
+                 for (w, h, c), (s_, t_, m_), o in zip(dims, cases, outs)]
+        d.flags = flags
+        return outs, gpu.poisson_solve_batch_device(d, items)
+
+    outs_p, reps_p = batch(0)
+    warmstart.enable()
+    try:
+        outs_l, reps_l = batch(hip.PH_FLAG_WARM_START_NET)
+        w, h, c = dims[1]
+        s_, t_, m_ = cases[1]
+        o = DeviceArray(t_.shape, np.float32)
+        alone = gpu.poisson_solve_device(
+            hip.make_desc(w, h, c, 100000, 1e-6, flags=hip.PH_FLAG_WARM_START_NET),
+            s_.ptr, t_.ptr, m_.ptr, o.ptr)
+    finally:
+        gpu.warm_start_set_weights(None)
+    for k, ((w, h, c), rp, rl) in enumerate(zip(dims, reps_p, reps_l)):
+        print(f"[gate] {w}x{h}x{c}: plain {rp.iterations}, learned {rl.iterations} "
+              f"(kept {rl.warm_start_used}, corrections {rl.coarse_corrections})", flush=True)
+    big = reps_l[0]
+    assert big.converged and big.warm_start_used > 0 and big.coarse_corrections > 0
+    assert big.iterations < reps_p[0].iterations
+    for k in (1, 2):
+        assert reps_l[k].warm_start_used == 0 and reps_l[k].warm_start_rel == 0.0, k
+        assert outs_l[k].numpy().tobytes() == outs_p[k].numpy().tobytes(), k
+        assert reps_l[k].residual_history == reps_p[k].residual_history, k
+        assert reps_l[k].iterations == reps_p[k].iterations, k
+    print(f"[gate] 256x224x1 alone: learned {alone.iterations} (kept {alone.warm_start_used})",
+          flush=True)
+    assert alone.converged and alone.warm_start_rel > 0.0
+
+
 def test_blend_split_into_mask_components(gpu):
     """components.py: the blend solved per 4-connected mask component in one
     batched call equals the whole-mask solve (both at 1e-9: within 1e-6), keeps
