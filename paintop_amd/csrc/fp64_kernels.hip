@@ -152,58 +152,96 @@ __global__ void k_coarse_count_all(const PhItemGeo *__restrict__ items, int *__r
   counts[G.coarse_off + c] = n;
 }
 
-// Fill plane cell (px, py) of an instance from the interleaved image channel.
-__device__ __forceinline__ void assemble_cell(const PhInst &I, const float *__restrict__ src,
-                                              const float *__restrict__ tgt,
-                                              const uint8_t *__restrict__ cls_img,
-                                              uint8_t *__restrict__ cls, int px, int py) {
-  const size_t p = (size_t)py * I.pitch + px;
-  const int ix = I.ox + px, iy = I.oy + py;
-  const int W = I.img_w, H = I.img_h, C = I.channels, ch = I.channel;
-  double a = 0.0, bp = 0.0;
-  uint8_t in = 0;
-  if (px < I.pw && ix >= 0 && ix < W && iy >= 0 && iy < H) {
-    const size_t q = (size_t)iy * W + ix;
-    a = (double)tgt[q * C + ch];
-    in = cls_img[q];
-    if (in) {
-      // interior cells are never on the image border: four in-image neighbours
-      const double cc = (double)src[q * C + ch];
-      const double l = (double)src[(q - 1) * C + ch];
-      const double r = (double)src[(q + 1) * C + ch];
-      const double u = (double)src[(q - W) * C + ch];
-      const double d = (double)src[(q + W) * C + ch];
-      const double b = fma(4.0, -cc, ((l + r) + u) + d);
-      bp = fma(I.lambda, -a, b);
-    }
-  }
-  I.x[p] = a;
-  I.bp[p] = bp;
-  cls[p] = in;
-  I.e[0][p] = 0.0f;
-  I.e[1][p] = 0.0f;
-  I.g[p] = __builtin_nanf("");
-}
-
-// Fill an instance's planes (grid: pitch / 256 x plane rows).
-__global__ void k_assemble(PhInst I, const float *__restrict__ src, const float *__restrict__ tgt,
-                           const uint8_t *__restrict__ cls_img, uint8_t *__restrict__ cls) {
-  const int px = blockIdx.x * blockDim.x + threadIdx.x;
-  if (px < I.pitch) assemble_cell(I, src, tgt, cls_img, cls, px, blockIdx.y);
-}
-
-// Fill the planes of every listed instance, one block per plane row.
+// Fill the planes of the listed images' channel instances, one block per plane
+// row (spans: slot = the first of the image's C consecutive descriptors, one
+// per channel, a descriptor with no planes (x == nullptr) is skipped; item =
+// the image). Each pixel's channels are read together, so the interleaved
+// source and target are read once per image, not once per channel.
+// b = fma(4, -c, ((l + r) + u) + d) of the source (poisson.rs:364-373),
+// b' = fma(lambda, -anchor, b) (poisson.rs:319); x starts at the anchor.
+// CC = C for 1..4 (the descriptors' pointers then sit in registers), 0 for
+// any C (read per cell).
+template <int CC>
 __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict__ insts,
                                                         const PhSpan *__restrict__ spans, int n,
                                                         const PhItemIO *__restrict__ io) {
   const int b = blockIdx.x;
   const PhSpan S = spans[ph_span_find(spans, n, b)];
-  const PhInst &I = insts[S.slot];
   const PhItemIO &G = io[S.item];
+  const PhInst I0 = insts[S.slot];  // the plane geometry, shared by the channels
   const int py = b - S.row0;
-  for (int px = threadIdx.x; px < I.pitch; px += 256)
-    assemble_cell(I, G.src, G.tgt, G.cls_img, G.cls_plane, px, py);
+  const int W = G.W, H = G.H, C = CC ? CC : G.C;
+  const float *__restrict__ src = G.src;
+  const float *__restrict__ tgt = G.tgt;
+  const uint8_t *__restrict__ cls_img = G.cls_img;
+  uint8_t *__restrict__ cls_plane = G.cls_plane;
+  constexpr int R = CC ? CC : 1;
+  double *xs[R], *bps[R];
+  float *e0s[R], *e1s[R], *gs[R];
+  if (CC) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      const PhInst &I = insts[S.slot + c];
+      xs[c] = I.x;
+      bps[c] = I.bp;
+      e0s[c] = I.e[0];
+      e1s[c] = I.e[1];
+      gs[c] = I.g;
+    }
+  }
+  const double lambda = I0.lambda;
+  for (int px = threadIdx.x; px < I0.pitch; px += 256) {
+    const size_t p = (size_t)py * I0.pitch + px;
+    const int ix = I0.ox + px, iy = I0.oy + py;
+    const bool inside = px < I0.pw && ix >= 0 && ix < W && iy >= 0 && iy < H;
+    const size_t q = inside ? (size_t)iy * W + ix : 0;
+    const uint8_t in = inside ? cls_img[q] : 0;
+    cls_plane[p] = in;
+#pragma unroll
+    for (int c = 0; c < (CC ? CC : 64); ++c) {
+      if (!CC && c >= C) break;
+      double *x = CC ? xs[c] : insts[S.slot + c].x;
+      if (!x) continue;
+      double a = 0.0, bp = 0.0;
+      if (inside) {
+        a = (double)tgt[q * C + c];
+        if (in) {
+          // interior cells are never on the image border: four in-image neighbours
+          const double cc = (double)src[q * C + c];
+          const double l = (double)src[(q - 1) * C + c];
+          const double r = (double)src[(q + 1) * C + c];
+          const double u = (double)src[(q - W) * C + c];
+          const double d = (double)src[(q + W) * C + c];
+          const double bb = fma(4.0, -cc, ((l + r) + u) + d);
+          bp = fma(lambda, -a, bb);
+        }
+      }
+      x[p] = a;
+      if (CC) {
+        bps[c][p] = bp;
+        e0s[c][p] = 0.0f;
+        e1s[c][p] = 0.0f;
+        gs[c][p] = __builtin_nanf("");
+      } else {
+        const PhInst &I = insts[S.slot + c];
+        I.bp[p] = bp;
+        I.e[0][p] = 0.0f;
+        I.e[1][p] = 0.0f;
+        I.g[p] = __builtin_nanf("");
+      }
+    }
+  }
 }
+template __global__ void k_assemble_spans<0>(const PhInst *, const PhSpan *, int,
+                                             const PhItemIO *);
+template __global__ void k_assemble_spans<1>(const PhInst *, const PhSpan *, int,
+                                             const PhItemIO *);
+template __global__ void k_assemble_spans<2>(const PhInst *, const PhSpan *, int,
+                                             const PhItemIO *);
+template __global__ void k_assemble_spans<3>(const PhInst *, const PhSpan *, int,
+                                             const PhItemIO *);
+template __global__ void k_assemble_spans<4>(const PhInst *, const PhSpan *, int,
+                                             const PhItemIO *);
 
 // g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and this
 // thread's part of the row's sum of r^2 (the block adds the parts in a fixed
@@ -312,24 +350,45 @@ __global__ __launch_bounds__(256) void k_residual_norm_spans(const PhSpan *__res
   }
 }
 
-// Write the solved interior cells of every listed instance into its image's
-// candidate, one block per plane row.
+// Write the solved interior cells of the listed images into their candidates,
+// one block per plane row (spans as for k_assemble_spans: C consecutive
+// descriptors per image, one per channel, those with no planes skipped), all
+// channels of a pixel by one thread, so each touched line is written once.
+// CC as for k_assemble_spans.
+template <int CC>
 __global__ __launch_bounds__(256) void k_scatter_spans(const PhInst *__restrict__ insts,
                                                        const PhSpan *__restrict__ spans, int n,
                                                        const PhItemIO *__restrict__ io) {
   const int b = blockIdx.x;
   const PhSpan S = spans[ph_span_find(spans, n, b)];
-  const PhInst &I = insts[S.slot];
-  float *out = io[S.item].out;
+  const PhItemIO &G = io[S.item];
+  const PhInst I0 = insts[S.slot];
   const int py = b - S.row0;
-  for (int px = threadIdx.x; px < I.pw; px += 256) {
-    const size_t p = (size_t)py * I.pitch + px;
-    if (I.cls[p]) {
-      const size_t q = (size_t)(I.oy + py) * I.img_w + (I.ox + px);
-      out[q * I.channels + I.channel] = (float)I.x[p];
+  const int C = CC ? CC : G.C;
+  float *__restrict__ out = G.out;
+  constexpr int R = CC ? CC : 1;
+  const double *xs[R];
+  if (CC) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) xs[c] = insts[S.slot + c].x;
+  }
+  for (int px = threadIdx.x; px < I0.pw; px += 256) {
+    const size_t p = (size_t)py * I0.pitch + px;
+    if (!I0.cls[p]) continue;
+    const size_t q = (size_t)(I0.oy + py) * I0.img_w + (I0.ox + px);
+#pragma unroll
+    for (int c = 0; c < (CC ? CC : 64); ++c) {
+      if (!CC && c >= C) break;
+      const double *x = CC ? xs[c] : insts[S.slot + c].x;
+      if (x) out[q * C + c] = (float)x[p];
     }
   }
 }
+template __global__ void k_scatter_spans<0>(const PhInst *, const PhSpan *, int, const PhItemIO *);
+template __global__ void k_scatter_spans<1>(const PhInst *, const PhSpan *, int, const PhItemIO *);
+template __global__ void k_scatter_spans<2>(const PhInst *, const PhSpan *, int, const PhItemIO *);
+template __global__ void k_scatter_spans<3>(const PhInst *, const PhSpan *, int, const PhItemIO *);
+template __global__ void k_scatter_spans<4>(const PhInst *, const PhSpan *, int, const PhItemIO *);
 
 // Write n values to their device addresses (one launch for many small updates).
 __global__ void k_put_values(const PhValue *__restrict__ v, int n) {

@@ -50,12 +50,13 @@ __global__ void k_bbox_spans(const PhSpan *, const int2 *, int *);
 __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 *, int, int, int,
                                      int *);
 __global__ void k_coarse_count_all(const PhItemGeo *, int *);
-__global__ void k_assemble(PhInst, const float *, const float *, const uint8_t *, uint8_t *);
+template <int CC>
 __global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
 __global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
 __global__ void k_residual_spans(const PhInst *, const PhSpan *, int, double *);
 __global__ void k_residual_norm_spans(const PhSpan *, const double *, double *);
 __global__ void k_scatter(PhInst, float *);
+template <int CC>
 __global__ void k_scatter_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
 __global__ void k_gather_hist(const PhHistCopy *, double *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
@@ -678,8 +679,8 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
   return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) +
          ntile_up * (sizeof(int2) + 2 * sizeof(int)) + sizeof(PhItemGeo) +
-         (size_t)C * 5 * ph_up * sizeof(double) + (size_t)(10 * C + 2) * sizeof(PhSpan) +
-         (size_t)C * (2 * sizeof(PhInst) + 4 * sizeof(PhCorrArgs) + sizeof(PhValue)) +
+         (size_t)C * 5 * ph_up * sizeof(double) + (size_t)(12 * C + 4) * sizeof(PhSpan) +
+         (size_t)C * (3 * sizeof(PhInst) + 4 * sizeof(PhCorrArgs) + sizeof(PhValue)) +
          sizeof(PhItemIO) + sizeof(PhHistCopy) + (size_t)(d->max_iterations + 2) * sizeof(double) +
          plane_up * (1 + (size_t)C * (3 * sizeof(float) + 2 * sizeof(double))) +
          (size_t)(C + 1) * (d->max_iterations + 2) * sizeof(double) +
@@ -688,6 +689,43 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
          (size_t)C * (sizeof(PhInst) + sizeof(PhCtl) + 256) + 40 * 256 +
          130 * 130 * sizeof(int);
 }
+
+// Per-image launches (assembly, write-back) specialise on the channel count
+// when every listed image has the same one (1..4); 0 = any.
+static int common_channels(const SpanTable &t, const std::vector<ItemWork> &iw) {
+  int c = -1;
+  for (const PhSpan &e : t.h) {
+    const int ci = iw[e.item].in.C;
+    if (c < 0) c = ci;
+    else if (c != ci) return 0;
+  }
+  return c >= 1 && c <= 4 ? c : 0;
+}
+
+template <template <int> class L, class... A>
+static void launch_by_channels(int cc, A... args) {
+  switch (cc) {
+    case 1: L<1>::go(args...); break;
+    case 2: L<2>::go(args...); break;
+    case 3: L<3>::go(args...); break;
+    case 4: L<4>::go(args...); break;
+    default: L<0>::go(args...); break;
+  }
+}
+template <int CC>
+struct AssembleL {
+  static void go(dim3 g, hipStream_t st, const PhInst *i, const PhSpan *sp, int n,
+                 const PhItemIO *io) {
+    hipLaunchKernelGGL(k_assemble_spans<CC>, g, dim3(256), 0, st, i, sp, n, io);
+  }
+};
+template <int CC>
+struct ScatterL {
+  static void go(dim3 g, hipStream_t st, const PhInst *i, const PhSpan *sp, int n,
+                 const PhItemIO *io) {
+    hipLaunchKernelGGL(k_scatter_spans<CC>, g, dim3(256), 0, st, i, sp, n, io);
+  }
+};
 
 static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipStream_t st) {
   const int nitem = (int)items.size();
@@ -999,12 +1037,15 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   // step over all plane rows (inst_all_d is the instances' descriptors)
   PhInst *inst_all_d = g_pool.take<PhInst>(std::max(ninst, 1));
   PhSpan *inst_rows_d = g_pool.take<PhSpan>(std::max(ninst, 1));
-  SpanTable inst_rows;
+  PhSpan *img_planes_d = g_pool.take<PhSpan>(std::max(nitem, 1));
+  SpanTable inst_rows, img_planes;
   std::vector<PhInst> inst_all(ninst);
   for (int k = 0; k < ninst; ++k) {
     inst_all[k] = inst[k].I;
     inst_rows.add(k, inst[k].I.ph, inst[k].item, 0, k);
   }
+  for (int i = 0; i < nitem; ++i)
+    if (iw[i].any) img_planes.add(iw[i].inst0, iw[i].phh, i);
   if (ninst) {
     for (int i = 0; i < nitem; ++i) io[i].cls_plane = iw[i].cls_plane;
     double *inst_sums_d = g_pool.take<double>((size_t)inst_rows.rows);
@@ -1012,8 +1053,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     HIP_OK(hipMemcpyAsync(inst_all_d, inst_all.data(), ninst * sizeof(PhInst),
                           hipMemcpyHostToDevice, st));
     inst_rows.upload(inst_rows_d, st);
-    hipLaunchKernelGGL(k_assemble_spans, dim3(inst_rows.rows), dim3(256), 0, st, inst_all_d,
-                       inst_rows_d, ninst, io_d);
+    img_planes.upload(img_planes_d, st);
+    launch_by_channels<AssembleL>(common_channels(img_planes, iw), dim3(img_planes.rows), st,
+                                  (const PhInst *)inst_all_d, (const PhSpan *)img_planes_d,
+                                  (int)img_planes.h.size(), (const PhItemIO *)io_d);
     hipLaunchKernelGGL(k_residual_spans, dim3(inst_rows.rows), dim3(256), 0, st, inst_all_d,
                        inst_rows_d, ninst, inst_sums_d);
     hipLaunchKernelGGL(k_residual_norm_spans, dim3(ninst), dim3(256), 0, st, inst_rows_d,
@@ -1112,8 +1155,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const uint64_t corr_min_cells = warm_gate("PAINTOP_HIP_CORR_MIN_CELLS", largest);
   // host staging of the warm start (kept until the stream is synchronised)
   std::vector<PhCorrArgs> warm_args;
-  std::vector<PhInst> re_inst;
-  SpanTable warm_rows, cand_rows, re_rows;
+  std::vector<PhInst> re_inst, re_group;
+  SpanTable warm_rows, cand_rows, re_rows, re_planes;
   if (!cand.empty()) {
     const int nw = (int)cand.size();
     if (!net_list.empty()) {
@@ -1228,13 +1271,34 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     // their assembly and FP64 residual in three launches
     if (!re_inst.empty()) {
       const int nr = (int)re_inst.size();
+      // the assembly takes each image's channels together: C consecutive
+      // descriptors per image, with no planes where a channel is not refilled
+      std::vector<int> group(nitem, -1);
+      for (int r = 0; r < nr; ++r) {
+        const int i = re_rows.h[r].item;
+        if (group[i] < 0) {
+          group[i] = (int)re_group.size();
+          PhInst blank = re_inst[r];
+          blank.x = nullptr;
+          re_group.insert(re_group.end(), iw[i].in.C, blank);
+          re_planes.add(group[i], re_inst[r].ph, i);
+        }
+        re_group[group[i] + re_inst[r].channel] = re_inst[r];
+      }
+      const int ng = (int)re_group.size();
       PhInst *re_d = g_pool.take<PhInst>(nr);
+      PhInst *re_group_d = g_pool.take<PhInst>(ng);
       PhSpan *re_spans_d = g_pool.take<PhSpan>(nr);
+      PhSpan *re_planes_d = g_pool.take<PhSpan>(re_planes.h.size());
       double *re_sums_d = g_pool.take<double>((size_t)re_rows.rows);
       HIP_OK(hipMemcpyAsync(re_d, re_inst.data(), nr * sizeof(PhInst), hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(re_group_d, re_group.data(), ng * sizeof(PhInst),
+                            hipMemcpyHostToDevice, st));
       re_rows.upload(re_spans_d, st);
-      hipLaunchKernelGGL(k_assemble_spans, dim3(re_rows.rows), dim3(256), 0, st, re_d, re_spans_d,
-                         nr, io_d);
+      re_planes.upload(re_planes_d, st);
+      launch_by_channels<AssembleL>(common_channels(re_planes, iw), dim3(re_planes.rows), st,
+                                    (const PhInst *)re_group_d, (const PhSpan *)re_planes_d,
+                                    (int)re_planes.h.size(), (const PhItemIO *)io_d);
       hipLaunchKernelGGL(k_residual_spans, dim3(re_rows.rows), dim3(256), 0, st, re_d, re_spans_d,
                          nr, re_sums_d);
       hipLaunchKernelGGL(k_residual_norm_spans, dim3(nr), dim3(256), 0, st, re_spans_d, re_sums_d,
@@ -1780,21 +1844,31 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     }
     // write-back of every solved instance in one launch (a plain twin's
     // planes stand in for its image's instance where the twin won)
+    // (each image's channels together: C consecutive descriptors per image,
+    // with no planes for the channels that were not solved)
     SpanTable out_rows;
     std::vector<PhInst> out_inst;
+    std::vector<int> group(nitem, -1);
     for (int a = 0; a < nact; ++a) {
       const int k = act[a];
       if (k >= ninst || !inst[k].R.active) continue;
-      out_rows.add((int)out_inst.size(), inst[k].I.ph, inst[k].item);
-      out_inst.push_back(inst[k].I);
+      const int i = inst[k].item;
+      if (group[i] < 0) {
+        group[i] = (int)out_inst.size();
+        PhInst blank = inst[k].I;
+        blank.x = nullptr;
+        out_inst.insert(out_inst.end(), iw[i].in.C, blank);
+        out_rows.add(group[i], inst[k].I.ph, i);
+      }
+      out_inst[group[i] + inst[k].channel] = inst[k].I;
     }
     if (!out_inst.empty()) {
-      const int nout = (int)out_inst.size();
-      HIP_OK(hipMemcpyAsync(inst_all_d, out_inst.data(), nout * sizeof(PhInst),
+      HIP_OK(hipMemcpyAsync(inst_all_d, out_inst.data(), out_inst.size() * sizeof(PhInst),
                             hipMemcpyHostToDevice, st));
       out_rows.upload(inst_rows_d, st);
-      hipLaunchKernelGGL(k_scatter_spans, dim3(out_rows.rows), dim3(256), 0, st, inst_all_d,
-                         inst_rows_d, nout, io_d);
+      launch_by_channels<ScatterL>(common_channels(out_rows, iw), dim3(out_rows.rows), st,
+                                   (const PhInst *)inst_all_d, (const PhSpan *)inst_rows_d,
+                                   (int)out_rows.h.size(), (const PhItemIO *)io_d);
       HIP_OK(hipGetLastError());
       HIP_OK(hipStreamSynchronize(st));  // out_rows and out_inst are host staging
     }
