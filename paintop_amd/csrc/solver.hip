@@ -61,8 +61,8 @@ __global__ void k_scatter_spans(const PhInst *, const PhSpan *, int, const PhIte
 __global__ void k_gather_hist(const PhHistCopy *, double *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
                           int, double, int, int, int, int, int, float *, float *, float *);
-__global__ void k_coarsen_list(const PhInst *, const PhCorrArgs *, const PhItemIO *, double, int,
-                               float *);
+__global__ void k_coarsen_groups(const PhInst *, const PhCorrArgs *, const PhSpan *,
+                                 const PhItemIO *, double, int, int, float *);
 __global__ void k_warm_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
                                    const PhItemIO *, const float *, const float *, int);
 __global__ void k_coarsen_plane(PhInst, int, int, int, int, int, float *);
@@ -1156,7 +1156,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   // host staging of the warm start (kept until the stream is synchronised)
   std::vector<PhCorrArgs> warm_args;
   std::vector<PhInst> re_inst, re_group;
-  SpanTable warm_rows, cand_rows, re_rows, re_planes;
+  SpanTable warm_rows, warm_groups, cand_rows, re_rows, re_planes;
   if (!cand.empty()) {
     const int nw = (int)cand.size();
     if (!net_list.empty()) {
@@ -1176,15 +1176,33 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         warm_args[j] = PhCorrArgs{k, g.x0, g.y0, g.f, 0, j, inst[k].item, 0, 1.0};
         warm_rows.add(k, inst[k].I.ph, inst[k].item);
       }
+      // the coarsening takes an image's channels together: groups of up to 4
+      // consecutive entries of one image inside one network batch
+      warm_groups.clear();
+      std::vector<int> group0;  // first group of each batch
+      for (int j0 = 0; j0 < nn; j0 += NNB) {
+        group0.push_back((int)warm_groups.h.size());
+        const int j1 = std::min(nn, j0 + NNB);
+        for (int j = j0; j < j1;) {
+          int e = j + 1;
+          while (e < j1 && e - j < 4 && warm_args[e].item == warm_args[j].item) ++e;
+          warm_groups.h.push_back(PhSpan{j, 0, e - j, 0, 0, warm_args[j].item, nullptr, 0.0});
+          j = e;
+        }
+      }
+      group0.push_back((int)warm_groups.h.size());
       PhCorrArgs *warm_args_d = g_pool.take<PhCorrArgs>(nn);
       PhSpan *warm_spans_d = g_pool.take<PhSpan>(nn);
+      PhSpan *warm_groups_d = g_pool.take<PhSpan>(nn);
       HIP_OK(hipMemcpyAsync(warm_args_d, warm_args.data(), nn * sizeof(PhCorrArgs),
                             hipMemcpyHostToDevice, st));
       warm_rows.upload(warm_spans_d, st);
-      for (int j0 = 0; j0 < nn; j0 += NNB) {
+      warm_groups.upload(warm_groups_d, st);
+      for (int j0 = 0, bi = 0; j0 < nn; j0 += NNB, ++bi) {
         const int nb = std::min(NNB, nn - j0);
-        hipLaunchKernelGGL(k_coarsen_list, dim3(NR * NR, nb), dim3(64), 0, st, inst_all_d,
-                           warm_args_d + j0, io_d, d->lambda, NR, net_in);
+        hipLaunchKernelGGL(k_coarsen_groups, dim3(NR * NR, group0[bi + 1] - group0[bi]), dim3(64),
+                           0, st, inst_all_d, warm_args_d, warm_groups_d + group0[bi], io_d,
+                           d->lambda, NR, j0, net_in);
         hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
                            net_part);
         hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,

@@ -109,18 +109,81 @@ __global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
                blockIdx.x);
 }
 
-// The warm-start inputs (mode 0) of every listed network instance in one
-// launch: blockIdx.y = entry j, whose input goes to in4 + res*res*4*j.
-__global__ __launch_bounds__(64) void k_coarsen_list(const PhInst *__restrict__ insts,
-                                                     const PhCorrArgs *__restrict__ args,
-                                                     const PhItemIO *__restrict__ io,
-                                                     double lambda, int res,
-                                                     float *__restrict__ in4) {
-  const PhCorrArgs A = args[blockIdx.y];
-  const PhItemIO &G = io[A.item];
-  coarsen_cell(G.src, G.tgt, G.mask, nullptr, G.W, G.H, G.C, insts[A.slot].channel, lambda, A.x0,
-               A.y0, A.f, res, 0, in4 + (size_t)res * res * 4 * blockIdx.y, nullptr, nullptr,
-               blockIdx.x);
+// The warm-start inputs (mode 0) of every network instance of a batch in one
+// launch, the channels of an image together: blockIdx.y = group, a run of up
+// to 4 consecutive entries args[slot .. slot + nrows) of one image (item), so
+// each pixel's channels and its interior and ring tests are read once for the
+// group. Per channel the sums run in k_coarsen's order (lane-local, then the
+// butterfly), so the inputs equal k_coarsen's. Entry j's input goes to
+// in4 + res*res*4*(net - j0).
+__global__ __launch_bounds__(64) void k_coarsen_groups(const PhInst *__restrict__ insts,
+                                                       const PhCorrArgs *__restrict__ args,
+                                                       const PhSpan *__restrict__ groups,
+                                                       const PhItemIO *__restrict__ io,
+                                                       double lambda, int res, int j0,
+                                                       float *__restrict__ in4) {
+  const PhSpan Gs = groups[blockIdx.y];
+  const PhItemIO &G = io[Gs.item];
+  const int ne = Gs.nrows;
+  const PhCorrArgs A0 = args[Gs.slot];
+  const int x0 = A0.x0, y0 = A0.y0, f = A0.f;
+  const int W = G.W, H = G.H, C = G.C;
+  const float *__restrict__ src = G.src;
+  const float *__restrict__ anc = G.tgt;
+  const float *__restrict__ mask = G.mask;
+  int ch[4] = {0, 0, 0, 0}, net[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    if (e < ne) {
+      const PhCorrArgs A = args[Gs.slot + e];
+      ch[e] = insts[A.slot].channel;
+      net[e] = A.net;
+    }
+  const int c = blockIdx.x;
+  const int cx = c % res, cy = c / res;
+  const int lane = threadIdx.x;
+  double s_rhs[4] = {0.0, 0.0, 0.0, 0.0}, s_ring[4] = {0.0, 0.0, 0.0, 0.0};
+  double n_int = 0.0, n_ring = 0.0;
+  for (int i = lane; i < f * f; i += 64) {
+    const int dy = i / f, dx = i - dy * f;
+    const int iy = y0 + cy * f + dy;
+    const int ix = x0 + cx * f + dx;
+    if (iy < 0 || iy >= H || ix < 0 || ix >= W) continue;
+    const size_t q = (size_t)iy * W + ix;
+    if (warm_interior(mask, W, H, ix, iy)) {
+      n_int += 1.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (e < ne) {
+          const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
+          s_rhs[e] += lambda * (S - A);
+        }
+    } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
+               warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
+      n_ring += 1.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (e < ne) {
+          const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
+          s_ring[e] += A - S;
+        }
+    }
+  }
+  n_int = wave_sum_d(n_int);
+  n_ring = wave_sum_d(n_ring);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (e >= ne) break;
+    const double sr = wave_sum_d(s_rhs[e]);
+    const double sg = wave_sum_d(s_ring[e]);
+    if (lane == 0) {
+      float *o = in4 + ((size_t)(net[e] - j0) * res * res + c) * 4;
+      o[0] = (float)sr;  // f^2 * mean
+      o[1] = n_ring > 0.0 ? (float)(sg / n_ring) : 0.0f;
+      o[2] = (float)(n_int * (1.0 / ((double)f * f)));
+      o[3] = (float)(n_ring / f);
+    }
+  }
 }
 
 // Net input of a coarse correction, from an instance's plane: per block the
