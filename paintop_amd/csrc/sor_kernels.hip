@@ -317,6 +317,9 @@ __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict
   const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double v = 0.0;
   const double *ps = partials + (size_t)s * ntiles;
+  // unrolled so the loads of several tiles are in flight at once; the adds
+  // keep their order
+#pragma unroll 8
   for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += ps[t];
   v = wave_sum(v);
   if (lane == 0) A[s] = v;
