@@ -390,6 +390,30 @@ template __global__ void k_scatter_spans<2>(const PhInst *, const PhSpan *, int,
 template __global__ void k_scatter_spans<3>(const PhInst *, const PhSpan *, int, const PhItemIO *);
 template __global__ void k_scatter_spans<4>(const PhInst *, const PhSpan *, int, const PhItemIO *);
 
+// Copy the anchor-start state of every listed instance (x, g) into the planes
+// of its copy and clear the copy's corrections, one block per plane row (span
+// entry i: from[i] -> to[i]). Taken before a warm start overwrites x and g, the
+// copy is the plain start of the warm-start race, equal to a fresh assembly
+// and residual bit for bit.
+__global__ __launch_bounds__(256) void k_copy_anchor_spans(const PhInst *__restrict__ from,
+                                                           const PhInst *__restrict__ to,
+                                                           const PhSpan *__restrict__ spans,
+                                                           int n) {
+  const int b = blockIdx.x;
+  const int i = ph_span_find(spans, n, b);
+  const PhSpan S = spans[i];
+  const PhInst &A = from[i];
+  const PhInst &B = to[i];
+  const size_t row = (size_t)(b - S.row0) * A.pitch;
+  for (int px = threadIdx.x; px < A.pitch; px += 256) {
+    const size_t p = row + px;
+    B.x[p] = A.x[p];
+    B.g[p] = A.g[p];
+    B.e[0][p] = 0.0f;
+    B.e[1][p] = 0.0f;
+  }
+}
+
 // Write n values to their device addresses (one launch for many small updates).
 __global__ void k_put_values(const PhValue *__restrict__ v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -70,6 +70,7 @@ __global__ void k_coarsen_plane_list(const PhInst *, const PhCorrArgs *, int, fl
 __global__ void k_correct_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
                                       const float *, const float *, const double *, int);
 __global__ void k_put_values(const PhValue *, int);
+__global__ void k_copy_anchor_spans(const PhInst *, const PhInst *, const PhSpan *, int);
 int coarsen_plane_cpb(int f);
 __global__ void k_in_norm_part(const float *, int, double *);
 __global__ void k_in_norm_apply(float *, int, const double *, float *);
@@ -1155,10 +1156,38 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const uint64_t corr_min_cells = warm_gate("PAINTOP_HIP_CORR_MIN_CELLS", largest);
   // host staging of the warm start (kept until the stream is synchronised)
   std::vector<PhCorrArgs> warm_args;
-  std::vector<PhInst> re_inst, re_group;
-  SpanTable warm_rows, warm_groups, cand_rows, re_rows, re_planes;
+  std::vector<PhInst> anchor_from, anchor_to;
+  SpanTable warm_rows, warm_groups, cand_rows, anchor_rows;
   if (!cand.empty()) {
     const int nw = (int)cand.size();
+    // the anchor-start state of every candidate, copied before its start
+    // overwrites it: the plain twin of the race, or the way back to the anchor
+    // start for a start rejected outright (the source term b' is shared)
+    for (int j = 0; j < nw; ++j) {
+      const InstWork &X = inst[cand[j]];
+      const size_t plane = (size_t)X.I.pitch * X.I.ph;
+      PhInst A = X.I;
+      A.e[0] = g_pool.take<float>(plane);
+      A.e[1] = g_pool.take<float>(plane);
+      A.g = g_pool.take<float>(plane);
+      A.x = g_pool.take<double>(plane);
+      anchor_from.push_back(X.I);
+      anchor_to.push_back(A);
+      anchor_rows.add(j, X.I.ph, X.item);
+    }
+    {
+      PhInst *from_d = g_pool.take<PhInst>(nw);
+      PhInst *to_d = g_pool.take<PhInst>(nw);
+      PhSpan *rows_d = g_pool.take<PhSpan>(nw);
+      HIP_OK(hipMemcpyAsync(from_d, anchor_from.data(), nw * sizeof(PhInst),
+                            hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(to_d, anchor_to.data(), nw * sizeof(PhInst), hipMemcpyHostToDevice,
+                            st));
+      anchor_rows.upload(rows_d, st);
+      hipLaunchKernelGGL(k_copy_anchor_spans, dim3(anchor_rows.rows), dim3(256), 0, st, from_d,
+                         to_d, rows_d, nw);
+      HIP_OK(hipGetLastError());
+    }
     if (!net_list.empty()) {
       const int nn = (int)net_list.size();
       for (int k : net_list) {
@@ -1244,24 +1273,16 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       rep->warm_start_rel = std::max(rep->warm_start_rel, std::isfinite(rw) ? rw / r0 : INFINITY);
       if (!(std::isfinite(rw) && rw <= WARM_MAX_START_RATIO * r0)) {
         // no race for a start this far off: back to the anchor start at once
-        re_rows.add((int)re_inst.size(), inst[k].I.ph, inst[k].item, 0, k);
-        re_inst.push_back(inst[k].I);
+        inst[k].I = anchor_to[j];
         continue;
       }
-      // the plain twin: its own planes, the anchor start
+      // the plain twin: the anchor-start copy
       InstWork T = inst[k];
-      const size_t plane = (size_t)w.pitch * w.phh;
-      T.I.e[0] = g_pool.take<float>(plane);
-      T.I.e[1] = g_pool.take<float>(plane);
-      T.I.g = g_pool.take<float>(plane);
-      T.I.x = g_pool.take<double>(plane);
-      T.I.bp = g_pool.take<double>(plane);
+      T.I = anchor_to[j];
       T.hist = g_pool.take<double>(std::max(d->max_iterations, 1u));
       inst.push_back(T);
       const int kt = (int)inst.size() - 1;
       grow();
-      re_rows.add((int)re_inst.size(), T.I.ph, T.item, 0, kt);
-      re_inst.push_back(T.I);
       InstWork &Xw = inst[k];
       const bool cycles = !w.in.start && max_corr > 0 && (uint64_t)w.n_interior >= corr_min_cells;
       const uint32_t n = warm_ramp_sweeps(w.omega, cycles);
@@ -1284,44 +1305,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         cyc_len[k] = warm_cycle_sweeps(n, half);
         cyc_next[k] = std::max<uint32_t>(cyc_len[k], (uint32_t)probe[k] + half);
       }
-    }
-    // the anchor start again for the rejected starts, and the plain twins:
-    // their assembly and FP64 residual in three launches
-    if (!re_inst.empty()) {
-      const int nr = (int)re_inst.size();
-      // the assembly takes each image's channels together: C consecutive
-      // descriptors per image, with no planes where a channel is not refilled
-      std::vector<int> group(nitem, -1);
-      for (int r = 0; r < nr; ++r) {
-        const int i = re_rows.h[r].item;
-        if (group[i] < 0) {
-          group[i] = (int)re_group.size();
-          PhInst blank = re_inst[r];
-          blank.x = nullptr;
-          re_group.insert(re_group.end(), iw[i].in.C, blank);
-          re_planes.add(group[i], re_inst[r].ph, i);
-        }
-        re_group[group[i] + re_inst[r].channel] = re_inst[r];
-      }
-      const int ng = (int)re_group.size();
-      PhInst *re_d = g_pool.take<PhInst>(nr);
-      PhInst *re_group_d = g_pool.take<PhInst>(ng);
-      PhSpan *re_spans_d = g_pool.take<PhSpan>(nr);
-      PhSpan *re_planes_d = g_pool.take<PhSpan>(re_planes.h.size());
-      double *re_sums_d = g_pool.take<double>((size_t)re_rows.rows);
-      HIP_OK(hipMemcpyAsync(re_d, re_inst.data(), nr * sizeof(PhInst), hipMemcpyHostToDevice, st));
-      HIP_OK(hipMemcpyAsync(re_group_d, re_group.data(), ng * sizeof(PhInst),
-                            hipMemcpyHostToDevice, st));
-      re_rows.upload(re_spans_d, st);
-      re_planes.upload(re_planes_d, st);
-      launch_by_channels<AssembleL>(common_channels(re_planes, iw), dim3(re_planes.rows), st,
-                                    (const PhInst *)re_group_d, (const PhSpan *)re_planes_d,
-                                    (int)re_planes.h.size(), (const PhItemIO *)io_d);
-      hipLaunchKernelGGL(k_residual_spans, dim3(re_rows.rows), dim3(256), 0, st, re_d, re_spans_d,
-                         nr, re_sums_d);
-      hipLaunchKernelGGL(k_residual_norm_spans, dim3(nr), dim3(256), 0, st, re_spans_d, re_sums_d,
-                         resv_d);
-      HIP_OK(hipGetLastError());
     }
   }
 
