@@ -220,8 +220,8 @@ def bench_batch(args, info):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is a design note, not implemented)"}))
+        print(json.dumps({"error": "the hip backend runs on one GPU; --gpus must be 1 "
+                          "(sharding across GPUs is designed, not implemented)"}))
         sys.exit(2)
     from paintop_amd import hip
     from paintop_amd.device import DeviceArray, synchronize
