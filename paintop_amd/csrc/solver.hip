@@ -420,9 +420,14 @@ static const double CORR_GOOD_GAIN = 4.0;
 static const double CORR_MAX_STEP = 1.5;
 // A good correction leaves the residual within about 1.6x of the uncorrected
 // one (bilinear interpolation error at the fine scale, which the next sweeps
-// remove). One that raises it more than this is taken back; the cycle then
-// runs on uncorrected, and a second one taken back in a row ends the cycles.
-static const double CORR_MAX_JUMP = 4.0;
+// remove). One that raises it more than this (PAINTOP_HIP_CORR_MAX_JUMP,
+// default 4) is taken back; the cycle then runs on uncorrected, and a second
+// one taken back in a row ends the cycles.
+static double corr_max_jump() {
+  double j = 4.0;
+  if (const char *e = getenv("PAINTOP_HIP_CORR_MAX_JUMP")) j = atof(e);
+  return j > 0.0 ? j : 4.0;
+}
 
 static uint32_t stall_window_for(double omega, bool tuned) {
   if (!tuned) return PH_STALL_WINDOW;
@@ -1532,12 +1537,13 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       HIP_OK(hipStreamSynchronize(st));
       std::vector<char> undo(nc, 0);
       std::vector<double> jump(nc, 0.0);
+      const double max_jump = corr_max_jump();
       undo_args.clear();
       undo_rows.clear();
       for (int j = 0; j < nc; ++j) {
         const int a = go[j].a, k = act[a];
         jump[j] = resv[2 * k] / go[j].gnorm;
-        if (alpha[j] != 0.0 && !(std::isfinite(jump[j]) && jump[j] <= CORR_MAX_JUMP)) {
+        if (alpha[j] != 0.0 && !(std::isfinite(jump[j]) && jump[j] <= max_jump)) {
           PhCorrArgs A = corr_args[j];
           A.sign = -1.0;
           undo_args.push_back(A);

@@ -518,6 +518,46 @@ def test_batch_gate_is_relative_to_the_largest_image(gpu):
     assert alone.converged and alone.warm_start_rel > 0.0
 
 
+def test_corrections_taken_back(gpu, monkeypatch):
+    """The take-back path of the correction rounds: with the largest allowed
+    residual jump set below 1 every correction raises the residual too much
+    (a good one leaves it within ~1.6x), so each is taken back in the same
+    round (x -= the same step, FP64 residual again, in one launch for all
+    channels), the second in a row ends the channel's cycles, none is counted,
+    and the solve still converges to the tolerance with the outside-mask
+    samples kept, within the error a 1e-6 residual allows of the plain solve."""
+    from paintop_amd import warmstart
+    from paintop_amd.device import DeviceArray
+    from tests.helpers import classify
+
+    w, h, c = 1024, 1024, 3
+    s_, t_, m_ = _fbm_case(w, h, c, 0.125, 860, "polygon")
+    d = hip.make_desc(w, h, c, 100000, 1e-6)
+    op = DeviceArray(t_.shape, np.float32)
+    rp = gpu.poisson_solve_device(d, s_.ptr, t_.ptr, m_.ptr, op.ptr)
+    ol, oc = DeviceArray(t_.shape, np.float32), DeviceArray(t_.shape, np.float32)
+    warmstart.enable()
+    try:
+        d.flags = hip.PH_FLAG_WARM_START_NET
+        rc = gpu.poisson_solve_device(d, s_.ptr, t_.ptr, m_.ptr, oc.ptr)
+        monkeypatch.setenv("PAINTOP_HIP_CORR_MAX_JUMP", "0.5")
+        rl = gpu.poisson_solve_device(d, s_.ptr, t_.ptr, m_.ptr, ol.ptr)
+    finally:
+        gpu.warm_start_set_weights(None)
+    print(f"[take-back] plain {rp.iterations}, corrected {rc.iterations} "
+          f"({rc.coarse_corrections} kept), taken back {rl.iterations} "
+          f"({rl.coarse_corrections} kept, start kept {rl.warm_start_used})", flush=True)
+    assert rc.coarse_corrections > 0
+    assert rl.warm_start_used > 0 and rl.coarse_corrections == 0
+    assert rl.converged and rl.final_residual <= 1e-6
+    assert rl.iterations > rc.iterations
+    a, b, t = op.numpy(), ol.numpy(), t_.numpy()
+    inside = classify(m_.numpy()).astype(bool)
+    for ch in range(c):
+        assert b[..., ch][~inside].tobytes() == t[..., ch][~inside].tobytes()
+    assert float(np.abs(a - b).max()) < 2e-4
+
+
 def test_blend_split_into_mask_components(gpu):
     """components.py: the blend solved per 4-connected mask component in one
     batched call equals the whole-mask solve (both at 1e-9: within 1e-6), keeps
