@@ -308,8 +308,10 @@ __global__ __launch_bounds__(256) void k_coarsen_plane_list(const PhInst *__rest
 }
 
 // Coarse cells per block of k_coarsen_plane: about one 4-column group per
-// thread (each thread's serial chain is then the f rows of its group).
-int coarsen_plane_cpb(int f) { return std::max(1, std::min(32, 1024 / f)); }
+// thread (each thread's serial chain is then the f rows of its group), at most
+// 256 cells (one finishing thread per cell). The result does not depend on it:
+// a cell's columns are always added in column order by one thread.
+int coarsen_plane_cpb(int f) { return std::max(1, std::min(256, 1024 / f)); }
 
 // The problems are linear: each net input is scaled so that its rhs and
 // boundary channels have unit RMS over the grid (scale[b] = that RMS, 1 when
