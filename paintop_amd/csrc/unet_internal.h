@@ -99,8 +99,11 @@ template <int MB, int NB>
 __global__ void k_conv3(UnConvArgs A);
 template <int MB>
 __global__ void k_conv3_sk(UnConvArgs A);
-// Convolutions over at most this many pixels take the split-K kernel.
-constexpr int UN_SPLITK_MAX_PIX = 16384;
+// Convolutions over at most this many pixels take the split-K kernel: the
+// solver's batch of three 128x128 grids (one RGB image) has too few waves for
+// the one-wave-per-tile kernel to hide its load chain, while a 64-grid batch's
+// 32x32 level (65536 pixels) is faster without the split.
+constexpr int UN_SPLITK_MAX_PIX = 49152;
 __global__ void k_gather_dihedral(const float *xin, const float *yin, const float *win, UnGather G,
                                   int res, float *xo, float *yo, float *wo);
 __global__ void k_conv3_dw(UnDwArgs A);
