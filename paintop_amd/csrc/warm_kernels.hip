@@ -142,6 +142,19 @@ __global__ __launch_bounds__(64) void k_coarsen_groups(const PhInst *__restrict_
   const int c = blockIdx.x;
   const int cx = c % res, cy = c / res;
   const int lane = threadIdx.x;
+  // a coarse cell wholly outside the image (small images leave most of the
+  // grid empty) has no interior or ring cell: its inputs are all zero
+  if (x0 + cx * f >= W || y0 + cy * f >= H || x0 + cx * f + f <= 0 || y0 + cy * f + f <= 0) {
+    if (lane < ne) {
+      const int nl = lane == 0 ? net[0] : lane == 1 ? net[1] : lane == 2 ? net[2] : net[3];
+      float *o = in4 + ((size_t)(nl - j0) * res * res + c) * 4;
+      o[0] = 0.0f;
+      o[1] = 0.0f;
+      o[2] = 0.0f;
+      o[3] = 0.0f;
+    }
+    return;
+  }
   double s_rhs[4] = {0.0, 0.0, 0.0, 0.0}, s_ring[4] = {0.0, 0.0, 0.0, 0.0};
   double n_int = 0.0, n_ring = 0.0;
   for (int i = lane; i < f * f; i += 64) {
