@@ -165,7 +165,7 @@ template <int CC>
 __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict__ insts,
                                                         const PhSpan *__restrict__ spans, int n,
                                                         const PhItemIO *__restrict__ io) {
-  const int b = blockIdx.x;
+  const int b = xcd_row_group(blockIdx.x, gridDim.x);  // reads the source rows around b
   const PhSpan S = spans[ph_span_find(spans, n, b)];
   const PhItemIO &G = io[S.item];
   const PhInst I0 = insts[S.slot];  // the plane geometry, shared by the channels
@@ -327,17 +327,7 @@ __global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__
 __global__ __launch_bounds__(256) void k_residual_spans(const PhInst *__restrict__ insts,
                                                         const PhSpan *__restrict__ spans, int n,
                                                         double *__restrict__ row_sums) {
-  // Blocks are dealt to the 8 XCDs in turn. Groups of 8 consecutive rows go
-  // to one XCD, the groups in turn over the XCDs: the rows above and below a
-  // row are mostly in the L2 that read them, and every XCD gets rows from all
-  // over the planes (contiguous bands per XCD left some XCDs with most of the
-  // mask). Rows past the last whole 64 keep their own block.
-  int b = blockIdx.x;
-  const int n64 = (int)gridDim.x & ~63;
-  if (b < n64) {
-    const int slot = b >> 3;
-    b = (((slot >> 3) << 3) + (b & 7)) * 8 + (slot & 7);
-  }
+  const int b = xcd_row_group(blockIdx.x, gridDim.x);
   const PhSpan S = spans[ph_span_find(spans, n, b)];
   const double s = block_sum(residual_row(insts[S.slot], b - S.row0));
   if (threadIdx.x == 0) row_sums[b] = s;

@@ -115,6 +115,18 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return t;
 }
 
+// Row of block b in a launch of one block per plane row (n blocks) whose rows
+// read the rows above and below: groups of 8 consecutive rows go to one XCD,
+// the groups dealt over the XCDs in turn, so neighbouring rows are mostly in
+// the L2 that read them while every XCD gets rows from all over the planes
+// (contiguous bands per XCD leave some XCDs most of a mask). Rows past the
+// last whole 64 keep their own block. A bijection of [0, n).
+__device__ __forceinline__ int xcd_row_group(int b, int n) {
+  if (b >= (n & ~63)) return b;
+  const int slot = b >> 3;
+  return (((slot >> 3) << 3) + (b & 7)) * 8 + (slot & 7);
+}
+
 // Slices of the two-stage reductions over a warm-start net input (input
 // normalisation, correction step).
 constexpr int PH_NET_SLICES = 16;
