@@ -134,22 +134,48 @@ __global__ void k_tile_occupancy_all(const PhItemGeo *__restrict__ items,
 
 // Interior-cell count of each f x f block of an image's region (the coarse
 // mask the SOR-factor tuner works on), for every image of a call in one
-// launch (blockIdx.y = image; images without a coarse grid have cw * ch = 0).
-__global__ void k_coarse_count_all(const PhItemGeo *__restrict__ items, int *__restrict__ counts) {
+// launch: blockIdx.y = image, blockIdx.x = coarse row (images without a coarse
+// grid have cw * ch = 0). The 4 waves share the coarse row's f image rows and
+// read along them (coalesced), each lane adds its columns over its rows, and
+// the sums meet in LDS counters (integer adds: the counts do not depend on
+// the order). cw <= PH_COARSE_MAX.
+constexpr int PH_COARSE_MAX = 256;
+__global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__restrict__ items,
+                                                          int *__restrict__ counts) {
   const PhItemGeo G = items[blockIdx.y];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= G.cw * G.ch) return;
-  const int cx = c % G.cw, cy = c / G.cw;
-  int n = 0;
-  for (int dy = 0; dy < G.cf; ++dy) {
-    const int iy = G.ry0 + cy * G.cf + dy;
-    if (iy < 0 || iy >= G.h) continue;
-    for (int dx = 0; dx < G.cf; ++dx) {
-      const int ix = G.rx0 + cx * G.cf + dx;
-      if (ix >= 0 && ix < G.w) n += G.cls[(size_t)iy * G.w + ix];
+  const int cy = blockIdx.x;
+  if (cy >= G.ch) return;
+  __shared__ int cnt[PH_COARSE_MAX];
+  for (int c = threadIdx.x; c < G.cw; c += 256) cnt[c] = 0;
+  __syncthreads();
+  const int ncols = G.cw * G.cf;
+  // wave w takes the image rows w, w + 4, ...; its lanes take columns
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int J = 16;  // columns per lane per chunk
+  for (int x0 = 0; x0 < ncols; x0 += 64 * J) {
+    int n[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) n[j] = 0;
+    for (int dy = wave; dy < G.cf; dy += 4) {
+      const int iy = G.ry0 + cy * G.cf + dy;
+      if (iy < 0 || iy >= G.h) continue;
+      const uint8_t *row = G.cls + (size_t)iy * G.w;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int x = x0 + j * 64 + lane;
+        const int ix = G.rx0 + x;
+        if (x < ncols && ix >= 0 && ix < G.w) n[j] += row[ix];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = x0 + j * 64 + lane;
+      if (n[j]) atomicAdd(&cnt[x / G.cf], n[j]);
     }
   }
-  counts[G.coarse_off + c] = n;
+  __syncthreads();
+  for (int c = threadIdx.x; c < G.cw; c += 256)
+    counts[G.coarse_off + (int64_t)cy * G.cw + c] = cnt[c];
 }
 
 // Fill the planes of the listed images' channel instances, one block per plane

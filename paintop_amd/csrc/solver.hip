@@ -890,7 +890,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     // every tile origin
     std::vector<PhItemGeo> geo(nitem);
     std::vector<int> origin_item(all_origins.size());
-    int max_coarse = 0;
+    int max_coarse_rows = 0;
     for (int i = 0; i < nitem; ++i) {
       const ItemWork &w = iw[i];
       PhItemGeo &G = geo[i];
@@ -907,7 +907,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         G.cw = w.cw;
         G.ch = w.ch;
         G.coarse_off = (int64_t)w.coarse_off;
-        max_coarse = std::max(max_coarse, w.cw * w.ch);
+        max_coarse_rows = std::max(max_coarse_rows, w.ch);
       }
       if (w.any)
         for (size_t k = 0; k < w.origins.size(); ++k) origin_item[w.origin_off + k] = i;
@@ -922,8 +922,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                           hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_tile_occupancy_all, dim3(all_origins.size()), dim3(256), 0, st, geo_d,
                        oitem_d, origins_d, K, WV, hb, counts_d);
-    if (tuned && max_coarse > 0)
-      hipLaunchKernelGGL(k_coarse_count_all, dim3((max_coarse + 255) / 256, nitem), dim3(256), 0,
+    if (tuned && max_coarse_rows > 0)
+      hipLaunchKernelGGL(k_coarse_count_all, dim3(max_coarse_rows, nitem), dim3(256), 0,
                          st, geo_d, coarse_d);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
