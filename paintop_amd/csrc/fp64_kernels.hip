@@ -134,17 +134,18 @@ __global__ void k_tile_occupancy_all(const PhItemGeo *__restrict__ items,
 
 // Interior-cell count of each f x f block of an image's region (the coarse
 // mask the SOR-factor tuner works on), for every image of a call in one
-// launch: blockIdx.y = image, blockIdx.x = coarse row (images without a coarse
-// grid have cw * ch = 0). The 4 waves share the coarse row's f image rows and
+// launch: one block per coarse row of every listed image (spans: slot =
+// image, nrows = its coarse rows). The 4 waves share the coarse row's f image rows and
 // read along them (coalesced), each lane adds its columns over its rows, and
 // the sums meet in LDS counters (integer adds: the counts do not depend on
 // the order). cw <= PH_COARSE_MAX.
 constexpr int PH_COARSE_MAX = 256;
 __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__restrict__ items,
-                                                          int *__restrict__ counts) {
-  const PhItemGeo G = items[blockIdx.y];
-  const int cy = blockIdx.x;
-  if (cy >= G.ch) return;
+                                                          const PhSpan *__restrict__ spans,
+                                                          int n, int *__restrict__ counts) {
+  const PhSpan S = spans[ph_span_find(spans, n, blockIdx.x)];
+  const PhItemGeo G = items[S.slot];
+  const int cy = blockIdx.x - S.row0;
   __shared__ int cnt[PH_COARSE_MAX];
   for (int c = threadIdx.x; c < G.cw; c += 256) cnt[c] = 0;
   __syncthreads();

@@ -49,7 +49,7 @@ __global__ void k_classify_spans(const PhSpan *, int, const PhItemIO *, int2 *);
 __global__ void k_bbox_spans(const PhSpan *, const int2 *, int *);
 __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 *, int, int, int,
                                      int *);
-__global__ void k_coarse_count_all(const PhItemGeo *, int *);
+__global__ void k_coarse_count_all(const PhItemGeo *, const PhSpan *, int, int *);
 template <int CC>
 __global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
 __global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
@@ -890,7 +890,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     // every tile origin
     std::vector<PhItemGeo> geo(nitem);
     std::vector<int> origin_item(all_origins.size());
-    int max_coarse_rows = 0;
+    SpanTable coarse_rows;  // one block per coarse row of every tuned image
     for (int i = 0; i < nitem; ++i) {
       const ItemWork &w = iw[i];
       PhItemGeo &G = geo[i];
@@ -907,7 +907,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         G.cw = w.cw;
         G.ch = w.ch;
         G.coarse_off = (int64_t)w.coarse_off;
-        max_coarse_rows = std::max(max_coarse_rows, w.ch);
+        if (w.ch > 0) coarse_rows.add(i, w.ch, i);
       }
       if (w.any)
         for (size_t k = 0; k < w.origins.size(); ++k) origin_item[w.origin_off + k] = i;
@@ -922,9 +922,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                           hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_tile_occupancy_all, dim3(all_origins.size()), dim3(256), 0, st, geo_d,
                        oitem_d, origins_d, K, WV, hb, counts_d);
-    if (tuned && max_coarse_rows > 0)
-      hipLaunchKernelGGL(k_coarse_count_all, dim3(max_coarse_rows, nitem), dim3(256), 0,
-                         st, geo_d, coarse_d);
+    if (coarse_rows.rows > 0) {
+      PhSpan *coarse_rows_d = g_pool.take<PhSpan>(coarse_rows.h.size());
+      coarse_rows.upload(coarse_rows_d, st);
+      hipLaunchKernelGGL(k_coarse_count_all, dim3(coarse_rows.rows), dim3(256), 0, st, geo_d,
+                         coarse_rows_d, (int)coarse_rows.h.size(), coarse_d);
+    }
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
                           hipMemcpyDeviceToHost, st));
