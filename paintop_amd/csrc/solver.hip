@@ -196,7 +196,8 @@ struct PinnedPool {
   template <class T>
   T *take(size_t n) {
     size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
-    if (used + bytes > cap) throw fail(PH_CLASS_EXECUTION, "E_HIP_WORKSPACE", "pinned staging overrun");
+    if (used + bytes > cap)
+      throw fail(PH_CLASS_EXECUTION, "E_HIP_WORKSPACE", "pinned staging overrun");
     T *p = reinterpret_cast<T *>(base + used);
     used += bytes;
     return p;
@@ -1499,7 +1500,8 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       for (int j = 0; j < nc; ++j) {
         const int a = go[j].a, k = act[a];
         const Geo &g = wgeo[k];
-        corr_args[j] = PhCorrArgs{a, g.x0, g.y0, g.f, coarsen_plane_cpb(g.f), j, inst[k].item, 0, 1.0};
+        corr_args[j] =
+            PhCorrArgs{a, g.x0, g.y0, g.f, coarsen_plane_cpb(g.f), j, inst[k].item, 0, 1.0};
         corr_rows.add(a, ai[a].ph, 0, 0, k);
       }
       HIP_OK(hipMemcpyAsync(corr_args_d, corr_args.data(), nc * sizeof(PhCorrArgs),
