@@ -1,10 +1,14 @@
 /*
  * generators.c -- test-infrastructure restatement of paintop's field.fbm@1
- * value-noise generator (crates/paintop-cpu/src/noise.rs). Used by the tests as
- * the checker for the device-side fbm generator, and by the benchmark's
- * cpu_baseline leg. Build: -O2 -ffp-contract=off, no fast-math.
+ * value-noise generator (crates/paintop-cpu/src/noise.rs), the pixel-centre
+ * polygon mask (mask.polygon@1's inside test, mask_polygon.rs:155-195) and the
+ * seeded Gaussian noise of the screened workload. Used by the tests as the
+ * checker for the device-side generators, by the fixture scripts to rebuild the
+ * full-size workloads on the CPU, and by the benchmark's cpu_baseline leg.
+ * Build: -O2 -ffp-contract=off, no fast-math.
  */
 #include <math.h>
+#include <stdlib.h>
 
 #include "oracle.h"
 
@@ -68,11 +72,56 @@ double or_fbm_eval(double x, double y, uint64_t seed, uint32_t octaves, double l
 void or_fbm(size_t width, size_t height, double frequency, uint64_t seed, uint32_t octaves,
             double lacunarity, double gain, float *out) {
   double scale = frequency / 64.0;
-  for (size_t y = 0; y < height; ++y)
+  /* pixels are independent: rows in parallel give the same bits */
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long yy = 0; yy < (long)height; ++yy)
     for (size_t x = 0; x < width; ++x) {
+      size_t y = (size_t)yy;
       double px = (double)(uint32_t)x + 0.5;
       double py = (double)(uint32_t)y + 0.5;
       out[y * width + x] =
           (float)or_fbm_eval(px * scale, py * scale, seed, octaves, lacunarity, gain);
     }
+}
+
+/* mask_polygon.rs:161-195: half-open ray cast to +x at each pixel centre. */
+void or_polygon_mask(size_t width, size_t height, const double *points_xy, size_t n_points,
+                     int even_odd, float *out) {
+#pragma omp parallel for schedule(static)
+  for (long yy = 0; yy < (long)height; ++yy)
+    for (size_t xx = 0; xx < width; ++xx) {
+      double x = (double)xx + 0.5, y = (double)yy + 0.5;
+      int winding = 0, parity = 0;
+      if (n_points >= 3)
+        for (size_t i = 0; i < n_points; ++i) {
+          size_t k = i + 1 == n_points ? 0 : i + 1;
+          double x0 = points_xy[2 * i], y0 = points_xy[2 * i + 1];
+          double x1 = points_xy[2 * k], y1 = points_xy[2 * k + 1];
+          int below0 = y0 <= y, below1 = y1 <= y;
+          if (below0 == below1) continue;
+          double t = (y - y0) / (y1 - y0);
+          double cross_x = fma(t, x1 - x0, x0);
+          if (cross_x > x) {
+            parity = !parity;
+            winding += below0 ? 1 : -1;
+          }
+        }
+      int inside = n_points >= 3 && (even_odd ? parity : winding != 0);
+      out[(size_t)yy * width + xx] = inside ? 1.0f : 0.0f;
+    }
+}
+
+/* Box-Muller on two SplitMix64 draws keyed by (seed, i). */
+void or_gauss_noise(float *out, size_t n, uint64_t seed, double sigma) {
+  const uint64_t G = 0x9e3779b97f4a7c15ULL;
+#pragma omp parallel for schedule(static)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    uint64_t i = (uint64_t)ii;
+    uint64_t a = or_mix64(seed + G * (2 * i + 1));
+    uint64_t b = or_mix64(seed + G * (2 * i + 2));
+    double u1 = ((double)(a >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+    double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+    double z = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    out[i] = out[i] + (float)(sigma * z);
+  }
 }

@@ -12,6 +12,12 @@
  *   or_blend_channels        crates/paintop-cpu/src/poisson_blend.rs:153-246
  *   or_fbm                   crates/paintop-cpu/src/noise.rs:72-160, 218-240, 660-694
  *   or_outside_scan          crates/paintop-cpu/src/assert.rs:72-85, 178-201, 405-446
+ *   or_tight_solve           the system of crates/paintop-cpu/src/poisson.rs:168-240,
+ *                            305-325 solved to a tight FP64 residual by
+ *                            multigrid-preconditioned conjugate gradients
+ *                            (the large-size parity reference)
+ *   or_polygon_mask          crates/paintop-cpu/src/mask_polygon.rs:155-195 inside
+ *                            test at pixel centres (the workload generator)
  *   or_rbsor_omp             no reference counterpart: the reference has no
  *                            cpu.optimized Poisson kernel, so this OpenMP
  *                            red-black SOR is the stand-in timed as the
@@ -85,6 +91,41 @@ int or_rbsor_omp(size_t width, size_t height, const uint8_t *cells, const double
                  const double *anchor, double lambda, uint32_t max_iterations, double tolerance,
                  double omega, const double *omega_tab, int threads, uint32_t stall_window,
                  double *field, double *history, or_report *report);
+
+/* Tight FP64 solve of the system poisson.rs:168-240 defines (Interior cells:
+ * nsum(u) - (4 + lambda) u = rhs - lambda anchor; u = anchor elsewhere), by
+ * conjugate gradients preconditioned with a Galerkin-aggregation multigrid
+ * V-cycle, restarted from the true residual until the relative residual
+ * ||r(u)|| / ||r(anchor)|| (or_residual_l2, the reference's formula and order)
+ * is <= tolerance or stops halving (the FP64 floor). alpha: coarse-correction
+ * factor in (0, 2) (<= 0: 1). threads <= 0: the OpenMP default. Returns the
+ * thread count used; the result is bit-identical for any thread count. */
+typedef struct {
+  double r0;             /* ||r(anchor)|| */
+  double final_residual; /* ||r(field)|| / r0 */
+  uint32_t iterations;   /* CG iterations, all restarts */
+  uint32_t restarts;
+  uint32_t levels;
+  int32_t converged;
+  int32_t threads;
+  int32_t _pad;
+} or_tight_report;
+
+int or_tight_solve(size_t width, size_t height, const uint8_t *cells, const double *rhs,
+                   const double *anchor, double lambda, double tolerance,
+                   uint32_t max_iterations, double alpha, int threads, double *field,
+                   or_tight_report *report);
+
+/* mask.polygon@1's inside test (mask_polygon.rs:161-195: half-open ray cast to
+ * +x, cross_x = fma(t, x1 - x0, x0), nonzero winding or even-odd) at each pixel
+ * centre (x + 0.5, y + 0.5): 1 inside, 0 outside. points_xy = x0,y0,x1,y1,... */
+void or_polygon_mask(size_t width, size_t height, const double *points_xy, size_t n_points,
+                     int even_odd, float *out);
+
+/* out[i] += (float)(sigma * z_i), z_i = sqrt(-2 ln u1) cos(2 pi u2) with
+ * u1 = ((mix64(seed + G (2i+1)) >> 11) + 1) 2^-53, u2 = (mix64(seed + G (2i+2)) >> 11) 2^-53,
+ * G = 0x9e3779b97f4a7c15 (SplitMix64's increment, mix64 = noise.rs:72-76). */
+void or_gauss_noise(float *out, size_t n, uint64_t seed, double sigma);
 
 /* assert.no_change_outside_mask@1's scan (crates/paintop-cpu/src/assert.rs:405-446).
  * Pixel indices are row-major; (x, y) = (i % width, i / width) (assert.rs:196-201). */

@@ -38,6 +38,19 @@ class _Report(ctypes.Structure):
     ]
 
 
+class _TightReport(ctypes.Structure):
+    _fields_ = [
+        ("r0", ctypes.c_double),
+        ("final_residual", ctypes.c_double),
+        ("iterations", ctypes.c_uint32),
+        ("restarts", ctypes.c_uint32),
+        ("levels", ctypes.c_uint32),
+        ("converged", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+        ("_pad", ctypes.c_int32),
+    ]
+
+
 class _OutsideResult(ctypes.Structure):
     _fields_ = [
         ("changed_outside", ctypes.c_uint64),
@@ -107,6 +120,12 @@ def lib():
                                    ctypes.c_double, ctypes.c_double, dp, ctypes.c_int,
                                    ctypes.c_uint32, dp, dp, ctypes.POINTER(_Report)]
         L.or_rbsor_omp.restype = ctypes.c_int
+        L.or_tight_solve.argtypes = [sz, sz, up, dp, dp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_uint32, ctypes.c_double, ctypes.c_int, dp,
+                                     ctypes.POINTER(_TightReport)]
+        L.or_tight_solve.restype = ctypes.c_int
+        L.or_polygon_mask.argtypes = [sz, sz, dp, sz, ctypes.c_int, fp]
+        L.or_gauss_noise.argtypes = [fp, sz, ctypes.c_uint64, ctypes.c_double]
         L.or_outside_scan.argtypes = [fp, fp, fp, sz, sz, sz, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.POINTER(_OutsideResult)]
         _lib = L
@@ -219,6 +238,43 @@ def rbsor_omp(cells, rhs, anchor, lam, max_iterations, tolerance, omega, threads
                               _p(hist, ctypes.c_double),
                               ctypes.byref(rep))
     return out, _report(rep, hist), int(used)
+
+
+def tight_solve(cells, rhs, anchor, lam, tolerance=1e-12, max_iterations=20000, alpha=1.0,
+                threads=0):
+    """The system of poisson.rs:168-240 solved to a tight FP64 relative residual
+    by multigrid-preconditioned CG (oracle/tight_solve.c). Returns (field,
+    report dict): r0, final_residual (the reference's residual formula and
+    order), CG iterations, restarts, multigrid levels, threads."""
+    cells = np.ascontiguousarray(cells, dtype=np.uint8)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    anchor = np.ascontiguousarray(anchor, dtype=np.float64)
+    h, w = cells.shape
+    out = np.empty((h, w), dtype=np.float64)
+    rep = _TightReport()
+    lib().or_tight_solve(w, h, _p(cells, ctypes.c_uint8), _p(rhs, ctypes.c_double),
+                         _p(anchor, ctypes.c_double), float(lam), float(tolerance),
+                         int(max_iterations), float(alpha), int(threads),
+                         _p(out, ctypes.c_double), ctypes.byref(rep))
+    return out, dict(r0=rep.r0, final_residual=rep.final_residual, iterations=rep.iterations,
+                     restarts=rep.restarts, levels=rep.levels, converged=bool(rep.converged),
+                     threads=rep.threads)
+
+
+def polygon_mask(width, height, points, even_odd=False) -> np.ndarray:
+    """mask.polygon@1's inside test at pixel centres (mask_polygon.rs:161-195)."""
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1))
+    out = np.empty((height, width), dtype=np.float32)
+    lib().or_polygon_mask(width, height, _p(pts, ctypes.c_double), pts.size // 2,
+                          int(bool(even_odd)), _p(out, ctypes.c_float))
+    return out
+
+
+def gauss_noise(out: np.ndarray, seed: int, sigma: float) -> None:
+    """out += sigma * N(0, 1) in place (seeded SplitMix64 + Box-Muller)."""
+    assert out.dtype == np.float32 and out.flags.c_contiguous
+    lib().or_gauss_noise(_p(out, ctypes.c_float), out.size, seed & 0xFFFFFFFFFFFFFFFF,
+                         float(sigma))
 
 
 def mix64(z: int) -> int:

@@ -1,10 +1,28 @@
-"""Full-size GPU cases, checked through size-independent properties: the
-outside-mask samples are byte-identical to the target/anchor, the FP64
-stopping decision holds, reruns are bit-identical, and the result agrees with
-the same system driven to 1e-12 (the error against a tight FP64 solve is
-stated). A mid-size case is compared with the oracle's red-black SOR.
+"""Full-size GPU cases against an independent FP64 solution.
+
+The fixtures under tests/golden/ hold the 4096^2 seamless-clone blend (R, G, B)
+and the 8192^2 screened-Poisson case solved on the CPU to a 1e-12 relative
+residual by the oracle's multigrid-preconditioned CG (oracle/tight_solve.c,
+pinned against a sparse direct solve in tests/test_oracle_tight.py), with the
+inputs rebuilt by the oracle's generators. Each GPU result (plain start, and
+the learned warm start at 4096^2) is compared with that solution at 4096
+seeded interior cells and through the mean over the interior cells of every
+64x64 (128x128) block, within the bounded tier's max-abs 1e-4 (on [0, 1]
+data); the measured error is printed. Beside it: the inputs are confirmed to
+be the fixture's (samples, classification CRC), outside-mask samples are
+byte-identical, reruns are bit-identical, and the FP64 residual of the
+returned f32 candidate is recomputed in numpy.
+
+The f32 candidate itself cannot meet 1e-6: rounding an FP64 field of values
+~0.5 to f32 moves each cell by up to half an ulp (~3e-8), which alone gives a
+residual of ~1e-4 absolute on 5M cells against r0 ~ 0.5 (the reference's
+candidate is the same f32 cast of its field, poisson_blend.rs:227-229). So the
+numpy residual is held to the bound the rounding allows,
+||r(candidate)|| <= tol * r0 + (8 + lambda) * ||spacing(candidate)||_2, and the
+convergence itself is judged by the error against the fixture.
 """
-import math
+import os
+import zlib
 
 import numpy as np
 import pytest
@@ -13,6 +31,9 @@ from paintop_amd import hip
 from tests.helpers import classify, laplacian, residual_norm
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+MAX_ABS = 1e-4  # bounded tier, paintop-testkit/src/differential.rs:58-62
 
 
 def _solve(desc, src, tgt, msk):
@@ -25,83 +46,153 @@ def _solve(desc, src, tgt, msk):
     return a, r
 
 
-def test_seamless_clone_4096_rgba(gpu):
+def _block_means(u, cells, b):
+    h, w = cells.shape
+    m = cells.astype(np.float64)
+    s = (u.astype(np.float64) * m).reshape(h // b, b, w // b, b).sum(axis=(1, 3))
+    c = m.reshape(h // b, b, w // b, b).sum(axis=(1, 3))
+    return np.where(c > 0, s / np.maximum(c, 1), 0.0)
+
+
+def _check_inputs(z, cells, chans):
+    """The device-generated inputs are the fixture's."""
+    assert int(cells.sum()) == int(z["n_interior"])
+    assert zlib.crc32(np.packbits(cells).tobytes()) == int(z["cells_crc32"])
+    pts = z["pts"]
+    for tag, (s, t) in chans.items():
+        assert t.reshape(-1)[pts].tobytes() == z[f"{tag}_tgt_pts"].tobytes()
+        ds = np.abs(s.reshape(-1)[pts].astype(np.float64) - z[f"{tag}_src_pts"])
+        # fbm is bit-exact; the screened guidance adds Gaussian noise whose
+        # log/cos may round differently on the device: at most one f32 ulp
+        assert ds.max() <= np.spacing(np.float32(2.0)), ds.max()
+
+
+def _against_fixture(z, tag, u, cells, lam, src, tgt, tol, label):
+    """Error of the f32 field u against the fixture's FP64 solution, and the
+    FP64 residual of u itself; returns the max-abs error at the samples."""
+    pts = z["pts"]
+    err = float(np.abs(u.reshape(-1)[pts].astype(np.float64) - z[f"{tag}_u_pts"]).max())
+    b = int(z["block"])
+    bm = _block_means(u, cells, b)
+    err_blk = float(np.abs(bm - z[f"{tag}_blk_mean"]).max())
+    rhs = np.where(cells == 1, laplacian(src), 0.0)
+    r0 = float(z[f"{tag}_report"][0])
+    r_u = residual_norm(cells, rhs, tgt, lam, u)
+    inside = cells.astype(bool)
+    bound = tol * r0 + (8.0 + lam) * float(np.sqrt(np.sum(np.spacing(u[inside]).astype(np.float64) ** 2)))
+    print(f"[large] {label} {tag}: max |gpu - fp64 tight| = {err:.3e} at 4096 cells, "
+          f"{err_blk:.3e} over block means; numpy ||r(candidate)||/r0 = {r_u / r0:.3e} "
+          f"(f32-rounding bound {bound / r0:.3e})", flush=True)
+    assert err <= MAX_ABS and err_blk <= MAX_ABS
+    assert r_u <= bound
+    return err
+
+
+def _seamless(warm):
+    from paintop_amd import warmstart
     from paintop_amd.inputs import seamless_clone_case
 
-    n = 4096
+    z = np.load(os.path.join(GOLDEN, "seamless_clone_4096_tight.npz"))
+    n = int(z["size"])
     src, tgt, msk = seamless_clone_case(n, channels=4)
-    desc = hip.make_desc(n, n, 4, 100000, 1e-6)
-    out, r = _solve(desc, src, tgt, msk)
-    print(f"[large] seamless 4096 RGBA: {r.iterations} sweeps, omega {r.omega:.6f}, "
-          f"refinements {r.refinements}, final {r.final_residual:.3e}", flush=True)
+    flags = hip.PH_FLAG_WARM_START_NET if warm else 0
+    if warm:
+        warmstart.enable()
+    try:
+        desc = hip.make_desc(n, n, 4, 100000, 1e-6, flags=flags)
+        out, r = _solve(desc, src, tgt, msk)
+        out2, r2 = _solve(desc, src, tgt, msk)
+    finally:
+        if warm:
+            warmstart.disable()
+    label = "seamless 4096 " + ("learned" if warm else "plain")
+    print(f"[large] {label}: {r.iterations} sweeps, omega {r.omega:.6f}, refinements "
+          f"{r.refinements}, warm kept {r.warm_start_used}, corrections "
+          f"{r.coarse_corrections}, final {r.final_residual:.3e}", flush=True)
     assert r.converged and r.final_residual <= 1e-6
     assert len(r.residual_history) == r.iterations
-    t = tgt.numpy()
-    m = msk.numpy()
-    inside = classify(m).astype(bool)
+    assert out2.tobytes() == out.tobytes() and r2.residual_history == r.residual_history
+    s, t, m = src.numpy(), tgt.numpy(), msk.numpy()
+    for a in (src, tgt, msk):
+        a.free()
+    cells = classify(m)
+    inside = cells.astype(bool)
+    _check_inputs(z, cells, {f"c{c}": (s[..., c], t[..., c]) for c in range(3)})
     for c in range(4):
         assert out[..., c][~inside].tobytes() == t[..., c][~inside].tobytes()
     assert out[..., 3].tobytes() == t[..., 3].tobytes()  # constant alpha: r0 == 0
-    out2, r2 = _solve(desc, src, tgt, msk)
-    assert out2.tobytes() == out.tobytes() and r2.residual_history == r.residual_history
-    tight, rt = _solve(hip.make_desc(n, n, 4, 100000, 1e-12), src, tgt, msk)
-    err = float(np.abs(out[..., :3] - tight[..., :3]).max())
-    print(f"[large] seamless 4096: max |u(1e-6) - u(1e-12)| = {err:.3e} "
-          f"({rt.iterations} sweeps, {rt.stop_reason} at {rt.final_residual:.2e})", flush=True)
-    # an FP64 residual of a field of size ~0.5 bottoms out near 1e-13..1e-12
-    # relative, so the tight solve may end as stalled just above 1e-12
-    assert rt.final_residual < 1e-11 and err < 1e-5
+    errs = [_against_fixture(z, f"c{c}", np.ascontiguousarray(out[..., c]), cells, 0.0,
+                             s[..., c], t[..., c], 1e-6, label) for c in range(3)]
+    return out, r, errs
+
+
+def test_seamless_clone_4096_rgba_plain(gpu):
+    _seamless(False)
+
+
+def test_seamless_clone_4096_rgba_learned(gpu):
+    """The headline learned form (warm start + coarse corrections) at 4096^2,
+    against the same independent solution."""
+    out, r, _ = _seamless(True)
+    assert r.warm_start_used >= 1
 
 
 def test_screened_8192(gpu):
     from paintop_amd.inputs import screened_case
 
-    n = 8192
+    z = np.load(os.path.join(GOLDEN, "screened_8192_tight.npz"))
+    n = int(z["size"])
     g, a, m = screened_case(n)
     desc = hip.make_desc(n, n, 1, 100000, 1e-7, 0.01, hip.PH_OP_SCREENED_POISSON)
     out, r = _solve(desc, g, a, m)
     print(f"[large] screened 8192: {r.iterations} sweeps, omega {r.omega:.6f}, "
           f"refinements {r.refinements}, final {r.final_residual:.3e}", flush=True)
     assert r.converged and r.final_residual <= 1e-7
-    an = a.numpy()
-    mn = m.numpy()
-    inside = classify(mn).astype(bool)
+    gn, an, mn = g.numpy(), a.numpy(), m.numpy()
+    for arr in (g, a, m):
+        arr.free()
+    cells = classify(mn)
+    del mn
+    inside = cells.astype(bool)
+    _check_inputs(z, cells, {"c0": (gn, an)})
     assert out[~inside].tobytes() == an[~inside].tobytes()
-    tight, rt = _solve(hip.make_desc(n, n, 1, 100000, 1e-12, 0.01, hip.PH_OP_SCREENED_POISSON),
-                       g, a, m)
-    err = float(np.abs(out - tight).max())
-    print(f"[large] screened 8192: max |u(1e-7) - u(1e-12)| = {err:.3e} "
-          f"({rt.iterations} sweeps, {rt.stop_reason} at {rt.final_residual:.2e})", flush=True)
-    assert rt.final_residual < 1e-11 and err < 1e-5
+    _against_fixture(z, "c0", out, cells, 0.01, gn, an, 1e-7, "screened 8192")
 
 
 def test_star_512_against_oracle_red_black(gpu, oracle):
     """512^2 fbm blend with the scaled star mask: GPU (FP32 sweeps + FP64
-    refinement, tuned omega) vs the oracle's FP64 red-black SOR, both to 1e-10."""
+    refinement, tuned omega) vs the oracle's FP64 red-black SOR and its tight
+    multigrid-CG solve, all to 1e-10."""
+    import math
+
     from paintop_amd.inputs import star_polygon
 
     n = 512
     sc = n / 4096
-    s = oracle.fbm(n, n, 0.125 / sc, 101, 6) * 0.5 + 0.5
-    t = oracle.fbm(n, n, 0.125 / sc, 201, 6) * 0.5 + 0.5
+    s = oracle.fbm(n, n, 0.125 / sc, 101, 6) * np.float32(0.5) + np.float32(0.5)
+    t = oracle.fbm(n, n, 0.125 / sc, 201, 6) * np.float32(0.5) + np.float32(0.5)
     verts = [(x * sc, y * sc) for (x, y) in
              star_polygon(2048, 2048, math.sqrt(0.3 * 4096 * 4096 / 1.5), 0.5, 6, 0.1, 301)]
     from paintop_amd.device import DeviceArray
     dm = DeviceArray((n, n), np.float32)
     gpu.polygon_mask_device(dm.ptr, n, n, verts)
     m = dm.numpy()
+    dm.free()
+    assert m.tobytes() == oracle.polygon_mask(n, n, verts).tobytes()
     desc = hip.make_desc(n, n, 1, 100000, 1e-10)
     out, r = gpu.poisson_solve(desc, s, t, m)
     cells = classify(m)
     rhs = np.where(cells == 1, laplacian(s), 0.0)
     ref, rep, _ = oracle.rbsor_omp(cells, rhs, t.astype(np.float64), 0.0, 100000, 1e-10, r.omega,
                                    stall_window=0)
+    tight, trep = oracle.tight_solve(cells, rhs, t.astype(np.float64), 0.0, tolerance=1e-12)
     inside = cells.astype(bool)
     d = np.abs(out.reshape(n, n).astype(np.float64) - ref)[inside]
+    dt = np.abs(out.reshape(n, n).astype(np.float64) - tight)[inside]
     print(f"[large] star 512: gpu {r.iterations} sweeps, oracle rb {rep.iterations} sweeps, "
-          f"max |gpu - oracle| {d.max():.3e}", flush=True)
+          f"max |gpu - oracle rb| {d.max():.3e}, max |gpu - tight| {dt.max():.3e}", flush=True)
     assert r.converged and rep.converged
-    assert d.max() < 1e-6
+    assert d.max() < 1e-6 and dt.max() < 1e-6
     assert out.reshape(n, n)[~inside].tobytes() == t[~inside].tobytes()
     r0 = residual_norm(cells, rhs, t, 0.0, t)
     assert residual_norm(cells, rhs, t, 0.0, out.reshape(n, n)) / r0 < 1e-5
