@@ -475,6 +475,7 @@ static void launch_sor(int K, int PF, int ntiles, int ninst, hipStream_t st, con
 // priority) and launched on the solve's stream, so a host iteration costs a
 // few graph launches instead of up to 128 kernel launches. The graphs hold
 // only kernel nodes. PAINTOP_HIP_GRAPH=0 launches the kernels directly.
+// The cache holds at most 64 graphs; launch_sor_batch evicts, never get().
 struct SweepGraphs {
   struct Entry {
     int K, PF, ntiles, ninst, hb, n;
@@ -496,7 +497,6 @@ struct SweepGraphs {
           e.partials == partials)
         return e.exec;
     if (!cs) HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    if (cache.size() >= 64) clear();
     hipGraph_t g = nullptr;
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
     try {
@@ -536,6 +536,12 @@ static void launch_sor_batch(int K, int PF, int ntiles, int ninst, hipStream_t s
   if (n == 1 || !use_graphs() || g_graphs.broken) {
     for (int i = 0; i < n; ++i) launch_sor(K, PF, ntiles, ninst, st, tiles, insts, ctl, partials, hb);
     return;
+  }
+  // a batch adds at most 7 graphs; make room before its first launch, once
+  // the stream (which may still run graphs of earlier batches) has drained
+  if (g_graphs.cache.size() + 7 > 64) {
+    HIP_OK(hipStreamSynchronize(st));
+    g_graphs.clear();
   }
   for (int b = 6; b >= 0; --b)
     if (n & (1 << b)) {
