@@ -173,6 +173,29 @@ ph_status ph_poisson_solve_device(const ph_solve_desc *desc, const float *d_sour
                                   float *d_out_candidate, void *stream, ph_report *report,
                                   ph_error *err);
 
+/* Device workspace of ph_poisson_solve_device_noalloc (128 MiB): enough for a
+ * 512x512 RGBA or a 1024x1024 single-channel solve. */
+#define PH_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* ph_poisson_solve_device without any allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory during a solve
+ * (serving loops, harnesses that forbid it): no hipMalloc, hipFree,
+ * hipHostMalloc or graph instantiation happens in the call. The solve's
+ * workspace is a fixed device array of PH_NOALLOC_SCRATCH_BYTES that each GPU
+ * the library runs on holds until the process ends; a solve that needs more
+ * is rejected before any launch (class Policy, E_HIP_SCRATCH_LIMIT). The
+ * plain solve only: PH_FLAG_WARM_START_NET and PH_FLAG_TIMING are rejected
+ * (class Schema, E_HIP_NOALLOC_FLAG). Results are bit-identical to
+ * ph_poisson_solve_device's. Every call on a GPU shares that GPU's array,
+ * which keeps the last call's contents (nothing in it is assumed zero); calls
+ * that overlapped on one GPU would race on it, and the library's lock
+ * serialises them (each call finishes its work on `stream` before it
+ * returns). */
+ph_status ph_poisson_solve_device_noalloc(const ph_solve_desc *desc, const float *d_source,
+                                          const float *d_target, const float *d_mask,
+                                          float *d_out_candidate, void *stream,
+                                          ph_report *report, ph_error *err);
+
 /* ph_poisson_solve_device from a caller's initial field d_start (same layout
  * as d_out_candidate; only its interior samples are read), e.g. the previous
  * frame's solution. No reference counterpart (the reference starts from the

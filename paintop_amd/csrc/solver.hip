@@ -151,17 +151,36 @@ void ensure_device() {
   HIP_OK(hipSetDevice(g_ctx.dev));
 }
 
-// Grow-only device workspace: one allocation reused across calls.
+// Device workspace of a solve. Normally a grow-only allocation reused across
+// calls; during a call of the allocation-free entry point it is the fixed
+// `scratch` block instead, and a solve that would need more is rejected before
+// any launch.
 struct Pool {
+  char *own = nullptr;  // hipMalloc'd, grow-only
+  size_t own_cap = 0;
+  char *ext = nullptr;  // fixed block in use instead (allocation-free calls), or null
+  size_t ext_cap = 0;
   char *base = nullptr;
   size_t cap = 0, used = 0;
   void reset(size_t need) {
-    if (need > cap) {
-      if (base) HIP_OK(hipFree(base));
-      base = nullptr;
-      cap = 0;
-      HIP_OK(hipMalloc(&base, need));
-      cap = need;
+    if (ext) {
+      if (need > ext_cap)
+        throw fail(PH_CLASS_POLICY, "E_HIP_SCRATCH_LIMIT",
+                   "the solve needs %zu bytes of device workspace; the allocation-free entry "
+                   "point has %zu (PH_NOALLOC_SCRATCH_BYTES)",
+                   need, ext_cap);
+      base = ext;
+      cap = ext_cap;
+    } else {
+      if (need > own_cap) {
+        if (own) HIP_OK(hipFree(own));
+        own = nullptr;
+        own_cap = 0;
+        HIP_OK(hipMalloc(&own, need));
+        own_cap = need;
+      }
+      base = own;
+      cap = own_cap;
     }
     used = 0;
   }
@@ -179,17 +198,34 @@ static Pool g_pool;
 // Pinned host memory for the readbacks of the sweep loop (control blocks,
 // residual norms): a small copy to pageable memory goes through a staging
 // buffer and waits on it; to pinned memory it is one DMA. Host code must not
-// change a pinned buffer while a copy from it is queued.
+// change a pinned buffer while a copy from it is queued. Allocation-free calls
+// use a fixed pageable block instead (ext).
 struct PinnedPool {
+  char *own = nullptr;
+  size_t own_cap = 0;
+  char *ext = nullptr;
+  size_t ext_cap = 0;
   char *base = nullptr;
   size_t cap = 0, used = 0;
   void reset(size_t need) {
-    if (need > cap) {
-      if (base) HIP_OK(hipHostFree(base));
-      base = nullptr;
-      cap = 0;
-      HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&base), need, hipHostMallocDefault));
-      cap = need;
+    if (ext) {
+      if (need > ext_cap)
+        throw fail(PH_CLASS_POLICY, "E_HIP_SCRATCH_LIMIT",
+                   "the solve needs %zu bytes of host staging; the allocation-free entry "
+                   "point has %zu",
+                   need, ext_cap);
+      base = ext;
+      cap = ext_cap;
+    } else {
+      if (need > own_cap) {
+        if (own) HIP_OK(hipHostFree(own));
+        own = nullptr;
+        own_cap = 0;
+        HIP_OK(hipHostMalloc(reinterpret_cast<void **>(&own), need, hipHostMallocDefault));
+        own_cap = need;
+      }
+      base = own;
+      cap = own_cap;
     }
     used = 0;
   }
@@ -204,6 +240,10 @@ struct PinnedPool {
   }
 };
 static PinnedPool g_pinned;
+
+// True while an allocation-free call runs: sweep batches are launched
+// directly (capturing and instantiating graphs may allocate).
+static bool g_noalloc = false;
 
 // ---------------------------------------------------------------------------
 // parameter validation (the same ranges and codes as resolve_controls,
@@ -533,7 +573,7 @@ static void launch_sor_batch(int K, int PF, int ntiles, int ninst, hipStream_t s
                              const PhTile *tiles, const PhInst *insts, PhCtl *ctl,
                              double *partials, int hb, int n) {
   if (n <= 0 || ntiles <= 0) return;
-  if (n == 1 || !use_graphs() || g_graphs.broken) {
+  if (n == 1 || !use_graphs() || g_graphs.broken || g_noalloc) {
     for (int i = 0; i < n; ++i) launch_sor(K, PF, ntiles, ninst, st, tiles, insts, ctl, partials, hb);
     return;
   }
@@ -1999,6 +2039,52 @@ static void solve_device(const ph_solve_desc *d, const float *src, const float *
 
 using namespace ph;
 
+// ---------------------------------------------------------------------------
+// allocation-free entry point
+
+// The workspace of ph_poisson_solve_device_noalloc: a fixed device array of
+// PH_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the
+// process ends. Host code never passes `scratch` itself (in host code the name
+// stands for a host-side shadow, not a device address): the device address
+// comes from hipGetSymbolAddress, once per device, and is kept per device.
+static __device__ __align__(256) char scratch[PH_NOALLOC_SCRATCH_BYTES];
+static void *g_scratch_dev[64];
+// Host staging of the allocation-free calls' readbacks (pageable).
+static char g_noalloc_host[1 << 20] __attribute__((aligned(256)));
+
+static void *noalloc_scratch() {
+  const int dev = g_ctx.dev;  // current device (ensure_device made it current)
+  if (dev < 0 || dev >= 64)
+    throw fail(PH_CLASS_POLICY, "E_HIP_LIMIT", "device index %d out of range", dev);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw fail(PH_CLASS_EXECUTION, "E_HIP_RUNTIME", "hipGetSymbolAddress(scratch) failed: %s",
+                 hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  return g_scratch_dev[dev];
+}
+
+// Routes the pools to the fixed blocks for the duration of one call.
+struct NoAllocScope {
+  explicit NoAllocScope(void *dev_block) {
+    g_pool.ext = static_cast<char *>(dev_block);
+    g_pool.ext_cap = PH_NOALLOC_SCRATCH_BYTES;
+    g_pinned.ext = g_noalloc_host;
+    g_pinned.ext_cap = sizeof(g_noalloc_host);
+    g_noalloc = true;
+  }
+  ~NoAllocScope() {
+    g_pool.ext = nullptr;
+    g_pool.ext_cap = 0;
+    g_pinned.ext = nullptr;
+    g_pinned.ext_cap = 0;
+    g_noalloc = false;
+  }
+};
+
 extern "C" int32_t ph_abi_version(void) { return PH_ABI_VERSION; }
 
 extern "C" void ph_force_unavailable(int32_t forced) {
@@ -2058,6 +2144,24 @@ extern "C" ph_status ph_poisson_solve_device(const ph_solve_desc *desc, const fl
   return run_guarded(desc, rep, err, [&]() {
     if ((size_t)desc->width * desc->height > 0 && (!d_source || !d_target || !d_mask || !d_out))
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
+    solve_device(desc, d_source, d_target, d_mask, d_out, (hipStream_t)stream, rep);
+  });
+}
+
+extern "C" ph_status ph_poisson_solve_device_noalloc(const ph_solve_desc *desc,
+                                                     const float *d_source, const float *d_target,
+                                                     const float *d_mask, float *d_out,
+                                                     void *stream, ph_report *rep,
+                                                     ph_error *err) {
+  return run_guarded(desc, rep, err, [&]() {
+    if (desc->flags & (PH_FLAG_WARM_START_NET | PH_FLAG_TIMING))
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_NOALLOC_FLAG",
+                 "the allocation-free solve takes neither PH_FLAG_WARM_START_NET nor "
+                 "PH_FLAG_TIMING");
+    if ((size_t)desc->width * desc->height > 0 && (!d_source || !d_target || !d_mask || !d_out))
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
+    void *blk = noalloc_scratch();
+    NoAllocScope scope(blk);
     solve_device(desc, d_source, d_target, d_mask, d_out, (hipStream_t)stream, rep);
   });
 }

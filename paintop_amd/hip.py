@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "ph_force_unavailable",
     "ph_poisson_solve",
     "ph_poisson_solve_device",
+    "ph_poisson_solve_device_noalloc",
     "ph_fbm_device",
     "ph_polygon_mask_device",
     "ph_ellipse_mask_device",
@@ -191,6 +192,10 @@ def lib():
     L.ph_poisson_solve_device.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp, vp,
                                           ctypes.POINTER(PhReport), ctypes.POINTER(PhError)]
     L.ph_poisson_solve_device.restype = ctypes.c_int
+    L.ph_poisson_solve_device_noalloc.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp,
+                                                  vp, ctypes.POINTER(PhReport),
+                                                  ctypes.POINTER(PhError)]
+    L.ph_poisson_solve_device_noalloc.restype = ctypes.c_int
     L.ph_poisson_solve_start_device.argtypes = [ctypes.POINTER(PhSolveDesc), vp, vp, vp, vp, vp,
                                                 vp, ctypes.POINTER(PhReport),
                                                 ctypes.POINTER(PhError)]
@@ -378,6 +383,29 @@ def poisson_solve_device(desc: PhSolveDesc, d_source: int, d_target: int, d_mask
         st = lib().ph_poisson_solve_start_device(ctypes.byref(desc), d_source, d_target, d_mask,
                                                  d_start, d_out, stream or None,
                                                  ctypes.byref(rep), ctypes.byref(err))
+    _check(st, err)
+    return _result(rep, hist)
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20  # PH_NOALLOC_SCRATCH_BYTES
+
+
+def poisson_solve_device_noalloc(desc: PhSolveDesc, d_source: int, d_target: int, d_mask: int,
+                                 d_out: int, stream: int = 0,
+                                 want_history: bool = True) -> SolveResult:
+    """ph_poisson_solve_device_noalloc: the plain device solve with no
+    allocation in the call (its workspace is a fixed device array of
+    NOALLOC_SCRATCH_BYTES)."""
+    cap = int(desc.max_iterations)
+    hist = np.zeros(max(cap, 1), dtype=np.float64)
+    rep = PhReport()
+    if want_history:
+        rep.residual_history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rep.history_capacity = cap
+    err = PhError()
+    st = lib().ph_poisson_solve_device_noalloc(ctypes.byref(desc), d_source, d_target, d_mask,
+                                               d_out, stream or None, ctypes.byref(rep),
+                                               ctypes.byref(err))
     _check(st, err)
     return _result(rep, hist)
 
