@@ -199,3 +199,26 @@ def test_star_polygon_area_is_about_30_percent():
     x, y = v[:, 0], v[:, 1]
     area = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
     assert 0.25 < area / (n * n) < 0.35
+
+
+def test_integration_rust_binding_matches_the_header():
+    """The Rust FFI block shown in INTEGRATION.md declares every entry point of
+    include/paintop_hip.h, and its PhSolveDesc / PhReport / PhBatchItem fields
+    are the header's, in order."""
+    md = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    rust_fns = set(re.findall(r"pub fn (ph_[a-z0-9_]+)\(", md))
+    assert rust_fns == set(declared_symbols())
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "paintop_hip.h")).read(),
+                 flags=re.S)
+
+    def c_fields(name):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), hdr, re.S).group(1)
+        return re.findall(r"\**([a-z_0-9]+)(?:\[[^\]]*\])*;", body)
+
+    def rust_fields(name):
+        body = re.search(r"pub struct %s \{(.*?)\n?\}" % name, md, re.S).group(1)
+        return re.findall(r"pub ([a-z_0-9]+):", body)
+
+    for c, r in (("ph_solve_desc", "PhSolveDesc"), ("ph_report", "PhReport"),
+                 ("ph_batch_item", "PhBatchItem")):
+        assert c_fields(c) == rust_fields(r), (c, c_fields(c), rust_fields(r))
