@@ -319,3 +319,38 @@ def outside_scan(before: np.ndarray, after: np.ndarray, mask: np.ndarray, decode
                 worst_pixel=None if r.worst_index < 0 else xy(r.worst_index),
                 worst_delta=float(r.worst_delta),
                 locations=[xy(r.locations[k]) for k in range(r.n_locations)])
+
+
+def sor_spectral_radius(mu: float, lam: float, omega: float) -> float:
+    """Spectral radius of lexicographic SOR (poisson.rs:244-265) on the 5-point
+    system with smallest operator eigenvalue mu (of (4 + lam) I - N on the
+    interior): the 5-point matrix in natural order is consistently ordered, so
+    Young's relation (l + omega - 1)^2 = l omega^2 rho_J^2 holds with the
+    Jacobi radius rho_J = (4 - mu) / (4 + lam). Below the optimal omega the
+    largest root is ((omega rho_J + sqrt(omega^2 rho_J^2 - 4 (omega - 1))) / 2)^2;
+    above it the radius is omega - 1."""
+    import math
+
+    rj = (4.0 - mu) / (4.0 + lam)
+    disc = omega * omega * rj * rj - 4.0 * (omega - 1.0)
+    if disc <= 0.0:
+        return omega - 1.0
+    return ((omega * rj + math.sqrt(disc)) / 2.0) ** 2
+
+
+def project_sor_sweeps(mu: float, lam: float, omega: float, tolerance: float) -> float:
+    """Sweeps lexicographic SOR needs for its relative residual to fall to
+    `tolerance`, projected from its asymptotic rate (the spectral radius)."""
+    import math
+
+    rho = sor_spectral_radius(mu, lam, omega)
+    return math.log(tolerance) / math.log(rho)
+
+
+def mu_from_optimal_omega(omega_opt: float, lam: float = 0.0) -> float:
+    """Invert omega_opt = 2 / (1 + sqrt(1 - rho_J^2)) for the smallest operator
+    eigenvalue mu = 4 - (4 + lam) rho_J."""
+    import math
+
+    rj = math.sqrt(max(0.0, 1.0 - (2.0 / omega_opt - 1.0) ** 2))
+    return 4.0 - (4.0 + lam) * rj

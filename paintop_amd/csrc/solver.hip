@@ -687,7 +687,11 @@ static TuneKey coarse_mask(const int *counts, int f, int cw, int ch, std::vector
   return TuneKey{cw, ch, f, hsh};
 }
 
+// PAINTOP_HIP_TUNE_CACHE=0: every solve pays the tuner (the cost a new mask
+// sees), e.g. to time it.
 static bool tune_lookup(const TuneKey &key, double &mu) {
+  const char *ev = getenv("PAINTOP_HIP_TUNE_CACHE");
+  if (ev && ev[0] == '0') return false;
   for (const auto &e : g_tune_cache)
     if (e.first == key) {
       mu = e.second;

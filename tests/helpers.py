@@ -64,3 +64,41 @@ def residual_norm(cells, rhs, anchor, lam, field) -> float:
     nsum[1:-1, 1:-1] = f[1:-1, :-2] + f[1:-1, 2:] + f[:-2, 1:-1] + f[2:, 1:-1]
     r = nsum - (4.0 + lam) * f - (rhs - lam * anchor.astype(np.float64))
     return float(np.sqrt(np.sum(r[cells.astype(bool)] ** 2)))
+
+
+def star_case(n):
+    """The 4096^2 seamless-clone workload scaled to n^2 (star mask, fbm with the
+    same features relative to the image): cells, rhs, anchor."""
+    import math
+
+    from oracle import oracle
+    from paintop_amd.inputs import star_polygon
+
+    sc = n / 4096
+    verts = [(x * sc, y * sc) for (x, y) in
+             star_polygon(2048, 2048, math.sqrt(0.3 * 4096 * 4096 / 1.5), 0.5, 6, 0.1, 301)]
+    cells = classify(oracle.polygon_mask(n, n, verts))
+    s = oracle.fbm(n, n, 0.125 / sc, 101, 6) * np.float32(0.5) + np.float32(0.5)
+    t = oracle.fbm(n, n, 0.125 / sc, 201, 6) * np.float32(0.5) + np.float32(0.5)
+    return cells, np.where(cells == 1, laplacian(s), 0.0), t.astype(np.float64)
+
+
+def smallest_eigenvalue(cells):
+    """Smallest eigenvalue of the 5-point operator 4I - N on the interior cells
+    (scipy shift-invert Lanczos)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    ys, xs = np.nonzero(cells)
+    idx = -np.ones(cells.shape, np.int64)
+    idx[ys, xs] = np.arange(ys.size)
+    rows, cols = [np.arange(ys.size)], [np.arange(ys.size)]
+    vals = [np.full(ys.size, 4.0)]
+    for dy, dx in ((0, 1), (0, -1), (1, 0), (-1, 0)):
+        nb = idx[ys + dy, xs + dx]
+        k = nb >= 0
+        rows.append(np.arange(ys.size)[k])
+        cols.append(nb[k])
+        vals.append(-np.ones(int(k.sum())))
+    A = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))))
+    return float(spla.eigsh(A, k=1, sigma=0.0, which="LM", return_eigenvectors=False)[0])
