@@ -62,10 +62,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 
-// Stop rules for one finished sweep m (poisson.rs:202-229). Returns true when
+// Stop rules for one finished sweep m (poisson.rs:202-229), given its squared
+// residual res2 and relative residual rel = sqrt(res2) / r0. Returns true when
 // the solve must halt at m.
-__device__ bool finalize_sweep(PhCtl &C, const PhInst &I, uint32_t m, double res2) {
-  const double rel = sqrt(res2) / C.r0;
+__device__ bool finalize_sweep(PhCtl &C, const PhInst &I, uint32_t m, double res2, double rel) {
   C.history[m - 1] = rel;
   C.sweeps_final = m;
   C.last_rel = rel;
@@ -96,10 +96,24 @@ __device__ bool finalize_sweep(PhCtl &C, const PhInst &I, uint32_t m, double res
   return false;
 }
 
-// Control for one instance after all of its tiles finished a launch. A holds
-// the instance's residual sums of the launch's K half-sweeps; runs on one lane.
+// Squared residual of sweep S + j of a launch that starts at sweep S (A: the
+// launch's K per-half-sweep sums; j = 0 is the previous launch's last sweep,
+// whose red half came in this launch).
 template <int K>
-__device__ void control(PhCtl &C, const PhInst &I, const double *A) {
+__device__ __forceinline__ double sweep_res2(const PhCtl &C, const PhInst &I, const double *A,
+                                             int j) {
+  if (j == 0) return C.prev_black + A[0];
+  const double wb = ph_omega_at(I, 2 * (uint64_t)C.sweeps_done + 2 * j - 1);  // black half of S + j
+  return (1.0 - wb) * (1.0 - wb) * A[2 * j - 1] + A[2 * j];
+}
+
+// Control for one instance after all of its tiles finished a launch. A holds
+// the instance's residual sums of the launch's K half-sweeps, R2 / RL the
+// squared and relative residual of each sweep (computed by parallel lanes);
+// the rules run on one lane.
+template <int K>
+__device__ void control(PhCtl &C, const PhInst &I, const double *A, const double *R2,
+                        const double *RL) {
   const uint32_t S = C.sweeps_done;
   const int k_run = min(K, (int)(2 * (C.sweep_limit - S)));
   const int nsw = k_run >> 1;
@@ -113,11 +127,8 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *A) {
   const uint32_t in_buf = C.cur_buf;
   const uint64_t h0 = 2 * (uint64_t)S;  // first half-sweep of this launch
   bool stop = false;
-  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, I, S, C.prev_black + A[0]);
-  for (int j = 1; j < nsw && !stop; ++j) {
-    const double wb = ph_omega_at(I, h0 + 2 * j - 1);  // black half-sweep of sweep S + j
-    stop = finalize_sweep(C, I, S + j, (1.0 - wb) * (1.0 - wb) * A[2 * j - 1] + A[2 * j]);
-  }
+  if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, I, S, R2[0], RL[0]);
+  for (int j = 1; j < nsw && !stop; ++j) stop = finalize_sweep(C, I, S + j, R2[j], RL[j]);
   const double wl = ph_omega_at(I, h0 + 2 * nsw - 1);
   C.prev_black = (1.0 - wl) * (1.0 - wl) * A[2 * nsw - 1];
   if (stop) {
@@ -342,27 +353,49 @@ __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict
                                                         const double *__restrict__ partials,
                                                         int ninst, int ntiles) {
   __shared__ double A[K];
+  __shared__ double R2[K / 2], RL[K / 2];
   const int a = blockIdx.x;
   if (a >= ninst) return;
   PhCtl &C = ctl[a];
-  if (C.halt != PH_RUN) return;
+  if (C.halt != PH_RUN) return;  // uniform across the block
   const PhInst &I = insts[a];
   const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the rules run on register copies (a chain of dependent global loads and
+  // stores would dominate this kernel otherwise), loaded before the sums
+  PhCtl c;
+  PhInst in;
+  if (threadIdx.x == 0) {
+    c = C;
+    in = I;
+  }
+  // wave s adds the tiles' sums of half-sweep s in tile order: each lane loads
+  // its tiles in batches of 16 (one memory latency for up to 1024 tiles), then
+  // adds them in order (a past-the-end slot adds 0, which leaves a sum of
+  // squares unchanged)
   double v = 0.0;
   const double *ps = partials + (size_t)s * ntiles;
-  // unrolled so the loads of several tiles are in flight at once; the adds
-  // keep their order
-#pragma unroll 8
-  for (int t = I.tile_begin + lane; t < I.tile_end; t += 64) v += ps[t];
+  const int te = I.tile_end;
+  constexpr int B = 16;
+  for (int t = I.tile_begin + lane; t < te; t += 64 * B) {
+    double q[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) q[k] = t + 64 * k < te ? ps[t + 64 * k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < B; ++k) v += q[k];
+  }
   v = wave_sum(v);
   if (lane == 0) A[s] = v;
   __syncthreads();
+  // the sweeps' residuals in parallel (lane j: sweep S + j)
+  if (threadIdx.x < K / 2) {
+    const int j = threadIdx.x;
+    const double r2 = sweep_res2<K>(C, I, A, j);
+    R2[j] = r2;
+    RL[j] = sqrt(r2) / C.r0;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    // one lane runs the stop rules on register copies: a chain of dependent
-    // global loads and stores would dominate this kernel otherwise
-    PhCtl c = C;
-    const PhInst in = I;
-    control<K>(c, in, A);
+    control<K>(c, in, A, R2, RL);
     C = c;
   }
 }
