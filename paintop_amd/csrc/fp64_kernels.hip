@@ -332,6 +332,45 @@ __device__ __forceinline__ double rows_sum(const double *__restrict__ rs, int nr
 // The refinement of many instances in three launches, not three per instance
 // (a batch of small images refines hundreds of instances a pass).
 // x += e[buf], e[buf] = 0 at the interior cells of every listed row.
+// ||r(x + e)||^2 of a plane row in FP64, written nowhere: the residual of the
+// state a member of the warm-start race has reached (x and the correction e
+// it carries), the value a refinement would compute, judged without one so
+// neither member's trajectory changes. e is 0 off the interior.
+__device__ __forceinline__ double probe_residual_row(const PhInst &I, const float *e, int py) {
+  double s = 0.0;
+  if (py >= 1 && py < I.ph - 1) {
+    const size_t row = (size_t)py * I.pitch;
+    const size_t pitch = (size_t)I.pitch;
+    for (int px = 2 * threadIdx.x; px < I.pw; px += 2 * blockDim.x) {
+      const size_t p = row + px;
+      const uchar2 c = *reinterpret_cast<const uchar2 *>(I.cls + p);
+      if (!(c.x | c.y)) continue;
+      const double xc0 = I.x[p] + (double)e[p], xc1 = I.x[p + 1] + (double)e[p + 1];
+      const double xl = I.x[p - 1] + (double)e[p - 1], xr = I.x[p + 2] + (double)e[p + 2];
+      const double xu0 = I.x[p - pitch] + (double)e[p - pitch];
+      const double xu1 = I.x[p + 1 - pitch] + (double)e[p + 1 - pitch];
+      const double xd0 = I.x[p + pitch] + (double)e[p + pitch];
+      const double xd1 = I.x[p + 1 + pitch] + (double)e[p + 1 + pitch];
+      const double n0 = ((xl + xc1) + xu0) + xd0;
+      const double n1 = ((xc0 + xr) + xu1) + xd1;
+      const double r0 = fma(I.dg64, -xc0, n0) - I.bp[p];
+      const double r1 = fma(I.dg64, -xc1, n1) - I.bp[p + 1];
+      if (c.x) s = fma(r0, r0, s);
+      if (c.y) s = fma(r1, r1, s);
+    }
+  }
+  return s;
+}
+__global__ __launch_bounds__(256) void k_probe_residual_spans(const PhInst *__restrict__ insts,
+                                                              const PhSpan *__restrict__ spans,
+                                                              int n,
+                                                              double *__restrict__ row_sums) {
+  const int b = xcd_row_group(blockIdx.x, gridDim.x);
+  const PhSpan S = spans[ph_span_find(spans, n, b)];
+  const PhInst &I = insts[S.slot];
+  const double s = block_sum(probe_residual_row(I, I.e[S.buf], b - S.row0));
+  if (threadIdx.x == 0) row_sums[b] = s;
+}
 __global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__restrict__ insts,
                                                                 const PhSpan *__restrict__ spans,
                                                                 int n) {

@@ -55,6 +55,7 @@ __global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhIt
 __global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
 __global__ void k_residual_spans(const PhInst *, const PhSpan *, int, double *);
 __global__ void k_residual_norm_spans(const PhSpan *, const double *, double *);
+__global__ void k_probe_residual_spans(const PhInst *, const PhSpan *, int, double *);
 __global__ void k_scatter(PhInst, float *);
 template <int CC>
 __global__ void k_scatter_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
@@ -402,6 +403,13 @@ static uint32_t warm_ramp_sweeps(double omega, bool cycles) {
 static uint32_t warm_probe_sweeps(uint32_t ramp) {
   return std::min<uint32_t>(96, std::max<uint32_t>(8, ramp / 4));
 }
+// The race is judged on the FP64 residual of each member's state at the
+// probe sweep. A warm start keeps the lead only when its residual is at most
+// RACE_MARGIN of the plain start's; a smaller lead extends the race to twice
+// the probe, at most RACE_MAX_EXTENSIONS times, after which the smaller
+// residual wins (the shipped network leads ~20-fold at the first probe).
+static const double RACE_MARGIN = 0.25;
+static const int RACE_MAX_EXTENSIONS = 4;
 // A warm start whose initial residual exceeds the anchor start's this many
 // times is dropped without a race. A coarse prediction leaves the fine
 // boundary texture unresolved, so good starts sit at 5-8x (the shipped
@@ -1461,11 +1469,22 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       Cc.since += 1;
       return Cc.since >= Cc.stall_window;
     };
+    // warm-start race state: members held at the probe, races decided, the
+    // probe residuals, the extensions taken and the members awaiting a probe
+    std::vector<char> held(nact, 0), hold_refine(nact, 0), resolved(inst.size(), 0);
+    std::vector<double> pval(inst.size(), -1.0);
+    std::vector<int> race_ext(inst.size(), 0), probe_pend;
     // sweep limit of a running instance: the end of its correction cycle, if any
     auto run_limit = [&](int a) -> uint32_t {
       const PhCtl &Cc = ctl[a];
-      const uint32_t c = cyc_next[act[a]];
-      return c > Cc.sweeps_done ? std::min(Cc.max_it, c) : Cc.max_it;
+      const int k = act[a];
+      uint32_t lim = Cc.max_it;
+      const uint32_t c = cyc_next[k];
+      if (c > Cc.sweeps_done) lim = std::min(lim, c);
+      // an open warm-start race holds both members at its probe
+      if (twin[k] >= 0 && !resolved[k] && (uint32_t)probe[k] > Cc.sweeps_done)
+        lim = std::min<uint32_t>(lim, (uint32_t)probe[k]);
+      return lim;
     };
     auto resume = [&](int a, double gnorm) {
       PhCtl &Cc = ctl[a];
@@ -1478,8 +1497,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     };
 
     const bool tracing = trace_on();
-    std::vector<char> held(nact, 0), hold_refine(nact, 0), resolved(inst.size(), 0);
-    std::vector<double> pval(inst.size(), -1.0);
     std::vector<int> slot(inst.size(), -1);
     for (int a = 0; a < nact; ++a) slot[act[a]] = a;
     const int PF = prefetch_rows(K);
@@ -1716,12 +1733,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
             break;
           case PH_HALT_LIMIT:
             if (twin[act[a]] >= 0 && !resolved[act[a]] && Cc.sweeps_done < Cc.max_it) {
-              // warm-start race probe: judged on the device's own residual of
-              // the last finalised sweep, with no refinement, so neither
-              // member's trajectory changes
+              // warm-start race probe: judged on the FP64 residual of the
+              // member's state (x + e), computed without a refinement so
+              // neither member's trajectory changes
               held[a] = 1;
               hold_refine[a] = Cc.refine_pending ? 1 : 0;
-              pval[act[a]] = Cc.last_rel * Cc.r0;
+              probe_pend.push_back(a);
               if (tracing)
                 fprintf(stderr, "[race] hold inst %d (%s) at sweep %u (final %u) rel %.3e\n",
                         act[a], is_twin[act[a]] ? "plain" : "warm", Cc.sweeps_done,
@@ -1793,14 +1810,57 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         }
         if (!corr.empty()) correct();
       }
+      if (!probe_pend.empty()) {
+        // the FP64 residuals of the race members held at their probe
+        ref.clear();
+        for (int a : probe_pend)
+          ref.add(a, ai[a].ph, 0, (int)ctl[a].cur_buf, act[a], nullptr, ctl[a].r0);
+        ref.upload(ref_spans_d, st);
+        const int nref = (int)ref.h.size();
+        hipLaunchKernelGGL(k_probe_residual_spans, dim3(ref.rows), dim3(256), 0, st, inst_d,
+                           ref_spans_d, nref, ref_sums_d);
+        hipLaunchKernelGGL(k_residual_norm_spans, dim3(nref), dim3(256), 0, st, ref_spans_d,
+                           ref_sums_d, resv_d);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * inst.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (int a : probe_pend) {
+          pval[act[a]] = resv[2 * act[a]];
+          if (tracing)
+            fprintf(stderr, "[race] inst %d (%s) at sweep %u: fp64 %.3e\n", act[a],
+                    is_twin[act[a]] ? "plain" : "warm", ctl[a].sweeps_done,
+                    resv[2 * act[a]] / ctl[a].r0);
+        }
+        probe_pend.clear();
+      }
       {
         // warm-start races whose two members both have a value: the smaller
-        // residual continues (a tie goes to the plain start)
+        // residual continues (a tie goes to the plain start); a warm lead
+        // smaller than RACE_MARGIN extends the race while both members run
         for (int k = 0; k < (int)inst.size(); ++k) {
           if (is_twin[k] || twin[k] < 0 || resolved[k]) continue;
           const int kt = twin[k];
           if (pval[k] < 0.0 || pval[kt] < 0.0) continue;
           const bool warm_wins = pval[k] < pval[kt];
+          if (warm_wins && pval[k] > RACE_MARGIN * pval[kt] && race_ext[k] < RACE_MAX_EXTENSIONS &&
+              !inst[k].R.done && !inst[kt].R.done && held[slot[k]] && held[slot[kt]] &&
+              (uint32_t)probe[k] < d->max_iterations) {
+            race_ext[k] += 1;
+            probe[k] = probe[kt] = (int)std::min<uint32_t>(2u * (uint32_t)probe[k],
+                                                           d->max_iterations);
+            if (tracing)
+              fprintf(stderr, "[race] extend inst %d: warm %.3e plain %.3e -> probe %d\n", k,
+                      pval[k], pval[kt], probe[k]);
+            pval[k] = pval[kt] = -1.0;
+            for (const int a : {slot[k], slot[kt]}) {
+              held[a] = 0;
+              PhCtl &Cc = ctl[a];
+              Cc.sweep_limit = run_limit(a);
+              Cc.halt = hold_refine[a] ? PH_HALT_REFINE : PH_RUN;
+            }
+            continue;
+          }
           const int win = warm_wins ? k : kt, lose = warm_wins ? kt : k;
           if (tracing)
             fprintf(stderr, "[race] judge inst %d: warm %.3e (sweep %u) plain %.3e (sweep %u) -> %s\n",

@@ -317,11 +317,68 @@ def test_trained_warm_start_ablation(gpu, n):
         assert b[..., c][~inside].tobytes() == t[..., c][~inside].tobytes()
     assert float(np.abs(a - b).max()) < 2e-4
     # an untrained network's start that loses the race leaves the plain solve,
-    # bit for bit; one that wins still has to converge
+    # bit for bit; one that wins has to converge in no more sweeps than the
+    # plain start
     if ru.warm_start_used == 0:
         assert o_u.numpy().tobytes() == a.tobytes() and ru.residual_history == rp.residual_history
     else:
         assert ru.converged and ru.final_residual <= 1e-6
+        assert ru.iterations <= rp.iterations
+
+
+@pytest.mark.parametrize("scale", [0.05, 0.2, 0.5, 1.0])
+def test_degraded_network_never_costs_sweeps(gpu, scale):
+    """The trained weights with multiplicative noise of growing scale (seeded):
+    whatever the start's quality, a start the race keeps needs no more sweeps
+    than the plain start on its channel, and a start it drops leaves the plain
+    solve, bit for bit. The race is judged on the FP64 residual of both members
+    at the same sweep and extended while the warm lead is under 4x."""
+    from paintop_amd import warmstart
+    from paintop_amd.device import DeviceArray
+    from paintop_amd.inputs import seamless_clone_case
+
+    n = 1024
+    src, tgt, msk = seamless_clone_case(n, channels=3)
+    w0 = warmstart.load_weights()
+    rng = np.random.default_rng(7000 + int(scale * 100))
+    noisy = (w0 * (1.0 + scale * rng.standard_normal(w0.shape))).astype(w0.dtype)
+    d = hip.make_desc(n, n, 3, 100000, 1e-6)
+    o_p = DeviceArray(tgt.shape, np.float32)
+    rp = gpu.poisson_solve_device(d, src.ptr, tgt.ptr, msk.ptr, o_p.ptr)
+    reps_p = []
+    for c in range(3):
+        dc = hip.make_desc(n, n, 1, 100000, 1e-6)
+        one = [DeviceArray.from_numpy(np.ascontiguousarray(v.numpy()[..., c]))
+               for v in (src, tgt)]
+        oc = DeviceArray((n, n), np.float32)
+        reps_p.append((one, oc, gpu.poisson_solve_device(dc, one[0].ptr, one[1].ptr, msk.ptr,
+                                                          oc.ptr)))
+    gpu.warm_start_set_weights(noisy)
+    try:
+        d.flags = hip.PH_FLAG_WARM_START_NET
+        o_w = DeviceArray(tgt.shape, np.float32)
+        rw = gpu.poisson_solve_device(d, src.ptr, tgt.ptr, msk.ptr, o_w.ptr)
+        per = []
+        for c, (one, oc, r1) in enumerate(reps_p):
+            dc = hip.make_desc(n, n, 1, 100000, 1e-6, flags=hip.PH_FLAG_WARM_START_NET)
+            ow = DeviceArray((n, n), np.float32)
+            per.append((r1, gpu.poisson_solve_device(dc, one[0].ptr, one[1].ptr, msk.ptr, ow.ptr),
+                        oc, ow))
+    finally:
+        gpu.warm_start_set_weights(None)
+    print(f"[degraded] noise {scale}: plain {rp.iterations} sweeps, net {rw.iterations} "
+          f"(kept {rw.warm_start_used}, start rel {rw.warm_start_rel:.2e}); per channel "
+          + ", ".join(f"{r1.iterations}->{r2.iterations}({r2.warm_start_used})"
+                      for r1, r2, _a, _b in per), flush=True)
+    assert rw.converged and rw.final_residual <= 1e-6
+    assert rw.iterations <= rp.iterations
+    for r1, r2, oc, ow in per:
+        assert r2.converged and r2.final_residual <= 1e-6
+        if r2.warm_start_used:
+            assert r2.iterations <= r1.iterations
+        else:
+            assert ow.numpy().tobytes() == oc.numpy().tobytes()
+            assert r2.residual_history == r1.residual_history
 
 
 def test_coarse_corrections_cut_sweeps(gpu, monkeypatch):
