@@ -485,9 +485,15 @@ static uint32_t stall_window_for(double omega, bool tuned) {
 }
 
 // One sweep launch: the tiles, then the per-instance control.
+static int sor_waves() {
+  const char *e = getenv("PAINTOP_HIP_SOR_WAVES");
+  const int w = e ? atoi(e) : 1;
+  return (w == 2 || w == 4 || w == 8) ? w : 1;
+}
 static void launch_sor(int K, int PF, int ntiles, int ninst, hipStream_t st, const PhTile *tiles,
                        const PhInst *insts, PhCtl *ctl, double *partials, int hb) {
-  const dim3 grid(ntiles), block(64), cgrid(ninst), cblock(64 * K);
+  const int W = sor_waves();
+  const dim3 grid((ntiles + W - 1) / W), block(64 * W), cgrid(ninst), cblock(64 * K);
 #define PH_SOR(KK, PP)                                                                      \
   hipLaunchKernelGGL((k_sor<KK, PP>), grid, block, 0, st, tiles, ntiles, insts, ctl, partials, hb)
   switch (K * 8 + PF) {
@@ -796,16 +802,18 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   const int nitem = (int)items.size();
   const int K = d->half_sweeps ? (int)d->half_sweeps : 16;
   const int WV = 256 - 2 * K;
-  // Tile height: one wave per tile streams hb + 2K rows per launch, and at
-  // 235 VGPRs two waves fit a SIMD (2048 on the chip), so a launch takes
-  // ceil(live tiles / 2048) rounds of an (hb + 2K)-row pass, and fewer, taller
-  // tiles hide latency worse. Measured plain solves (ms): 1024^2 RGBA 15.1 at
-  // 16 rows against 18.4 at 32; 2048^2 23.7 at 16, 27.0 at 32; 4096^2 66.7
-  // at 40, 71.8 at 48, 93-100 at 24-36 (a third round), 84-112 at 64-128;
-  // 8192^2 screened and the 512-blend batch equal at 40 and 48.
+  const int PF = prefetch_rows(K);
+  // Tile height: one wave per tile streams its hb core rows and 2K halo rows
+  // per launch in periods of U = K + PF + 1 row steps; with hb a multiple of
+  // U the last period ends on the last row (otherwise it runs past it). Two
+  // waves fit a SIMD (2048 on the chip), so a launch takes about
+  // ceil(live tiles / 2048) rounds of one tile pass, and fewer, taller tiles
+  // hide latency worse: 2U rows for large solves, U for small ones.
+  const int U = K + PF + 1;
   uint64_t work_px = 0;
   for (const ItemIn &it : items) work_px += (uint64_t)it.W * it.H * it.C;
-  const int hb = d->tile_rows ? (int)d->tile_rows : (work_px >= (32ull << 20) ? 40 : 16);
+  const int hb = d->tile_rows ? (int)d->tile_rows
+                               : (work_px >= (32ull << 20) ? 2 : 1) * ((16 + U - 1) / U) * U;
   const bool timing = (d->flags & PH_FLAG_TIMING) != 0;
   const bool tuned = d->omega <= 0.0;
   const bool want_warm =
@@ -1417,6 +1425,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         ai[a].tile_end = (int)tl.size();
       }
       ntiles = (int)tl.size();
+      if (trace_on()) fprintf(stderr, "[tiles] %d tiles of %d rows\n", ntiles, hb);
       if (ntiles)
         HIP_OK(hipMemcpyAsync(tiles_d, tl.data(), ntiles * sizeof(PhTile), hipMemcpyHostToDevice,
                               st));
@@ -1499,7 +1508,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     const bool tracing = trace_on();
     std::vector<int> slot(inst.size(), -1);
     for (int a = 0; a < nact; ++a) slot[act[a]] = a;
-    const int PF = prefetch_rows(K);
     int batch = 4;
     int remaining = nact;
     double guard = 0;

@@ -180,14 +180,26 @@ __device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t rs, int off, f
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), rs, off, 0, 0);
 }
 
-// One ring period of U row steps of k_sor, every step unconditional. At step
-// t the wave applies half-sweep s (s = 1..K, red for odd s) to row t - s + 1,
-// in increasing s; HEAD: the period holds steps t < 2K - 1, the only ones with
-// updates outside the dependence cone (rows i < s), which are skipped (the
-// test is uniform across the wave, and the skipped updates are closed under
-// dependence, so no core value or residual sum changes). Rows past the tile
-// read its last row again and rows outside the valid core are not stored;
-// steps past the last useful one touch only rows outside the core's cone.
+// The ring's lead: the first period starts at step -M, so that the periods
+// holding the steps t < 2K - 1 end exactly at t = 2K - 2 and every later
+// period is a whole ring of steps (ring slot of row r: (r + M) mod U).
+template <int K, int PF>
+constexpr int ring_lead() {
+  constexpr int U = K + PF + 1;
+  return (U - (2 * K - 1) % U) % U;
+}
+
+// One ring period of U row steps of k_sor from step t0 (t0 = -M mod U),
+// every step unconditional. At step t the wave applies half-sweep s (s =
+// 1..K, red for odd s) to row t - s + 1, in increasing s. HEAD: the period
+// holds steps t < 2K - 1 (t0 < 0 for the first), the only ones with updates
+// outside the dependence cone (rows i < s), which are skipped (the test is
+// uniform across the wave, and the skipped updates are closed under
+// dependence, so no core value or residual sum changes). Rows before the
+// tile read as 0 (their offsets fall outside the buffer); rows past it read
+// its last row again; rows outside the valid core are not stored; when hb is
+// not a multiple of U, steps past the last useful one touch only rows
+// outside the core's cone.
 template <int K, int PF, bool HEAD>
 __device__ __forceinline__ void sor_period(int t0, int nrows, int hb, int rowb, int loff,
                                              int soff, float (&E)[K + PF + 1][4],
@@ -197,13 +209,15 @@ __device__ __forceinline__ void sor_period(int t0, int nrows, int hb, int rowb, 
                                              __amdgpu_buffer_rsrc_t rsg,
                                              __amdgpu_buffer_rsrc_t rso) {
   constexpr int U = K + PF + 1;
+  constexpr int M = ring_lead<K, PF>();
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int t = t0 + u;
     {
       const int sl = (u + PF) % U;  // row t+PF replaces row t+PF-U (< t-K, dead)
       // the row part of the offset is uniform (scalar); a row past the tile
-      // reads the last row again, whose values are never used
+      // reads the last row again, whose values are never used; a row before
+      // it has a negative offset, outside the buffer as an unsigned offset
       const int off = loff + min(t + PF, nrows - 1) * rowb;
       const v4f ev = buf_load4(rse, off);
       const v4f gv = buf_load4(rsg, off);
@@ -217,7 +231,7 @@ __device__ __forceinline__ void sor_period(int t0, int nrows, int hb, int rowb, 
       const int su = ((u - s) % U + U) % U;
       const int sd = ((u - s + 2) % U + U) % U;
       const int colour = (s - 1) & 1;
-      const int j0 = (colour + (u - s + 1 + 2 * U)) & 1;
+      const int j0 = (colour + (u - s + 1 + M + 2 * U)) & 1;  // parity of row t - s + 1
       const int i = t - s + 1;
       float *Er = E[si];
       const float *Eu = E[su];
@@ -321,18 +335,22 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
       __builtin_amdgcn_make_buffer_rsrc(I.g + tile0, 0, nbytes, PH_BUF_FLAGS);
   const __amdgpu_buffer_rsrc_t rso =
       __builtin_amdgcn_make_buffer_rsrc(I.e[cur ^ 1u] + tile0, 0, nbytes, PH_BUF_FLAGS);
+  // Steps t = 0 .. nrows - 2 carry every update the core needs. The head
+  // periods (from step -M) end at t = 2K - 2; whole periods follow, exactly
+  // hb / U of them when U divides hb. Rows the first period's steps do not
+  // load (r < PF - M) are loaded here.
+  constexpr int M = ring_lead<K, PF>();
 #pragma unroll
-  for (int r = 0; r < PF; ++r) {  // rows 0 .. PF-1 (nrows > PF always)
+  for (int r = 0; r < PF - M; ++r) {
     const v4f ev = buf_load4(rse, loff + r * rowb);
     const v4f gv = buf_load4(rsg, loff + r * rowb);
-    E[r][0] = ev.x; E[r][1] = ev.y; E[r][2] = ev.z; E[r][3] = ev.w;
-    G[r][0] = gv.x; G[r][1] = gv.y; G[r][2] = gv.z; G[r][3] = gv.w;
+    const int sl = (r + M) % U;
+    E[sl][0] = ev.x; E[sl][1] = ev.y; E[sl][2] = ev.z; E[sl][3] = ev.w;
+    G[sl][0] = gv.x; G[sl][1] = gv.y; G[sl][2] = gv.z; G[sl][3] = gv.w;
   }
-  // steps t = 0 .. nrows - 2 carry every update the core needs; whole periods
-  // run, the steps past them touching only rows outside the core's cone
   const int nsteps = nrows - 1;
-  int t0 = 0;
-  for (; t0 < nsteps && t0 < 2 * K - 1; t0 += U)
+  int t0 = -M;
+  for (; t0 < 2 * K - 1; t0 += U)
     sor_period<K, PF, true>(t0, nrows, hb, rowb, loff, soff, E, G, acc, cs, cw, dg, rse, rsg, rso);
   for (; t0 < nsteps; t0 += U)
     sor_period<K, PF, false>(t0, nrows, hb, rowb, loff, soff, E, G, acc, cs, cw, dg, rse, rsg, rso);

@@ -297,6 +297,38 @@ def test_results_do_not_depend_on_half_sweeps_per_launch(gpu):
     assert len(set(outs.values())) == 1
 
 
+@pytest.mark.parametrize("k", [16, 8])
+def test_results_do_not_depend_on_tile_shape(gpu, monkeypatch, k):
+    """Tiles own disjoint cores and read only the previous launch's buffer, so
+    the field is bit-identical for every tile height (a multiple of the row
+    ring, which ends on a short period, or not, which runs whole periods) and
+    every prefetch depth. The residual estimates sum per-tile partial sums, so
+    their rounding depends on the tiling: they agree to a relative 1e-6."""
+    from paintop_amd.device import DeviceArray
+    from paintop_amd.inputs import seamless_clone_case
+
+    n = 640
+    src, tgt, msk = seamless_clone_case(n, channels=3)
+    runs = {}
+    for pf in ("1", "3", "5"):
+        monkeypatch.setenv("PAINTOP_HIP_PREFETCH", pf)
+        for hb in (16, 20, 22, 24, 40, 44, 60, 64, 88):
+            o = DeviceArray(tgt.shape, np.float32)
+            d = hip.make_desc(n, n, 3, 40, 1e-12, omega=1.6, half_sweeps=k, tile_rows=hb)
+            r = hip.poisson_solve_device(d, src.ptr, tgt.ptr, msk.ptr, o.ptr)
+            # (a fixed omega may end on the stall rule; several launches run)
+            assert r.iterations >= 16
+            runs[(pf, hb)] = (o.numpy().tobytes(), np.array(r.residual_history))
+    f0, h0 = next(iter(runs.values()))
+    bad = [key for key, (f, _h) in runs.items() if f != f0]
+    hdev = max(float(np.max(np.abs(h / h0 - 1.0))) if h.shape == h0.shape else np.inf
+               for _f, h in runs.values())
+    print(f"[tiles] K {k}: {len(runs)} tile shapes, fields differ {bad}, history max rel "
+          f"deviation {hdev:.2e}", flush=True)
+    assert not bad
+    assert hdev <= 1e-6
+
+
 def test_sweeps_match_fp64_red_black_reference(gpu):
     """A few plain red-black sweeps on the GPU agree with a numpy FP64
     red-black SOR of the same sweeps (FP32 correction arithmetic: close, not equal)."""
