@@ -103,33 +103,46 @@ __global__ __launch_bounds__(256) void k_bbox_spans(const PhSpan *__restrict__ s
   }
 }
 
-// Interior-cell count of each tile's valid core, for every tile origin of
-// every image of a call in one launch (one block per origin; origin_item
-// gives its image). Images of a batch have few tiles each, so one launch per
-// image left the GPU idle between launches.
+// Interior cells of each tile origin, for every image of a call in one launch
+// (one block per origin): counts[2i] over the tile's valid core (hb rows x
+// wv columns from (K, K)), counts[2i + 1] over every cell the tile loads
+// (hb + 2K rows x 256 columns); a tile whose loaded cells are all interior
+// needs no masking of non-interior cells.
 __global__ void k_tile_occupancy_all(const PhItemGeo *__restrict__ items,
                                      const int *__restrict__ origin_item,
                                      const int2 *__restrict__ origins, int K, int wv, int hb,
                                      int *__restrict__ counts) {
   const PhItemGeo G = items[origin_item[blockIdx.x]];
   const int2 o = origins[blockIdx.x];
-  int n = 0;
-  for (int r = 0; r < hb; ++r) {
-    const int iy = G.oy + o.y + K + r;
+  int n = 0, nall = 0;
+  for (int r = 0; r < hb + 2 * K; ++r) {
+    const int iy = G.oy + o.y + r;
     if (iy < 0 || iy >= G.h) continue;
-    for (int cidx = threadIdx.x; cidx < wv; cidx += blockDim.x) {
-      const int ix = G.ox + o.x + K + cidx;
-      if (ix >= 0 && ix < G.w) n += G.cls[(size_t)iy * G.w + ix];
+    const bool core_row = r >= K && r < K + hb;
+    for (int cidx = threadIdx.x; cidx < wv + 2 * K; cidx += blockDim.x) {
+      const int ix = G.ox + o.x + cidx;
+      if (ix >= 0 && ix < G.w) {
+        const int v = G.cls[(size_t)iy * G.w + ix];
+        nall += v;
+        if (core_row && cidx >= K && cidx < K + wv) n += v;
+      }
     }
   }
-  __shared__ int red[256];
-  red[threadIdx.x] = n;
+  __shared__ int red[2][256];
+  red[0][threadIdx.x] = n;
+  red[1][threadIdx.x] = nall;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) {
+    counts[2 * blockIdx.x] = red[0][0];
+    counts[2 * blockIdx.x + 1] = red[1][0];
+  }
 }
 
 // Interior-cell count of each f x f block of an image's region (the coarse

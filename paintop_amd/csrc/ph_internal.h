@@ -104,7 +104,13 @@ __host__ __device__ inline bool ph_in_ramp(const PhInst &I, uint32_t m) {
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of work items (tiles, rows) so neighbours, which share
-// halo rows, hit the same L2. A bijection of [0, n) for b < n.
+// halo rows, hit the same L2. A bijection of [0, n) for b < n; XCD x gets the
+// items [xcd_range_begin(x, n), xcd_range_begin(x + 1, n)), in block order.
+__host__ __device__ inline int xcd_range_begin(int x, int n) {
+  const int per = (n + 7) >> 3;
+  const int full = n - (per - 1) * 8;
+  return x <= full ? x * per : full * per + (x - full) * (per - 1);
+}
 __device__ __forceinline__ int xcd_tile(int b, int n) {
   const int per = (n + 7) >> 3;
   const int xcd = b & 7, slot = b >> 3;
@@ -204,5 +210,8 @@ struct PhTile {
   int32_t inst;
   int32_t px0; // first loaded plane column (valid columns start at px0 + K)
   int32_t py0; // first loaded plane row (valid rows start at py0 + K)
-  int32_t pad;
+  // bit 0 PH_TILE_INTERIOR: every cell the tile loads is interior; bits 1..:
+  // the tile's slot in the residual partial sums (its index in instance order)
+  int32_t flags;
 };
+#define PH_TILE_INTERIOR 1

@@ -658,6 +658,7 @@ struct ItemWork {
   std::vector<int2> origins;
   size_t origin_off = 0;
   std::vector<int2> live;  // tile origins whose core holds interior cells
+  std::vector<char> live_interior;  // 1: every cell the live tile loads is interior
   double n_interior = 0;
   int cf = 1, cw = 0, ch = 0;
   size_t coarse_off = 0;
@@ -948,10 +949,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       coarse_total += (size_t)w.cw * w.ch;
     }
   }
-  std::vector<int> counts(all_origins.size()), coarse(coarse_total);
+  std::vector<int> counts(2 * all_origins.size()), coarse(coarse_total);
   if (!all_origins.empty()) {
     int2 *origins_d = g_pool.take<int2>(all_origins.size());
-    int *counts_d = g_pool.take<int>(all_origins.size());
+    int *counts_d = g_pool.take<int>(2 * all_origins.size());
     int *coarse_d = coarse_total ? g_pool.take<int>(coarse_total) : nullptr;
     // one launch each for every image: per-image geometry and the image of
     // every tile origin
@@ -1009,10 +1010,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   for (int i = 0; i < nitem; ++i) {
     ItemWork &w = iw[i];
     if (!w.any) continue;
+    const int full = 256 * (hb + 2 * K);
     for (size_t k = 0; k < w.origins.size(); ++k)
-      if (counts[w.origin_off + k] > 0) {
+      if (counts[2 * (w.origin_off + k)] > 0) {
         w.live.push_back(w.origins[k]);
-        w.n_interior += counts[w.origin_off + k];
+        w.live_interior.push_back(counts[2 * (w.origin_off + k) + 1] == full ? 1 : 0);
+        w.n_interior += counts[2 * (w.origin_off + k)];
       }
   }
   // SOR factor per image: the coarse-mask eigenvalue for large domains (cached
@@ -1420,12 +1423,28 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       tl.clear();
       for (int a = 0; a < nact; ++a) {
         ai[a].tile_begin = (int)tl.size();
-        if (running[a])
-          for (const int2 &o : iw[inst[act[a]].item].live) tl.push_back(PhTile{a, o.x, o.y, 0});
+        if (running[a]) {
+          const ItemWork &w = iw[inst[act[a]].item];
+          for (size_t q = 0; q < w.live.size(); ++q)
+            tl.push_back(PhTile{a, w.live[q].x, w.live[q].y,
+                                (int)(tl.size() << 1) | (w.live_interior[q] ? PH_TILE_INTERIOR : 0)});
+        }
         ai[a].tile_end = (int)tl.size();
       }
       ntiles = (int)tl.size();
-      if (trace_on()) fprintf(stderr, "[tiles] %d tiles of %d rows\n", ntiles, hb);
+      // Dispatch order: within each XCD's range, the masked tiles first. A
+      // launch of about one tile per wave slot then puts at most one masked
+      // tile (the slower kind) on a SIMD where the ranges allow, instead of
+      // two on some; the partial sums keep instance order (the slot in flags).
+      for (int x = 0; x < 8; ++x)
+        std::stable_partition(tl.begin() + xcd_range_begin(x, ntiles),
+                              tl.begin() + xcd_range_begin(x + 1, ntiles),
+                              [](const PhTile &t) { return !(t.flags & PH_TILE_INTERIOR); });
+      if (trace_on()) {
+        int nin = 0;
+        for (const PhTile &t : tl) nin += t.flags & PH_TILE_INTERIOR;
+        fprintf(stderr, "[tiles] %d tiles of %d rows, %d all interior\n", ntiles, hb, nin);
+      }
       if (ntiles)
         HIP_OK(hipMemcpyAsync(tiles_d, tl.data(), ntiles * sizeof(PhTile), hipMemcpyHostToDevice,
                               st));

@@ -145,9 +145,14 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *A, const double
 }
 
 // One SOR update of cell j of a lane's row, given its four neighbours.
+template <bool MASK>
 __device__ __forceinline__ float relax(float &e, float g, float l, float r, float u, float d,
                                        float c, float dg) {
   const float rp = fmaf(-dg, e, (((l + r) + u) + d) - g);
+  if constexpr (!MASK) {  // a tile whose cells are all interior
+    e = fmaf(c, rp, e);
+    return rp;
+  }
   // g is NaN at non-interior cells, so rp is NaN exactly there (e and its
   // neighbours are finite). Testing rp, which is new at every stage, keeps the
   // compare next to its use: a test of g would be hoisted out of the row's
@@ -200,7 +205,7 @@ constexpr int ring_lead() {
 // its last row again; rows outside the valid core are not stored; when hb is
 // not a multiple of U, steps past the last useful one touch only rows
 // outside the core's cone.
-template <int K, int PF, bool HEAD>
+template <int K, int PF, bool HEAD, bool MASK>
 __device__ __forceinline__ void sor_period(int t0, int nrows, int hb, int rowb, int loff,
                                              int soff, float (&E)[K + PF + 1][4],
                                              float (&G)[K + PF + 1][4], float (&acc)[K],
@@ -241,12 +246,12 @@ __device__ __forceinline__ void sor_period(int t0, int nrows, int hb, int rowb, 
       float ra, rb;
       if (j0 == 0) {
         const float l0 = from_left(Er[3]);
-        ra = relax(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
-        rb = relax(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
+        ra = relax<MASK>(Er[0], Gr[0], l0, Er[1], Eu[0], Ed[0], c, dg);
+        rb = relax<MASK>(Er[2], Gr[2], Er[1], Er[3], Eu[2], Ed[2], c, dg);
       } else {
         const float r3n = from_right(Er[0]);
-        ra = relax(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
-        rb = relax(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
+        ra = relax<MASK>(Er[1], Gr[1], Er[0], Er[2], Eu[1], Ed[1], c, dg);
+        rb = relax<MASK>(Er[3], Gr[3], Er[2], r3n, Eu[3], Ed[3], c, dg);
       }
       if (i >= K && i < K + hb) {  // uniform across the wave (becomes a select)
         if constexpr (K % 4 == 0) {
@@ -278,21 +283,15 @@ __device__ __forceinline__ void sor_period(int t0, int nrows, int hb, int rowb, 
 
 // PF: rows loaded ahead of use (odd, so the ring size stays even and the
 // red/black colour of a ring slot is fixed at compile time).
-template <int K, int PF>
-__global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, int ntiles,
-                                            const PhInst *__restrict__ insts,
-                                            const PhCtl *__restrict__ ctl,
-                                            double *__restrict__ partials, int hb) {
+// One tile's pass of K half-sweeps (MASK: the tile holds non-interior cells,
+// whose updates are masked out; a tile of interior cells only needs no mask,
+// and its result is the same).
+template <int K, int PF, bool MASK>
+__device__ __forceinline__ void sor_tile(const PhTile &T, const PhInst &I, const PhCtl &C,
+                                         int lane, int slot, int ntiles,
+                                         double *__restrict__ partials, int hb) {
   constexpr int CPL = 4;
-  constexpr int U = K + PF + 1;  // live rows t-K .. t+1, plus rows in flight up to t+PF
-  static_assert((U & 1) == 0, "ring size must be even");
-  const int lane = threadIdx.x;
-  const int b = xcd_tile(blockIdx.x, ntiles);
-  const PhTile T = tiles[b];
-  const PhInst &I = insts[T.inst];
-  const PhCtl &C = ctl[T.inst];
-  if (C.halt != PH_RUN) return;
-
+  constexpr int U = K + PF + 1;
   const int k_run = min(K, (int)(2 * (C.sweep_limit - C.sweeps_done)));
   const uint32_t cur = C.cur_buf;
   const size_t pitch = (size_t)I.pitch;
@@ -351,15 +350,33 @@ __global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, in
   const int nsteps = nrows - 1;
   int t0 = -M;
   for (; t0 < 2 * K - 1; t0 += U)
-    sor_period<K, PF, true>(t0, nrows, hb, rowb, loff, soff, E, G, acc, cs, cw, dg, rse, rsg, rso);
+    sor_period<K, PF, true, MASK>(t0, nrows, hb, rowb, loff, soff, E, G, acc, cs, cw, dg, rse, rsg, rso);
   for (; t0 < nsteps; t0 += U)
-    sor_period<K, PF, false>(t0, nrows, hb, rowb, loff, soff, E, G, acc, cs, cw, dg, rse, rsg, rso);
+    sor_period<K, PF, false, MASK>(t0, nrows, hb, rowb, loff, soff, E, G, acc, cs, cw, dg, rse, rsg, rso);
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const float a = (K % 4 == 0) ? (lane_full ? acc[s] : 0.0f) : acc[s];
     const double tot = wave_sum((double)a);
-    if (lane == 0) partials[(size_t)s * ntiles + b] = tot;  // stage-major: coalesced sums
+    if (lane == 0) partials[(size_t)s * ntiles + slot] = tot;  // stage-major: coalesced sums
   }
+}
+
+template <int K, int PF>
+__global__ __launch_bounds__(64) void k_sor(const PhTile *__restrict__ tiles, int ntiles,
+                                            const PhInst *__restrict__ insts,
+                                            const PhCtl *__restrict__ ctl,
+                                            double *__restrict__ partials, int hb) {
+  static_assert(((K + PF + 1) & 1) == 0, "ring size must be even");
+  const int lane = threadIdx.x;
+  const int b = xcd_tile(blockIdx.x, ntiles);
+  const PhTile T = tiles[b];
+  const PhInst &I = insts[T.inst];
+  const PhCtl &C = ctl[T.inst];
+  if (C.halt != PH_RUN) return;
+  if (T.flags & PH_TILE_INTERIOR)
+    sor_tile<K, PF, false>(T, I, C, lane, T.flags >> 1, ntiles, partials, hb);
+  else
+    sor_tile<K, PF, true>(T, I, C, lane, T.flags >> 1, ntiles, partials, hb);
 }
 
 // After the sweep launch (the kernel boundary orders it after every tile):
