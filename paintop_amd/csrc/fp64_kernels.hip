@@ -384,6 +384,74 @@ __global__ __launch_bounds__(256) void k_probe_residual_spans(const PhInst *__re
   const double s = block_sum(probe_residual_row(I, I.e[S.buf], b - S.row0));
   if (threadIdx.x == 0) row_sums[b] = s;
 }
+// A learned coarse correction into the correction buffer and the FP64
+// residual of x + e, in one pass: one block per plane row of every listed
+// correction (span entry i takes args[i] and its correction buffer S.buf).
+// At interior cells e = sign * alpha * scale * E, where E interpolates the
+// prediction h bilinearly in FP32 (image cell ix sits at coarse coordinate
+// (ix - x0 + 0.5) / f - 0.5, clamped to the grid): vertically once per row
+// into LDS, then along the row. row_sums[b] = the row's sum of r(x + e)^2 in
+// FP64. A block computes E at its row's neighbour cells with the same
+// arithmetic as the blocks that own them, so it reads no other block's
+// writes. sign 0 writes e = 0 (a correction taken back).
+__global__ __launch_bounds__(256) void k_correct_probe_spans(const PhInst *__restrict__ insts,
+                                                             const PhSpan *__restrict__ spans,
+                                                             int n,
+                                                             const PhCorrArgs *__restrict__ args,
+                                                             const float *__restrict__ hc,
+                                                             const float *__restrict__ scale,
+                                                             const double *__restrict__ alpha,
+                                                             int res,
+                                                             double *__restrict__ row_sums) {
+  const int b = xcd_row_group(blockIdx.x, gridDim.x);
+  const int i = ph_span_find(spans, n, b);
+  const PhSpan S = spans[i];
+  const PhCorrArgs A = args[i];
+  const PhInst &I = insts[S.slot];
+  const float ca = (float)(A.sign * alpha[A.net] * (double)scale[A.net]);
+  const float *h = hc + (size_t)res * res * A.net;
+  float *e = I.e[S.buf];
+  const int py = b - S.row0;
+  const float invf = 1.0f / (float)A.f;
+  __shared__ float vrow[3][PH_CORR_RES_MAX];  // rows py - 1, py, py + 1
+  for (int t = threadIdx.x; t < 3 * res; t += 256) {
+    const int r = t / res, c = t - r * res;
+    const int iy = I.oy + py - 1 + r;
+    const float v = fminf(fmaxf(((float)(iy - A.y0) + 0.5f) * invf - 0.5f, 0.0f), (float)(res - 1));
+    const int j0 = min((int)v, res - 1), j1 = min(j0 + 1, res - 1);
+    const float h0 = h[(size_t)j0 * res + c], h1 = h[(size_t)j1 * res + c];
+    vrow[r][c] = fmaf(v - (float)j0, h1 - h0, h0);
+  }
+  __syncthreads();
+  // E at plane cell (px, row r of the three), 0 off the interior
+  auto corr = [&](int r, int px, size_t q) -> double {
+    if (!I.cls[q]) return 0.0;
+    const float u =
+        fminf(fmaxf(((float)(I.ox + px - A.x0) + 0.5f) * invf - 0.5f, 0.0f), (float)(res - 1));
+    const int i0 = min((int)u, res - 1), i1 = min(i0 + 1, res - 1);
+    return (double)(ca * fmaf(u - (float)i0, vrow[r][i1] - vrow[r][i0], vrow[r][i0]));
+  };
+  double s = 0.0;
+  if (py >= 1 && py < I.ph - 1) {
+    const size_t row = (size_t)py * I.pitch, pitch = (size_t)I.pitch;
+    for (int px = threadIdx.x; px < I.pw; px += 256) {
+      const size_t p = row + px;
+      if (!I.cls[p]) continue;
+      const double ec = corr(1, px, p);
+      e[p] = (float)ec;
+      const double xc = I.x[p] + ec;
+      const double xl = I.x[p - 1] + corr(1, px - 1, p - 1);
+      const double xr = I.x[p + 1] + corr(1, px + 1, p + 1);
+      const double xu = I.x[p - pitch] + corr(0, px, p - pitch);
+      const double xd = I.x[p + pitch] + corr(2, px, p + pitch);
+      const double r = fma(I.dg64, -xc, ((xl + xr) + xu) + xd) - I.bp[p];
+      s = fma(r, r, s);
+    }
+  }
+  const double tot = block_sum(s);
+  if (threadIdx.x == 0) row_sums[b] = tot;
+}
+
 __global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__restrict__ insts,
                                                                 const PhSpan *__restrict__ spans,
                                                                 int n) {

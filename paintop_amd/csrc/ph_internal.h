@@ -189,6 +189,9 @@ struct PhCorrArgs {
   double sign;        // +1 applies the correction, -1 takes it back
 };
 
+// Largest coarse-grid side a correction kernel stages in LDS.
+constexpr int PH_CORR_RES_MAX = 256;
+
 // A value for one device address (batched small updates).
 struct PhValue {
   double *dst;

@@ -68,8 +68,9 @@ __global__ void k_warm_apply_spans(const PhInst *, const PhSpan *, int, const Ph
                                    const PhItemIO *, const float *, const float *, int);
 __global__ void k_coarsen_plane(PhInst, int, int, int, int, int, float *);
 __global__ void k_coarsen_plane_list(const PhInst *, const PhCorrArgs *, int, float *);
-__global__ void k_correct_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
-                                      const float *, const float *, const double *, int);
+__global__ void k_correct_probe_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
+                                      const float *, const float *, const double *, int,
+                                      double *);
 __global__ void k_put_values(const PhValue *, int);
 __global__ void k_copy_anchor_spans(const PhInst *, const PhInst *, const PhSpan *, int);
 int coarsen_plane_cpb(int f);
@@ -750,7 +751,7 @@ static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int 
   const size_t plane_up = pw_up * ph_up;
   const size_t ntile_up = (pw_up / WV + 2) * (ph_up / hb + 2);
   return npx + 8 * 1024 + 2 * (size_t)H * sizeof(int) +
-         ntile_up * (sizeof(int2) + 2 * sizeof(int)) + sizeof(PhItemGeo) +
+         ntile_up * (sizeof(int2) + 3 * sizeof(int)) + sizeof(PhItemGeo) +
          (size_t)C * 5 * ph_up * sizeof(double) + (size_t)(12 * C + 4) * sizeof(PhSpan) +
          (size_t)C * (3 * sizeof(PhInst) + 4 * sizeof(PhCorrArgs) + sizeof(PhValue)) +
          sizeof(PhItemIO) + sizeof(PhHistCopy) + (size_t)(d->max_iterations + 2) * sizeof(double) +
@@ -1218,6 +1219,9 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   struct Geo { int x0, y0, f; };
   std::vector<Geo> wgeo(ninst, Geo{0, 0, 1});
   const int NR = warm_net_res(), NNB = warm_net_batch();
+  if (NR > PH_CORR_RES_MAX)
+    throw fail(PH_CLASS_EXECUTION, "E_HIP_CONTROL", "network grid %d exceeds %d", NR,
+               PH_CORR_RES_MAX);
   const size_t NR2 = (size_t)NR * NR;
   float *net_in = nullptr, *net_h = nullptr, *net_scale = nullptr;
   double *corr_alpha = nullptr, *net_part = nullptr;
@@ -1407,13 +1411,14 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     // correction rounds: arguments and span tables of the corrections and
     // of the take-backs, the history entries of the kept ones
     PhCorrArgs *corr_args_d = nullptr, *undo_args_d = nullptr;
-    PhSpan *corr_spans_d = nullptr, *undo_spans_d = nullptr;
+    PhSpan *corr_spans_d = nullptr, *undo_spans_d = nullptr, *fin_spans_d = nullptr;
     PhValue *corr_vals_d = nullptr;
     if (!net_list.empty()) {
       corr_args_d = g_pool.take<PhCorrArgs>(nact);
       undo_args_d = g_pool.take<PhCorrArgs>(nact);
       corr_spans_d = g_pool.take<PhSpan>(nact);
       undo_spans_d = g_pool.take<PhSpan>(nact);
+      fin_spans_d = g_pool.take<PhSpan>(nact);
       corr_vals_d = g_pool.take<PhValue>(nact);
     }
     SpanTable ref;
@@ -1562,7 +1567,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     std::vector<PhCorrArgs> corr_args, undo_args;
     std::vector<PhValue> corr_vals;
     std::vector<PhInst> ai_stage;
-    SpanTable corr_rows, undo_rows;
+    SpanTable corr_rows, undo_rows, fin_rows;
     auto correct = [&]() {
       // cycles that stopped paying for themselves end here, without a correction
       std::vector<Corr> go;
@@ -1596,7 +1601,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         const Geo &g = wgeo[k];
         corr_args[j] =
             PhCorrArgs{a, g.x0, g.y0, g.f, coarsen_plane_cpb(g.f), j, inst[k].item, 0, 1.0};
-        corr_rows.add(a, ai[a].ph, 0, 0, k);
+        corr_rows.add(a, ai[a].ph, 0, (int)ctl[a].cur_buf, k);
       }
       HIP_OK(hipMemcpyAsync(corr_args_d, corr_args.data(), nc * sizeof(PhCorrArgs),
                             hipMemcpyHostToDevice, st));
@@ -1620,11 +1625,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                            CORR_MAX_STEP, corr_alpha + c0);
         HIP_OK(hipGetLastError());
       }
-      // every correction and its FP64 residual in three launches
-      hipLaunchKernelGGL(k_correct_apply_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
-                         corr_spans_d, nc, corr_args_d, net_h, net_scale, corr_alpha, NR);
-      hipLaunchKernelGGL(k_residual_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
-                         corr_spans_d, nc, ref_sums_d);
+      // every correction into its correction buffer, and the FP64 residual
+      // of x + e (x and g stay as the refinement left them)
+      hipLaunchKernelGGL(k_correct_probe_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
+                         corr_spans_d, nc, corr_args_d, net_h, net_scale, corr_alpha, NR,
+                         ref_sums_d);
       hipLaunchKernelGGL(k_residual_norm_spans, dim3(nc), dim3(256), 0, st, corr_spans_d,
                          ref_sums_d, resv_d);
       HIP_OK(hipGetLastError());
@@ -1641,32 +1646,30 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       undo_rows.clear();
       for (int j = 0; j < nc; ++j) {
         const int a = go[j].a, k = act[a];
+        if (alpha[j] == 0.0) resv[2 * k] = go[j].gnorm;  // e stays 0
         jump[j] = resv[2 * k] / go[j].gnorm;
         if (alpha[j] != 0.0 && !(std::isfinite(jump[j]) && jump[j] <= max_jump)) {
           PhCorrArgs A = corr_args[j];
-          A.sign = -1.0;
+          A.sign = 0.0;
           undo_args.push_back(A);
-          undo_rows.add(a, ai[a].ph, 0, 0, k);
+          undo_rows.add(a, ai[a].ph, 0, (int)ctl[a].cur_buf, k);
           undo[j] = 1;
+          resv[2 * k] = go[j].gnorm;  // e = 0 again: the refinement's residual
         }
       }
       if (!undo_args.empty()) {
+        // taken back: the correction buffers of these instances are zeroed
         const int nu = (int)undo_args.size();
         HIP_OK(hipMemcpyAsync(undo_args_d, undo_args.data(), nu * sizeof(PhCorrArgs),
                               hipMemcpyHostToDevice, st));
         undo_rows.upload(undo_spans_d, st);
-        hipLaunchKernelGGL(k_correct_apply_spans, dim3(undo_rows.rows), dim3(256), 0, st, inst_d,
-                           undo_spans_d, nu, undo_args_d, net_h, net_scale, corr_alpha, NR);
-        hipLaunchKernelGGL(k_residual_spans, dim3(undo_rows.rows), dim3(256), 0, st, inst_d,
-                           undo_spans_d, nu, ref_sums_d);
-        hipLaunchKernelGGL(k_residual_norm_spans, dim3(nu), dim3(256), 0, st, undo_spans_d,
-                           ref_sums_d, resv_d);
+        hipLaunchKernelGGL(k_correct_probe_spans, dim3(undo_rows.rows), dim3(256), 0, st, inst_d,
+                           undo_spans_d, nu, undo_args_d, net_h, net_scale, corr_alpha, NR,
+                           ref_sums_d);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * inst.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
       }
       corr_vals.clear();
+      fin_rows.clear();
       bool ramp_moved = false;
       for (int j = 0; j < nc; ++j) {
         const int a = go[j].a, k = act[a];
@@ -1701,11 +1704,20 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           trend[a] = Trend{m, rel};
         }
         if (rel <= Cc.tol) {
+          // converged with a kept correction in e: x takes it (x += e is the
+          // sum the residual was evaluated at)
+          if (alpha[j] != 0.0 && !undo[j]) fin_rows.add(a, ai[a].ph, 0, (int)Cc.cur_buf, k);
           finish(a, PH_STOP_CONVERGED, m, rel);
           --remaining;
         } else {
           resume(a, gnorm);
         }
+      }
+      if (!fin_rows.h.empty()) {
+        fin_rows.upload(fin_spans_d, st);
+        hipLaunchKernelGGL(k_apply_correction_spans, dim3(fin_rows.rows), dim3(256), 0, st, inst_d,
+                           fin_spans_d, (int)fin_rows.h.size());
+        HIP_OK(hipGetLastError());
       }
       // the restarted ramps and the kept corrections' history entries
       if (ramp_moved) {

@@ -448,33 +448,6 @@ __global__ void k_corr_alpha(const double *__restrict__ part, int n, double max_
   alpha[b] = (s2 < 0.0 && isfinite(a)) ? fmin(fmax(a, 0.0), max_step) : 0.0;
 }
 
-// x += sign * alpha * scale * bilinear(h) at the interior cells of every
-// correction of a round, one block per plane row (span entry i takes args[i];
-// sign -1 takes a correction back).
-__global__ __launch_bounds__(256) void k_correct_apply_spans(const PhInst *__restrict__ insts,
-                                                             const PhSpan *__restrict__ spans,
-                                                             int n,
-                                                             const PhCorrArgs *__restrict__ args,
-                                                             const float *__restrict__ hc,
-                                                             const float *__restrict__ scale,
-                                                             const double *__restrict__ alpha,
-                                                             int res) {
-  const int b = blockIdx.x;
-  const int i = ph_span_find(spans, n, b);
-  const PhSpan S = spans[i];
-  const PhCorrArgs A = args[i];
-  const PhInst &I = insts[S.slot];
-  const double a = A.sign * alpha[A.net];
-  if (a == 0.0) return;
-  const double sc = (double)scale[A.net];
-  const float *h = hc + (size_t)res * res * A.net;
-  const int py = b - S.row0;
-  for (int px = threadIdx.x; px < I.pw; px += 256) {
-    const size_t p = (size_t)py * I.pitch + px;
-    if (I.cls[p]) I.x[p] += a * sc * warm_bilinear(h, res, A.x0, A.y0, A.f, I.ox + px, I.oy + py);
-  }
-}
-
 // x = P + scale * bilinear(h) at the interior cells of every listed network
 // instance, one block per plane row (span entry i takes args[i]; cell centres:
 // image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5, clamped
