@@ -423,29 +423,58 @@ __global__ __launch_bounds__(256) void k_correct_probe_spans(const PhInst *__res
     vrow[r][c] = fmaf(v - (float)j0, h1 - h0, h0);
   }
   __syncthreads();
-  // E at plane cell (px, row r of the three), 0 off the interior
-  auto corr = [&](int r, int px, size_t q) -> double {
-    if (!I.cls[q]) return 0.0;
+  // The instance's fields in registers (a store to e could alias them).
+  const double *__restrict__ x = I.x;
+  const double *__restrict__ bp = I.bp;
+  const uint8_t *__restrict__ cls = I.cls;
+  const double dg = I.dg64;
+  const int pw = I.pw, ox = I.ox;
+  const size_t pitch = (size_t)I.pitch;
+  // the horizontal interpolation position of plane column px
+  auto col = [&](int px, int &i0, int &i1, float &w) {
     const float u =
-        fminf(fmaxf(((float)(I.ox + px - A.x0) + 0.5f) * invf - 0.5f, 0.0f), (float)(res - 1));
-    const int i0 = min((int)u, res - 1), i1 = min(i0 + 1, res - 1);
-    return (double)(ca * fmaf(u - (float)i0, vrow[r][i1] - vrow[r][i0], vrow[r][i0]));
+        fminf(fmaxf(((float)(ox + px - A.x0) + 0.5f) * invf - 0.5f, 0.0f), (float)(res - 1));
+    i0 = min((int)u, res - 1);
+    i1 = min(i0 + 1, res - 1);
+    w = u - (float)i0;
+  };
+  // E at a cell of row r (0: py - 1, 1: py, 2: py + 1), 0 off the interior
+  auto corr = [&](int r, bool in, int i0, int i1, float w) -> double {
+    return in ? (double)(ca * fmaf(w, vrow[r][i1] - vrow[r][i0], vrow[r][i0])) : 0.0;
   };
   double s = 0.0;
   if (py >= 1 && py < I.ph - 1) {
-    const size_t row = (size_t)py * I.pitch, pitch = (size_t)I.pitch;
-    for (int px = threadIdx.x; px < I.pw; px += 256) {
+    const size_t row = (size_t)py * pitch;
+    // two cells per thread (pw is even; interior cells lie in [1, pw - 1))
+    for (int px = 2 * threadIdx.x; px < pw; px += 512) {
       const size_t p = row + px;
-      if (!I.cls[p]) continue;
-      const double ec = corr(1, px, p);
-      e[p] = (float)ec;
-      const double xc = I.x[p] + ec;
-      const double xl = I.x[p - 1] + corr(1, px - 1, p - 1);
-      const double xr = I.x[p + 1] + corr(1, px + 1, p + 1);
-      const double xu = I.x[p - pitch] + corr(0, px, p - pitch);
-      const double xd = I.x[p + pitch] + corr(2, px, p + pitch);
-      const double r = fma(I.dg64, -xc, ((xl + xr) + xu) + xd) - I.bp[p];
-      s = fma(r, r, s);
+      const uchar2 c = *reinterpret_cast<const uchar2 *>(cls + p);
+      if (!(c.x | c.y)) continue;
+      const uchar2 cu = *reinterpret_cast<const uchar2 *>(cls + p - pitch);
+      const uchar2 cd = *reinterpret_cast<const uchar2 *>(cls + p + pitch);
+      const bool cl = cls[p - 1] != 0, cr = cls[p + 2] != 0;
+      int a0, a1, b0, b1, l0, l1, r0_, r1_;
+      float wa, wb, wl, wr;
+      col(px, a0, a1, wa);
+      col(px + 1, b0, b1, wb);
+      col(px - 1, l0, l1, wl);
+      col(px + 2, r0_, r1_, wr);
+      const double e0 = corr(1, c.x, a0, a1, wa), e1 = corr(1, c.y, b0, b1, wb);
+      if (c.x) e[p] = (float)e0;
+      if (c.y) e[p + 1] = (float)e1;
+      const double2 xc = *reinterpret_cast<const double2 *>(x + p);
+      const double2 xu = *reinterpret_cast<const double2 *>(x + p - pitch);
+      const double2 xd = *reinterpret_cast<const double2 *>(x + p + pitch);
+      const double2 bb = *reinterpret_cast<const double2 *>(bp + p);
+      const double xc0 = xc.x + e0, xc1 = xc.y + e1;
+      const double xl = x[p - 1] + corr(1, cl, l0, l1, wl);
+      const double xr = x[p + 2] + corr(1, cr, r0_, r1_, wr);
+      const double xu0 = xu.x + corr(0, cu.x, a0, a1, wa), xu1 = xu.y + corr(0, cu.y, b0, b1, wb);
+      const double xd0 = xd.x + corr(2, cd.x, a0, a1, wa), xd1 = xd.y + corr(2, cd.y, b0, b1, wb);
+      const double q0 = fma(dg, -xc0, ((xl + xc1) + xu0) + xd0) - bb.x;
+      const double q1 = fma(dg, -xc1, ((xc0 + xr) + xu1) + xd1) - bb.y;
+      if (c.x) s = fma(q0, q0, s);
+      if (c.y) s = fma(q1, q1, s);
     }
   }
   const double tot = block_sum(s);
