@@ -296,24 +296,32 @@ template __global__ void k_assemble_spans<4>(const PhInst *, const PhSpan *, int
 __device__ __forceinline__ double residual_row(const PhInst &I, int py) {
   double s = 0.0;
   if (py >= 1 && py < I.ph - 1) {
-    const size_t row = (size_t)py * I.pitch;
-    for (int px = 2 * threadIdx.x; px < I.pw; px += 2 * blockDim.x) {
+    // the instance's fields in registers: the store to g could alias them
+    const double *__restrict__ x = I.x;
+    const double *__restrict__ bp = I.bp;
+    const uint8_t *__restrict__ cls = I.cls;
+    float *__restrict__ g = I.g;
+    const double dg = I.dg64;
+    const int pw = I.pw;
+    const size_t pitch = (size_t)I.pitch;
+    const size_t row = (size_t)py * pitch;
+    for (int px = 2 * threadIdx.x; px < pw; px += 2 * blockDim.x) {
       const size_t p = row + px;
-      const uchar2 c = *reinterpret_cast<const uchar2 *>(I.cls + p);
+      const uchar2 c = *reinterpret_cast<const uchar2 *>(cls + p);
       if (!(c.x | c.y)) continue;
-      const double2 xc = *reinterpret_cast<const double2 *>(I.x + p);
-      const double2 xu = *reinterpret_cast<const double2 *>(I.x + p - I.pitch);
-      const double2 xd = *reinterpret_cast<const double2 *>(I.x + p + I.pitch);
-      const double2 b = *reinterpret_cast<const double2 *>(I.bp + p);
-      const double xl = I.x[p - 1], xr = I.x[p + 2];
+      const double2 xc = *reinterpret_cast<const double2 *>(x + p);
+      const double2 xu = *reinterpret_cast<const double2 *>(x + p - pitch);
+      const double2 xd = *reinterpret_cast<const double2 *>(x + p + pitch);
+      const double2 b = *reinterpret_cast<const double2 *>(bp + p);
+      const double xl = x[p - 1], xr = x[p + 2];
       const double n0 = ((xl + xc.y) + xu.x) + xd.x;
       const double n1 = ((xc.x + xr) + xu.y) + xd.y;
-      const double r0 = fma(I.dg64, -xc.x, n0) - b.x;
-      const double r1 = fma(I.dg64, -xc.y, n1) - b.y;
+      const double r0 = fma(dg, -xc.x, n0) - b.x;
+      const double r1 = fma(dg, -xc.y, n1) - b.y;
       float2 gv;
       gv.x = c.x ? (float)(-r0) : __builtin_nanf("");
       gv.y = c.y ? (float)(-r1) : __builtin_nanf("");
-      *reinterpret_cast<float2 *>(I.g + p) = gv;
+      *reinterpret_cast<float2 *>(g + p) = gv;
       if (c.x) s = fma(r0, r0, s);
       if (c.y) s = fma(r1, r1, s);
     }
@@ -487,14 +495,29 @@ __global__ __launch_bounds__(256) void k_apply_correction_spans(const PhInst *__
   const int b = blockIdx.x;
   const PhSpan S = spans[ph_span_find(spans, n, b)];
   const PhInst &I = insts[S.slot];
+  // the instance's fields in registers (the stores could alias them); cell
+  // pairs (pw and the pitch are even) with vector loads and stores
+  double *__restrict__ x = I.x;
+  const uint8_t *__restrict__ cls = I.cls;
+  float *__restrict__ e = I.e[S.buf];
+  const int pw = I.pw;
   const size_t row = (size_t)(b - S.row0) * I.pitch;
-  float *e = I.e[S.buf];
-  for (int px = threadIdx.x; px < I.pw; px += 256) {
+  for (int px = 2 * threadIdx.x; px < pw; px += 512) {
     const size_t p = row + px;
-    if (I.cls[p]) {
-      I.x[p] += (double)e[p];
-      e[p] = 0.0f;
+    const uchar2 c = *reinterpret_cast<const uchar2 *>(cls + p);
+    if (!(c.x | c.y)) continue;
+    double2 xv = *reinterpret_cast<const double2 *>(x + p);
+    float2 ev = *reinterpret_cast<const float2 *>(e + p);
+    if (c.x) {
+      xv.x += (double)ev.x;
+      ev.x = 0.0f;
     }
+    if (c.y) {
+      xv.y += (double)ev.y;
+      ev.y = 0.0f;
+    }
+    *reinterpret_cast<double2 *>(x + p) = xv;
+    *reinterpret_cast<float2 *>(e + p) = ev;
   }
 }
 
