@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--plain-only", action="store_true",
                    help="time only the plain start (no learned warm start)")
+    p.add_argument("--learned-only", action="store_true",
+                   help="time only the learned form (a profiling aid: prints its time and "
+                        "sweep counts, with no plain comparison or roofline)")
     p.add_argument("--learned", action="store_true",
                    help="batch workload: run the learned warm start and corrections")
     p.add_argument("--workload", choices=["seamless", "screened", "batch", "outside-scan", "copy"],
@@ -268,7 +271,12 @@ def bench_batch(args, info):
     its = np.array([r.iterations for r in res])
     usw = sum(r.unknown_sweeps for r in res)
     sweep_ms = res[0].sweep_ms
-    achieved = 12.0 * usw / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
+    # compulsory traffic: 12 B per unknown per launch, each launch running K/2
+    # sweeps per HBM round trip (as for the single-image workloads)
+    K = max(res[0].half_sweeps, 2)
+    achieved = BYTES_PER_UNKNOWN_SWEEP * usw / (K / 2) / (sweep_ms * 1e-3) / 1e9 \
+        if sweep_ms > 0 else 0.0
+    effective = BYTES_PER_UNKNOWN_SWEEP * usw / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
     line = {
         "metric": METRIC, "value": round(ms / (px / 1e6), 4), "unit": "ms/MP", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -284,9 +292,12 @@ def bench_batch(args, info):
                    "warm_start_kept": sum(r.warm_start_used for r in res),
                    "coarse_corrections": sum(r.coarse_corrections for r in res),
                    "device": info["name"]},
-        "roofline": {"kernel": f"k_sor<{res[0].half_sweeps}>", "bound": "hbm",
+        "roofline": {"kernel": f"k_sor<{K},3> + k_sor_control<{K}>", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "effective_GBps_12B_per_unknown_sweep": round(effective, 1),
+                     "note": "achieved = compulsory bytes (12 B per unknown per launch) / "
+                             "hipEvent time of the sweep launches"},
     }
     print(json.dumps(line), flush=True)
 
@@ -349,6 +360,20 @@ def main():
         return float(np.median(times)), results, out.numpy()
 
     mp = size * size / 1e6
+    if args.learned_only:
+        from paintop_amd import warmstart
+        warmstart.enable()
+        ms, results, _ = timed(hip.PH_FLAG_WARM_START_NET)
+        rl = results[len(results) // 2]
+        print(json.dumps({"metric": METRIC + " (learned form only)", "value": round(ms / mp, 4),
+                          "unit": "ms/MP", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": round(ms, 3),
+                          "config": {"workload": wl, "sweeps": rl.iterations,
+                                     "launches": rl.launches,
+                                     "coarse_corrections": rl.coarse_corrections,
+                                     "warm_start_channels_kept": rl.warm_start_used}}),
+              flush=True)
+        return
     learned = warm_only = None
     cold_ms = None
     if args.workload == "seamless" and not args.plain_only:
