@@ -365,6 +365,22 @@ ph_status ph_coarsen_plane_device(const float *d_g, const uint8_t *d_cls, int32_
                                   int32_t y0, int32_t factor, int32_t res, float *d_in,
                                   void *stream, ph_error *err);
 
+/* The same input from the state x + e of a plane between refinements (no
+ * counterpart in the reference; the solver's correction rounds use it): g is
+ * the FP32 copy of -r(x) taken at the last refinement and d_e the FP32
+ * correction since (0 off the interior), so the state's residual is
+ * r = nsum(e) - (4 + lambda) e - g, formed per cell in FP32 as the sweeps form
+ * it, with nsum the sum of the four neighbours' e. in[..][0] = the sum of -r
+ * over interior cells (= the sum of g for e = 0, bit for bit); channels 1..3
+ * as above; d_rsq ([res][res] f64) receives each block's sum of r^2 over its
+ * interior cells. Sums run in FP32 down a column's rows of a block, in FP64
+ * across its columns. d_e has g's layout and alignment. */
+ph_status ph_correction_input_device(const float *d_e, const float *d_g, const uint8_t *d_cls,
+                                     int32_t pitch, int32_t pw, int32_t ph, int32_t ox,
+                                     int32_t oy, int32_t x0, int32_t y0, int32_t factor,
+                                     int32_t res, double lambda, float *d_in, double *d_rsq,
+                                     void *stream, ph_error *err);
+
 /* Net input for one channel of an image (and, when d_solution is given, the
  * training target): the region [x0, x0 + res*f) x [y0, y0 + res*f) of the
  * image in f x f blocks. With interior = (mask > 0.5 and not on the image

@@ -392,33 +392,33 @@ __global__ __launch_bounds__(256) void k_probe_residual_spans(const PhInst *__re
   const double s = block_sum(probe_residual_row(I, I.e[S.buf], b - S.row0));
   if (threadIdx.x == 0) row_sums[b] = s;
 }
-// A learned coarse correction into the correction buffer and the FP64
-// residual of x + e, in one pass: one block per plane row of every listed
-// correction (span entry i takes args[i] and its correction buffer S.buf).
-// At interior cells e = sign * alpha * scale * E, where E interpolates the
-// prediction h bilinearly in FP32 (image cell ix sits at coarse coordinate
-// (ix - x0 + 0.5) / f - 0.5, clamped to the grid): vertically once per row
-// into LDS, then along the row. row_sums[b] = the row's sum of r(x + e)^2 in
-// FP64. A block computes E at its row's neighbour cells with the same
-// arithmetic as the blocks that own them, so it reads no other block's
-// writes. sign 0 writes e = 0 (a correction taken back).
-__global__ __launch_bounds__(256) void k_correct_probe_spans(const PhInst *__restrict__ insts,
-                                                             const PhSpan *__restrict__ spans,
-                                                             int n,
-                                                             const PhCorrArgs *__restrict__ args,
-                                                             const float *__restrict__ hc,
-                                                             const float *__restrict__ scale,
-                                                             const double *__restrict__ alpha,
-                                                             int res,
-                                                             double *__restrict__ row_sums) {
+// A learned coarse correction added to the FP32 state, and the residual of
+// the corrected state, in one pass: one block per plane row of every listed
+// correction (span entry i takes args[i]). The state is x + e with e in the
+// buffer the entry's control block names; the corrected correction e' = e + alpha * scale * E goes to
+// the other buffer (so taking the correction back costs nothing: the solve
+// keeps S.buf), where E interpolates the prediction h bilinearly in FP32
+// (image cell ix sits at coarse coordinate (ix - x0 + 0.5) / f - 0.5, clamped
+// to the grid): vertically once per row into LDS, then along the row.
+// row_sums[b] = the row's sum of r(x + e')^2 = (L e' - g)^2 in FP64 over
+// each cell's FP32 residual (the sweeps' own formula). A block computes e' at its row's neighbour
+// cells with the same arithmetic as the blocks that own them, so it reads no
+// other block's writes.
+__global__ __launch_bounds__(256) void k_corr_apply_spans(const PhInst *__restrict__ insts,
+                                                          const PhCtl *__restrict__ ctl,
+                                                          const PhSpan *__restrict__ spans, int n,
+                                                          const PhCorrArgs *__restrict__ args,
+                                                          const float *__restrict__ hc,
+                                                          const float *__restrict__ scale,
+                                                          const double *__restrict__ alpha,
+                                                          int res, double *__restrict__ row_sums) {
   const int b = xcd_row_group(blockIdx.x, gridDim.x);
   const int i = ph_span_find(spans, n, b);
   const PhSpan S = spans[i];
   const PhCorrArgs A = args[i];
   const PhInst &I = insts[S.slot];
-  const float ca = (float)(A.sign * alpha[A.net] * (double)scale[A.net]);
+  const float ca = (float)(alpha[A.net] * (double)scale[A.net]);
   const float *h = hc + (size_t)res * res * A.net;
-  float *e = I.e[S.buf];
   const int py = b - S.row0;
   const float invf = 1.0f / (float)A.f;
   __shared__ float vrow[3][PH_CORR_RES_MAX];  // rows py - 1, py, py + 1
@@ -432,10 +432,12 @@ __global__ __launch_bounds__(256) void k_correct_probe_spans(const PhInst *__res
   }
   __syncthreads();
   // The instance's fields in registers (a store to e could alias them).
-  const double *__restrict__ x = I.x;
-  const double *__restrict__ bp = I.bp;
+  const uint32_t buf = ctl[S.slot].cur_buf;
+  const float *__restrict__ eo = I.e[buf];
+  float *__restrict__ en = I.e[buf ^ 1u];
+  const float *__restrict__ g = I.g;
   const uint8_t *__restrict__ cls = I.cls;
-  const double dg = I.dg64;
+  const float dg = I.dg;
   const int pw = I.pw, ox = I.ox;
   const size_t pitch = (size_t)I.pitch;
   // the horizontal interpolation position of plane column px
@@ -446,9 +448,10 @@ __global__ __launch_bounds__(256) void k_correct_probe_spans(const PhInst *__res
     i1 = min(i0 + 1, res - 1);
     w = u - (float)i0;
   };
-  // E at a cell of row r (0: py - 1, 1: py, 2: py + 1), 0 off the interior
-  auto corr = [&](int r, bool in, int i0, int i1, float w) -> double {
-    return in ? (double)(ca * fmaf(w, vrow[r][i1] - vrow[r][i0], vrow[r][i0])) : 0.0;
+  // e' at a cell of row r (0: py - 1, 1: py, 2: py + 1) from its e; e = 0 and
+  // no correction off the interior
+  auto upd = [&](float ev, int r, bool in, int i0, int i1, float w) -> float {
+    return in ? ev + ca * fmaf(w, vrow[r][i1] - vrow[r][i0], vrow[r][i0]) : ev;
   };
   double s = 0.0;
   if (py >= 1 && py < I.ph - 1) {
@@ -467,20 +470,20 @@ __global__ __launch_bounds__(256) void k_correct_probe_spans(const PhInst *__res
       col(px + 1, b0, b1, wb);
       col(px - 1, l0, l1, wl);
       col(px + 2, r0_, r1_, wr);
-      const double e0 = corr(1, c.x, a0, a1, wa), e1 = corr(1, c.y, b0, b1, wb);
-      if (c.x) e[p] = (float)e0;
-      if (c.y) e[p + 1] = (float)e1;
-      const double2 xc = *reinterpret_cast<const double2 *>(x + p);
-      const double2 xu = *reinterpret_cast<const double2 *>(x + p - pitch);
-      const double2 xd = *reinterpret_cast<const double2 *>(x + p + pitch);
-      const double2 bb = *reinterpret_cast<const double2 *>(bp + p);
-      const double xc0 = xc.x + e0, xc1 = xc.y + e1;
-      const double xl = x[p - 1] + corr(1, cl, l0, l1, wl);
-      const double xr = x[p + 2] + corr(1, cr, r0_, r1_, wr);
-      const double xu0 = xu.x + corr(0, cu.x, a0, a1, wa), xu1 = xu.y + corr(0, cu.y, b0, b1, wb);
-      const double xd0 = xd.x + corr(2, cd.x, a0, a1, wa), xd1 = xd.y + corr(2, cd.y, b0, b1, wb);
-      const double q0 = fma(dg, -xc0, ((xl + xc1) + xu0) + xd0) - bb.x;
-      const double q1 = fma(dg, -xc1, ((xc0 + xr) + xu1) + xd1) - bb.y;
+      const float2 ec = *reinterpret_cast<const float2 *>(eo + p);
+      const float2 eu = *reinterpret_cast<const float2 *>(eo + p - pitch);
+      const float2 ed = *reinterpret_cast<const float2 *>(eo + p + pitch);
+      const float2 gv = *reinterpret_cast<const float2 *>(g + p);
+      const float e0 = upd(ec.x, 1, c.x, a0, a1, wa), e1 = upd(ec.y, 1, c.y, b0, b1, wb);
+      if (c.x) en[p] = e0;
+      if (c.y) en[p + 1] = e1;
+      const float el = upd(eo[p - 1], 1, cl, l0, l1, wl);
+      const float er = upd(eo[p + 2], 1, cr, r0_, r1_, wr);
+      const float eu0 = upd(eu.x, 0, cu.x, a0, a1, wa), eu1 = upd(eu.y, 0, cu.y, b0, b1, wb);
+      const float ed0 = upd(ed.x, 2, cd.x, a0, a1, wa), ed1 = upd(ed.y, 2, cd.y, b0, b1, wb);
+      // the sweeps' own FP32 residual of each cell, squares summed in FP64
+      const double q0 = fmaf(-dg, e0, (((el + e1) + eu0) + ed0) - gv.x);
+      const double q1 = fmaf(-dg, e1, (((e0 + er) + eu1) + ed1) - gv.y);
       if (c.x) s = fma(q0, q0, s);
       if (c.y) s = fma(q1, q1, s);
     }

@@ -32,8 +32,7 @@ struct PhInst {
   int32_t img_w, img_h, channels, channel; // source image geometry for gather/scatter
   double omega;        // SOR factor (after the ramp)
   double ramp_log;     // ln(2 - omega) / ramp_sweeps
-  uint32_t ramp_sweeps; // sweeps of the warm-start ramp (0: none)
-  uint32_t ramp_from;   // first sweep of the current ramp (it restarts after a coarse correction)
+  uint32_t ramp_sweeps; // sweeps of the warm-start ramp (0: none; it starts at PhCtl::ramp_from)
   float dg;         // 4 + lambda (FP32)
   double dg64;      // 4 + lambda (FP64)
   double lambda;
@@ -79,7 +78,7 @@ struct PhCtl {
   uint32_t stall_window; // sweeps without a new best that end the solve as stalled
   uint32_t refine_every; // refine at least this often (sweeps): bounds |e| and the FP32 floor
   uint32_t last_refine;  // sweep count of the last refinement
-  uint32_t pad_;
+  uint32_t ramp_from;    // first sweep of the current omega ramp (it restarts after a coarse correction)
 };
 
 // SOR factor of half-sweep h (0-based). Both half-sweeps of sweep k = h/2
@@ -88,18 +87,18 @@ struct PhCtl {
 // 2 - (2 - omega)^((k - ramp_from) / ramp_sweeps): Gauss-Seidel first (the
 // high-frequency error a coarse prediction leaves decays fastest there), then
 // omega.
-__host__ __device__ inline double ph_omega_at(const PhInst &I, uint64_t h) {
+__host__ __device__ inline double ph_omega_at(const PhInst &I, uint32_t ramp_from, uint64_t h) {
   const uint64_t k = h >> 1;
-  if (k < I.ramp_from || k - I.ramp_from >= I.ramp_sweeps) return I.omega;
-  return 2.0 - exp((double)(k - I.ramp_from) * I.ramp_log);
+  if (k < ramp_from || k - ramp_from >= I.ramp_sweeps) return I.omega;
+  return 2.0 - exp((double)(k - ramp_from) * I.ramp_log);
 }
 
 // Sweep m (1-based) runs inside an omega ramp. The residual of a ramp is not
 // monotone (it dips while omega is near 1, then rises as omega grows and the
 // smooth error shows), so the stall guard holds still during one.
-__host__ __device__ inline bool ph_in_ramp(const PhInst &I, uint32_t m) {
+__host__ __device__ inline bool ph_in_ramp(const PhInst &I, uint32_t ramp_from, uint32_t m) {
   const uint32_t k = m - 1;
-  return I.ramp_sweeps != 0 && k >= I.ramp_from && k - I.ramp_from < I.ramp_sweeps;
+  return I.ramp_sweeps != 0 && k >= ramp_from && k - ramp_from < I.ramp_sweeps;
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -186,7 +185,6 @@ struct PhCorrArgs {
   int32_t cpb;        // coarse cells per block of the input coarsening
   int32_t net;        // network batch index: prediction h + res*res*net, scale[net], alpha[net]
   int32_t item, pad;  // image index (the warm start reads the image)
-  double sign;        // +1 applies the correction, -1 takes it back
 };
 
 // Largest coarse-grid side a correction kernel stages in LDS.

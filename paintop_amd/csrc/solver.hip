@@ -67,17 +67,18 @@ __global__ void k_coarsen_groups(const PhInst *, const PhCorrArgs *, const PhSpa
 __global__ void k_warm_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
                                    const PhItemIO *, const float *, const float *, int);
 __global__ void k_coarsen_plane(PhInst, int, int, int, int, int, float *);
-__global__ void k_coarsen_plane_list(const PhInst *, const PhCorrArgs *, int, float *);
-__global__ void k_correct_probe_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
-                                      const float *, const float *, const double *, int,
-                                      double *);
+__global__ void k_corr_input_list(const PhInst *, const PhCtl *, const PhCorrArgs *, int, float *,
+                                  double *);
+__global__ void k_corr_apply_spans(const PhInst *, const PhCtl *, const PhSpan *, int,
+                                   const PhCorrArgs *, const float *, const float *,
+                                   const double *, int, double *);
 __global__ void k_put_values(const PhValue *, int);
 __global__ void k_copy_anchor_spans(const PhInst *, const PhInst *, const PhSpan *, int);
 int coarsen_plane_cpb(int f);
 __global__ void k_in_norm_part(const float *, int, double *);
 __global__ void k_in_norm_apply(float *, int, const double *, float *);
-__global__ void k_corr_alpha_part(const float *, const float *, int, double *);
-__global__ void k_corr_alpha(const double *, int, double, double *);
+__global__ void k_corr_alpha_part(const float *, const float *, const double *, int, double *);
+__global__ void k_corr_alpha(const double *, int, double, double *, double *);
 __global__ void k_start_apply(PhInst, const float *);
 bool warm_net_ready();
 int warm_net_res();
@@ -637,6 +638,15 @@ struct ChanResult {
 // absorbing the late, small updates.
 static const double REFINE_RATIO = 1e-3;
 static const uint32_t REFINE_EVERY = 1024;
+// A cycle end refines (and its correction follows on the refined state) once
+// the residual has fallen 1/REFINE_RATIO/CYCLE_REFINE_AHEAD-fold since the
+// last refinement, about 30x: a cycle gains 10-50x, so the refinement
+// trigger would otherwise fire inside the next cycle, halting the instance
+// in the middle of a batch of launches planned to the cycle's end.
+static const double CYCLE_REFINE_AHEAD = 0.03;
+// Right after a correction the residual dips within a quarter of its omega
+// ramp; near the tolerance a batch then plans one launch at a time.
+static const double NEAR_TOLERANCE = 30.0;
 // Below this many interior cells the SOR factor comes from the geometric
 // bounds instead of the coarse-mask eigenvalue (small domains converge in a
 // few hundred sweeps; the host-side eigen solve would cost more than it saves).
@@ -741,7 +751,23 @@ struct SpanTable {
     if (!h.empty())
       HIP_OK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(PhSpan), hipMemcpyHostToDevice, st));
   }
+  // through pinned staging (one DMA, no staging wait): the sweep loop's
+  // tables; the stage must hold the entries and stay unchanged until the
+  // stream has been synchronised
+  void upload(PhSpan *d, hipStream_t st, PhSpan *stage) const {
+    if (h.empty()) return;
+    memcpy(stage, h.data(), h.size() * sizeof(PhSpan));
+    HIP_OK(hipMemcpyAsync(d, stage, h.size() * sizeof(PhSpan), hipMemcpyHostToDevice, st));
+  }
 };
+
+// n values of T to the device through pinned staging (see SpanTable::upload).
+template <class T>
+static void upload_pinned(T *d, const T *h, size_t n, T *stage, hipStream_t st) {
+  if (n == 0) return;
+  memcpy(stage, h, n * sizeof(T));
+  HIP_OK(hipMemcpyAsync(d, stage, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
 
 static size_t item_need(const ph_solve_desc *d, int W, int H, int C, int K, int hb) {
   const int WV = 256 - 2 * K;
@@ -843,12 +869,17 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       for (const ItemIn &it : items) nch += it.C;
       need += (R2 * 4 * warm_net_batch() + (R2 + 1) * nch) * sizeof(float) +
               (nch + 2 * PH_NET_SLICES * (size_t)warm_net_batch()) * sizeof(double) + 1536;
+      // the correction rounds' squared-residual cells and norms
+      need += (R2 + 1) * nch * sizeof(double) + 512;
     }
     g_pool.reset(need);
     size_t nch = 0;
     for (const ItemIn &it : items) nch += it.C;
-    g_pinned.reset(4096 + 2 * std::max<size_t>(nch, 1) * (2 * sizeof(double) + sizeof(PhCtl)) +
-                   2 * 256);
+    // control blocks, norms and the sweep loop's staging (instances and twins)
+    g_pinned.reset(4096 + 2 * std::max<size_t>(nch, 1) *
+                              (4 * sizeof(double) + sizeof(PhCtl) + 2 * sizeof(PhSpan) +
+                               sizeof(PhCorrArgs) + sizeof(PhValue) + sizeof(PhInst)) +
+                   10 * 256);
   }
   // host time per phase (printed with PAINTOP_HIP_TRACE=1)
   const bool phase_trace = trace_on();
@@ -1224,15 +1255,14 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                PH_CORR_RES_MAX);
   const size_t NR2 = (size_t)NR * NR;
   float *net_in = nullptr, *net_h = nullptr, *net_scale = nullptr;
-  double *corr_alpha = nullptr, *net_part = nullptr;
+  double *net_part = nullptr;
   if (!net_list.empty()) {
     net_in = g_pool.take<float>(NR2 * 4 * NNB);
     // one prediction and one input scale per network instance (the warm
     // starts, then the corrections of any subset of them)
     net_h = g_pool.take<float>(NR2 * net_list.size());
     net_scale = g_pool.take<float>(net_list.size());
-    corr_alpha = g_pool.take<double>(net_list.size());
-    net_part = g_pool.take<double>((size_t)2 * PH_NET_SLICES * NNB);
+    net_part = g_pool.take<double>((size_t)3 * PH_NET_SLICES * NNB);
   }
   const uint32_t max_corr = warm_max_corrections();
   const uint64_t corr_min_cells = warm_gate("PAINTOP_HIP_CORR_MIN_CELLS", largest);
@@ -1284,7 +1314,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       for (int j = 0; j < nn; ++j) {
         const int k = net_list[j];
         const Geo &g = wgeo[k];
-        warm_args[j] = PhCorrArgs{k, g.x0, g.y0, g.f, 0, j, inst[k].item, 0, 1.0};
+        warm_args[j] = PhCorrArgs{k, g.x0, g.y0, g.f, 0, j, inst[k].item, 0};
         warm_rows.add(k, inst[k].I.ph, inst[k].item);
       }
       // the coarsening takes an image's channels together: groups of up to 4
@@ -1410,16 +1440,25 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     PhSpan *ref_spans_d = g_pool.take<PhSpan>(nact);
     // correction rounds: arguments and span tables of the corrections and
     // of the take-backs, the history entries of the kept ones
-    PhCorrArgs *corr_args_d = nullptr, *undo_args_d = nullptr;
-    PhSpan *corr_spans_d = nullptr, *undo_spans_d = nullptr, *fin_spans_d = nullptr;
+    PhCorrArgs *corr_args_d = nullptr;
+    PhSpan *corr_spans_d = nullptr;
     PhValue *corr_vals_d = nullptr;
-    if (!net_list.empty()) {
+    // a round's results in one array, read back in one copy: per entry j the
+    // corrected state's norm and its square (2j, 2j + 1), from 2 nn on the
+    // steps, from 3 nn on the norms of the states before the corrections
+    const size_t nn = net_list.size();
+    double *corr_rsq_d = nullptr, *round_d = nullptr, *corr_alpha = nullptr,
+           *corr_rnorm_d = nullptr;
+    if (nn) {
       corr_args_d = g_pool.take<PhCorrArgs>(nact);
-      undo_args_d = g_pool.take<PhCorrArgs>(nact);
       corr_spans_d = g_pool.take<PhSpan>(nact);
-      undo_spans_d = g_pool.take<PhSpan>(nact);
-      fin_spans_d = g_pool.take<PhSpan>(nact);
       corr_vals_d = g_pool.take<PhValue>(nact);
+      // per coarse cell of each correction: the sum of r(x + e)^2 (a round
+      // corrects only network instances)
+      corr_rsq_d = g_pool.take<double>(NR2 * nn);
+      round_d = g_pool.take<double>(4 * nn);
+      corr_alpha = round_d + 2 * nn;
+      corr_rnorm_d = round_d + 3 * nn;
     }
     SpanTable ref;
     std::vector<PhTile> tl;
@@ -1460,6 +1499,13 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     build_tiles(running);
 
     PhCtl *ctl = g_pinned.take<PhCtl>(nact);
+    // pinned staging of the loop's small transfers (each region is written
+    // once between two synchronisations of the stream)
+    PhSpan *ref_stage = g_pinned.take<PhSpan>(nact);
+    PhSpan *corr_stage = g_pinned.take<PhSpan>(nact);
+    PhCorrArgs *args_stage = g_pinned.take<PhCorrArgs>(nact);
+    PhValue *vals_stage = g_pinned.take<PhValue>(nact);
+    double *round_h = g_pinned.take<double>(4 * (size_t)nact);
     for (int a = 0; a < nact; ++a) {
       const InstWork &X = inst[act[a]];
       const ItemWork &w = iw[X.item];
@@ -1493,7 +1539,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     };
     // stall rule of the reference for a host-evaluated sweep (poisson.rs:220-228)
     auto stall_step = [&](int a, PhCtl &Cc, uint32_t m, double rel) -> bool {
-      if (ph_in_ramp(ai[a], m)) return false;  // on hold inside an omega ramp
+      if (ph_in_ramp(ai[a], Cc.ramp_from, m)) return false;  // on hold inside an omega ramp
       if (rel < Cc.best * (1.0 - PH_STALL_IMPROVEMENT)) {
         Cc.best = rel;
         Cc.since = 0;
@@ -1555,64 +1601,86 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     std::vector<Trend> trend(nact);
     uint32_t total_launches = 0, total_refine = 0;
     double sweep_ms = 0.0;
-    // learned coarse corrections at the cycle ends collected by pass 2 (all
-    // instances of one pass share the network batches and one readback)
+    // Learned coarse corrections at the cycle ends. An entry comes either from
+    // a refinement at its cycle end (gnorm = its FP64 residual, e = 0) or from
+    // a sweep halt at its cycle end with no refinement due (gnorm < 0: the
+    // residual of sweep m is not known yet). All entries of a pass share the
+    // network batches and one readback.
     struct Corr {
       int a;
       uint32_t m;
       double gnorm;
     };
-    std::vector<Corr> corr;
+    std::vector<Corr> corr, cyc_end;
     // host staging of the correction rounds (kept until the stream is synchronised)
-    std::vector<PhCorrArgs> corr_args, undo_args;
+    std::vector<PhCorrArgs> corr_args;
     std::vector<PhValue> corr_vals;
-    std::vector<PhInst> ai_stage;
-    SpanTable corr_rows, undo_rows, fin_rows;
+    SpanTable corr_rows;
+    // a cycle that ends without a usable correction: the cycles end
+    auto end_cycles = [&](int a, uint32_t m, double rel, const char *why) {
+      const int k = act[a];
+      cyc_next[k] = 0;
+      ctl[a].sweep_limit = run_limit(a);
+      if (trace_on())
+        fprintf(stderr, "[corr] inst %d: cycles end at sweep %u (rel %.3e, %u kept, %s)\n", k, m,
+                rel, cyc_count[k], why);
+    };
+    // the gain rules on the residual at a cycle end: false ends the cycles
+    auto cycle_gains = [&](int k, double rel) -> bool {
+      // (after a correction taken back, the cycle ran uncorrected: no test)
+      bool gain = true;
+      if (cyc_count[k] > 0 && !cyc_undone[k]) {
+        const bool weak = rel * CORR_GOOD_GAIN > cyc_prev[k];
+        gain = rel * CORR_MIN_GAIN <= cyc_prev[k] && !(weak && cyc_weak[k]);
+        cyc_weak[k] = weak ? 1 : 0;
+      }
+      cyc_prev[k] = rel;
+      return gain && cyc_count[k] < max_corr;
+    };
     auto correct = [&]() {
-      // cycles that stopped paying for themselves end here, without a correction
+      // refined entries: cycles that stopped paying for themselves end here,
+      // without a correction
       std::vector<Corr> go;
       for (const Corr &c : corr) {
-        const int k = act[c.a];
-        const double rel = c.gnorm / inst[k].r0;
-        // (after a correction taken back, the cycle ran uncorrected: no test)
-        bool gain = true;
-        if (cyc_count[k] > 0 && !cyc_undone[k]) {
-          const bool weak = rel * CORR_GOOD_GAIN > cyc_prev[k];
-          gain = rel * CORR_MIN_GAIN <= cyc_prev[k] && !(weak && cyc_weak[k]);
-          cyc_weak[k] = weak ? 1 : 0;
-        }
-        cyc_prev[k] = rel;
-        if (!gain || cyc_count[k] >= max_corr || !(c.gnorm > 0.0)) {
-          cyc_next[k] = 0;
-          ctl[c.a].sweep_limit = run_limit(c.a);
-          if (trace_on())
-            fprintf(stderr, "[corr] inst %d: cycles end at sweep %u (rel %.3e, %u kept)\n", k,
-                    c.m, rel, cyc_count[k]);
+        const double rel = c.gnorm / inst[act[c.a]].r0;
+        if (!(c.gnorm > 0.0) || !cycle_gains(act[c.a], rel)) {
+          end_cycles(c.a, c.m, rel, "gain");
           continue;
         }
         go.push_back(c);
       }
+      // unrefined entries run the round speculatively; their sweep-m residual
+      // comes back with it and decides afterwards
+      for (const Corr &c : cyc_end) go.push_back(c);
+      corr.clear();
+      cyc_end.clear();
       if (go.empty()) return;
       const int nc = (int)go.size();
-      corr_args.resize(nc);
-      corr_rows.clear();
-      for (int j = 0; j < nc; ++j) {
-        const int a = go[j].a, k = act[a];
-        const Geo &g = wgeo[k];
-        corr_args[j] =
-            PhCorrArgs{a, g.x0, g.y0, g.f, coarsen_plane_cpb(g.f), j, inst[k].item, 0, 1.0};
-        corr_rows.add(a, ai[a].ph, 0, (int)ctl[a].cur_buf, k);
+      // the round's tables depend on its instances only (the kernels take
+      // each state's buffer from its control block): uploaded when they change
+      bool same = (int)corr_args.size() == nc;
+      for (int j = 0; same && j < nc; ++j) same = corr_args[j].slot == go[j].a;
+      if (!same) {
+        corr_args.resize(nc);
+        corr_rows.clear();
+        for (int j = 0; j < nc; ++j) {
+          const int a = go[j].a, k = act[a];
+          const Geo &g = wgeo[k];
+          corr_args[j] = PhCorrArgs{a, g.x0, g.y0, g.f, coarsen_plane_cpb(g.f), j, inst[k].item,
+                                    0};
+          corr_rows.add(a, ai[a].ph, 0, 0, j);
+        }
+        upload_pinned(corr_args_d, corr_args.data(), nc, args_stage, st);
+        corr_rows.upload(corr_spans_d, st, corr_stage);
       }
-      HIP_OK(hipMemcpyAsync(corr_args_d, corr_args.data(), nc * sizeof(PhCorrArgs),
-                            hipMemcpyHostToDevice, st));
-      corr_rows.upload(corr_spans_d, st);
       for (int c0 = 0; c0 < nc; c0 += NNB) {
         const int nb = std::min(NNB, nc - c0);
         int gx = 1;
         for (int j = c0; j < c0 + nb; ++j)
           gx = std::max(gx, (NR + corr_args[j].cpb - 1) / corr_args[j].cpb);
-        hipLaunchKernelGGL(k_coarsen_plane_list, dim3(gx, NR, nb), dim3(256), 0, st, inst_d,
-                           corr_args_d + c0, NR, net_in);
+        hipLaunchKernelGGL(k_corr_input_list, dim3(gx, NR, nb), dim3(256), 0, st, inst_d,
+                           (const PhCtl *)ctl_d, corr_args_d + c0, NR, net_in,
+                           corr_rsq_d + NR2 * c0);
         hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
                            net_part);
         hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
@@ -1620,70 +1688,77 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         HIP_OK(hipGetLastError());
         warm_net_forward(net_in, net_h + NR2 * c0, nb, st);
         hipLaunchKernelGGL(k_corr_alpha_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in,
-                           net_h + NR2 * c0, NR, net_part);
+                           net_h + NR2 * c0, (const double *)(corr_rsq_d + NR2 * c0), NR,
+                           net_part);
         hipLaunchKernelGGL(k_corr_alpha, dim3((nb + 63) / 64), dim3(64), 0, st, net_part, nb,
-                           CORR_MAX_STEP, corr_alpha + c0);
+                           CORR_MAX_STEP, corr_alpha + c0, corr_rnorm_d + c0);
         HIP_OK(hipGetLastError());
       }
-      // every correction into its correction buffer, and the FP64 residual
-      // of x + e (x and g stay as the refinement left them)
-      hipLaunchKernelGGL(k_correct_probe_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
-                         corr_spans_d, nc, corr_args_d, net_h, net_scale, corr_alpha, NR,
-                         ref_sums_d);
+      // every correction into the other correction buffer and the residual
+      // of the corrected state (the norm of the state each entry's sweeps
+      // left came with its step)
+      hipLaunchKernelGGL(k_corr_apply_spans, dim3(corr_rows.rows), dim3(256), 0, st, inst_d,
+                         (const PhCtl *)ctl_d, corr_spans_d, nc, corr_args_d, net_h, net_scale,
+                         corr_alpha, NR, ref_sums_d);
       hipLaunchKernelGGL(k_residual_norm_spans, dim3(nc), dim3(256), 0, st, corr_spans_d,
-                         ref_sums_d, resv_d);
+                         ref_sums_d, round_d);
       HIP_OK(hipGetLastError());
-      std::vector<double> alpha(nc);
-      HIP_OK(hipMemcpyAsync(alpha.data(), corr_alpha, nc * sizeof(double), hipMemcpyDeviceToHost,
-                            st));
-      HIP_OK(hipMemcpyAsync(resv, resv_d, 2 * inst.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
+      HIP_OK(hipMemcpyAsync(round_h, round_d, 4 * nn * sizeof(double), hipMemcpyDeviceToHost, st));
       HIP_OK(hipStreamSynchronize(st));
-      std::vector<char> undo(nc, 0);
-      std::vector<double> jump(nc, 0.0);
+      const double *alpha = round_h + 2 * nn, *rnorm = round_h + 3 * nn;
       const double max_jump = corr_max_jump();
-      undo_args.clear();
-      undo_rows.clear();
-      for (int j = 0; j < nc; ++j) {
-        const int a = go[j].a, k = act[a];
-        if (alpha[j] == 0.0) resv[2 * k] = go[j].gnorm;  // e stays 0
-        jump[j] = resv[2 * k] / go[j].gnorm;
-        if (alpha[j] != 0.0 && !(std::isfinite(jump[j]) && jump[j] <= max_jump)) {
-          PhCorrArgs A = corr_args[j];
-          A.sign = 0.0;
-          undo_args.push_back(A);
-          undo_rows.add(a, ai[a].ph, 0, (int)ctl[a].cur_buf, k);
-          undo[j] = 1;
-          resv[2 * k] = go[j].gnorm;  // e = 0 again: the refinement's residual
-        }
-      }
-      if (!undo_args.empty()) {
-        // taken back: the correction buffers of these instances are zeroed
-        const int nu = (int)undo_args.size();
-        HIP_OK(hipMemcpyAsync(undo_args_d, undo_args.data(), nu * sizeof(PhCorrArgs),
-                              hipMemcpyHostToDevice, st));
-        undo_rows.upload(undo_spans_d, st);
-        hipLaunchKernelGGL(k_correct_probe_spans, dim3(undo_rows.rows), dim3(256), 0, st, inst_d,
-                           undo_spans_d, nu, undo_args_d, net_h, net_scale, corr_alpha, NR,
-                           ref_sums_d);
-        HIP_OK(hipGetLastError());
-      }
       corr_vals.clear();
-      fin_rows.clear();
-      bool ramp_moved = false;
       for (int j = 0; j < nc; ++j) {
         const int a = go[j].a, k = act[a];
         const uint32_t m = go[j].m;
         PhCtl &Cc = ctl[a];
-        const double gnorm = resv[2 * k];
-        const double rel = gnorm / Cc.r0;
+        double gnorm = go[j].gnorm;
+        int hist_at = -1;  // this entry's history value of sweep m in corr_vals
+        if (gnorm < 0.0) {
+          // sweep m of an unrefined entry: its residual, ||L e - g|| in FP64
+          // from the stored state (the grid covers every interior cell)
+          gnorm = rnorm[j];
+          const double rel = gnorm / Cc.r0;
+          // a stop, a stall or a refinement due at sweep m: the FP64 rules of
+          // a refinement decide at m instead (the correction is not taken;
+          // the refinement's pass queues the cycle's correction again)
+          const bool hold = ph_in_ramp(ai[a], Cc.ramp_from, m);
+          const bool better = rel < Cc.best * (1.0 - PH_STALL_IMPROVEMENT);
+          const uint32_t since = (hold || better) ? Cc.since : Cc.since + 1;
+          if (!(rel > Cc.tol) || !std::isfinite(rel) || since >= Cc.stall_window ||
+              (!hold && !better && since == (Cc.stall_window >> 1)) ||
+              gnorm * gnorm <= Cc.refine_abs2) {
+            Cc.halt = PH_HALT_REFINE;
+            if (trace_on())
+              fprintf(stderr, "[corr] inst %d sweep %u: rel %.3e, refinement first\n", k, m, rel);
+            continue;
+          }
+          // the reference's rules for sweep m (no rule fires)
+          if (!hold) {
+            if (better) Cc.best = rel;
+            Cc.since = since;
+          }
+          Cc.sweeps_final = m;
+          Cc.last_rel = rel;
+          hist_at = (int)corr_vals.size();
+          corr_vals.push_back(PhValue{Cc.history + (m - 1), rel});
+          if (!cycle_gains(k, rel)) {
+            end_cycles(a, m, rel, "gain");
+            Cc.halt = PH_RUN;
+            continue;
+          }
+        }
+        const double jump = round_h[2 * j] / gnorm;
+        const bool usable = alpha[j] != 0.0;
+        const bool undo = usable && !(std::isfinite(jump) && jump <= max_jump);
+        const double rel_new = round_h[2 * j] / Cc.r0;
         if (trace_on())
           fprintf(stderr, "[corr] inst %d sweep %u: rel %.3e -> %.3e (step %.3f%s)\n", k, m,
-                  go[j].gnorm / Cc.r0, go[j].gnorm * jump[j] / Cc.r0, alpha[j],
-                  undo[j] ? ", taken back" : "");
-        if (alpha[j] == 0.0 || (undo[j] && cyc_undone[k])) {
-          cyc_next[k] = 0;  // no usable correction: the cycles end
-        } else if (undo[j]) {
+                  gnorm / Cc.r0, rel_new, alpha[j], undo ? ", taken back" : "");
+        if (!usable || (undo && cyc_undone[k])) {
+          end_cycles(a, m, gnorm / Cc.r0, usable ? "taken back twice" : "no step");
+        } else if (undo) {
+          // the correction buffer is not switched: the state stays the sweeps'
           cyc_undone[k] = 1;
           cyc_next[k] = m + cyc_len[k];
         } else {
@@ -1691,43 +1766,32 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           cyc_count[k] += 1;
           iw[inst[k].item].in.rep->coarse_corrections += 1;
           cyc_next[k] = m + cyc_len[k];
+          Cc.cur_buf ^= 1u;  // the corrected state
           // the omega ramp restarts at the corrected state
-          ai[a].ramp_from = m;
-          inst[k].I.ramp_from = m;
-          ramp_moved = true;
-          corr_vals.push_back(PhValue{Cc.history + (m - 1), rel});
+          Cc.ramp_from = m;
+          if (hist_at >= 0) corr_vals[hist_at].val = rel_new;
+          else corr_vals.push_back(PhValue{Cc.history + (m - 1), rel_new});
           // the residual dips and then rises through the restarted ramp: the
           // stall guard starts again after it, the decay estimate from here
           Cc.best = INFINITY;
           Cc.since = 0;
-          Cc.last_rel = rel;
-          trend[a] = Trend{m, rel};
+          Cc.sweeps_final = m;
+          Cc.last_rel = rel_new;
+          trend[a] = Trend{m, rel_new};
+          if (!(rel_new > Cc.tol)) {
+            // converged on the correction: a refinement (x += e) confirms it
+            // with the FP64 rules
+            Cc.halt = PH_HALT_REFINE;
+            continue;
+          }
         }
-        if (rel <= Cc.tol) {
-          // converged with a kept correction in e: x takes it (x += e is the
-          // sum the residual was evaluated at)
-          if (alpha[j] != 0.0 && !undo[j]) fin_rows.add(a, ai[a].ph, 0, (int)Cc.cur_buf, k);
-          finish(a, PH_STOP_CONVERGED, m, rel);
-          --remaining;
-        } else {
-          resume(a, gnorm);
-        }
+        Cc.halt = PH_RUN;
+        Cc.mode = PH_MODE_NORMAL;
+        Cc.sweep_limit = run_limit(a);
       }
-      if (!fin_rows.h.empty()) {
-        fin_rows.upload(fin_spans_d, st);
-        hipLaunchKernelGGL(k_apply_correction_spans, dim3(fin_rows.rows), dim3(256), 0, st, inst_d,
-                           fin_spans_d, (int)fin_rows.h.size());
-        HIP_OK(hipGetLastError());
-      }
-      // the restarted ramps and the kept corrections' history entries
-      if (ramp_moved) {
-        ai_stage = ai;
-        HIP_OK(hipMemcpyAsync(inst_d, ai_stage.data(), nact * sizeof(PhInst),
-                              hipMemcpyHostToDevice, st));
-      }
+      // the restarted ramps and the history entries of sweep m
       if (!corr_vals.empty()) {
-        HIP_OK(hipMemcpyAsync(corr_vals_d, corr_vals.data(), corr_vals.size() * sizeof(PhValue),
-                              hipMemcpyHostToDevice, st));
+        upload_pinned(corr_vals_d, corr_vals.data(), corr_vals.size(), vals_stage, st);
         hipLaunchKernelGGL(k_put_values, dim3((corr_vals.size() + 255) / 256), dim3(256), 0, st,
                            corr_vals_d, (int)corr_vals.size());
         HIP_OK(hipGetLastError());
@@ -1735,16 +1799,26 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     };
 
     while (remaining > 0) {
-      if (timing) HIP_OK(hipEventRecord(eb0, st));
-      launch_sor_batch(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb, batch);
-      total_launches += (uint32_t)batch;
-      if (timing) HIP_OK(hipEventRecord(eb1, st));
-      HIP_OK(hipMemcpyAsync(ctl, ctl_d, nact * sizeof(PhCtl), hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      if (timing) {
-        float bms = 0.f;
-        HIP_OK(hipEventElapsedTime(&bms, eb0, eb1));
-        sweep_ms += bms;
+      // no instance running and one waiting for its refinement: no launch
+      // (the host copy of the control blocks is the current one)
+      bool any_run = false, any_refine = false;
+      for (int a = 0; a < nact; ++a) {
+        if (inst[act[a]].R.done || held[a]) continue;
+        any_run = any_run || ctl[a].halt == PH_RUN;
+        any_refine = any_refine || ctl[a].halt == PH_HALT_REFINE;
+      }
+      if (any_run || !any_refine) {
+        if (timing) HIP_OK(hipEventRecord(eb0, st));
+        launch_sor_batch(K, PF, ntiles, nact, st, tiles_d, inst_d, ctl_d, partials_d, hb, batch);
+        total_launches += (uint32_t)batch;
+        if (timing) HIP_OK(hipEventRecord(eb1, st));
+        HIP_OK(hipMemcpyAsync(ctl, ctl_d, nact * sizeof(PhCtl), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (timing) {
+          float bms = 0.f;
+          HIP_OK(hipEventElapsedTime(&bms, eb0, eb1));
+          sweep_ms += bms;
+        }
       }
       // pass 1: every instance that needs an FP64 evaluation gets its
       // refinement kernels; one readback serves all of them
@@ -1784,6 +1858,16 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                         Cc.sweeps_final, Cc.last_rel);
               break;
             }
+            if (cyc_next[act[a]] && Cc.sweeps_done >= cyc_next[act[a]] && !Cc.refine_pending &&
+                Cc.mode == PH_MODE_NORMAL && Cc.sweeps_done < Cc.max_it &&
+                !(Cc.last_rel * Cc.r0 * CYCLE_REFINE_AHEAD * Cc.last_rel * Cc.r0 *
+                      CYCLE_REFINE_AHEAD <=
+                  Cc.refine_abs2)) {
+              // a cycle end with no refinement due: the correction round reads
+              // the sweeps' state as it is
+              cyc_end.push_back(Corr{a, Cc.sweeps_done, -1.0});
+              break;
+            }
             pend.push_back({a, Cc.sweeps_done, -1});
             break;
           case PH_HALT_REFINE:
@@ -1801,7 +1885,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           const PhCtl &Cc = ctl[P.a];
           ref.add(P.a, ai[P.a].ph, 0, (int)Cc.cur_buf, act[P.a], Cc.history + (P.m - 1), Cc.r0);
         }
-        ref.upload(ref_spans_d, st);
+        ref.upload(ref_spans_d, st, ref_stage);
         const int nref = (int)ref.h.size();
         hipLaunchKernelGGL(k_apply_correction_spans, dim3(ref.rows), dim3(256), 0, st, inst_d,
                            ref_spans_d, nref);
@@ -1816,7 +1900,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         HIP_OK(hipStreamSynchronize(st));
         total_refine += (uint32_t)pend.size();
         // pass 2: the reference's rules on the exact FP64 residual of sweep m
-        corr.clear();
         for (size_t q = 0; q < pend.size(); ++q) {
           const Pending &P = pend[q];
           PhCtl &Cc = ctl[P.a];
@@ -1847,14 +1930,14 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
             corr.push_back(Corr{P.a, P.m, gnorm});
           }
         }
-        if (!corr.empty()) correct();
       }
+      if (!corr.empty() || !cyc_end.empty()) correct();
       if (!probe_pend.empty()) {
         // the FP64 residuals of the race members held at their probe
         ref.clear();
         for (int a : probe_pend)
           ref.add(a, ai[a].ph, 0, (int)ctl[a].cur_buf, act[a], nullptr, ctl[a].r0);
-        ref.upload(ref_spans_d, st);
+        ref.upload(ref_spans_d, st, ref_stage);
         const int nref = (int)ref.h.size();
         hipLaunchKernelGGL(k_probe_residual_spans, dim3(ref.rows), dim3(256), 0, st, inst_d,
                            ref_spans_d, nref, ref_sums_d);
@@ -1950,6 +2033,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
           uint32_t next = std::min(Cc.sweep_limit, Cc.last_refine + Cc.refine_every);
           if (Cc.mode == PH_MODE_REDO) next = Cc.sweep_limit;
           uint32_t todo = next > Cc.sweeps_done ? next - Cc.sweeps_done : 1;
+          if (Cc.mode == PH_MODE_NORMAL && ai[a].ramp_sweeps != 0 &&
+              Cc.sweeps_done >= Cc.ramp_from &&
+              Cc.sweeps_done - Cc.ramp_from < ai[a].ramp_sweeps / 4 && Cc.last_rel > 0.0 &&
+              Cc.last_rel < NEAR_TOLERANCE * Cc.tol)
+            todo = std::min<uint32_t>(todo, (uint32_t)(K / 2));
           Trend &T = trend[a];
           if (Cc.mode == PH_MODE_NORMAL && Cc.sweeps_final >= T.m + 16 && T.rel > 0.0 &&
               Cc.last_rel > 0.0 && Cc.last_rel < T.rel) {

@@ -74,7 +74,7 @@ __device__ bool finalize_sweep(PhCtl &C, const PhInst &I, uint32_t m, double res
     C.halt_sweep = m;
     return true;
   }
-  if (ph_in_ramp(I, m)) {
+  if (ph_in_ramp(I, C.ramp_from, m)) {
     // stall guard on hold
   } else if (rel < C.best * (1.0 - PH_STALL_IMPROVEMENT)) {
     C.best = rel;
@@ -103,7 +103,7 @@ template <int K>
 __device__ __forceinline__ double sweep_res2(const PhCtl &C, const PhInst &I, const double *A,
                                              int j) {
   if (j == 0) return C.prev_black + A[0];
-  const double wb = ph_omega_at(I, 2 * (uint64_t)C.sweeps_done + 2 * j - 1);  // black half of S + j
+  const double wb = ph_omega_at(I, C.ramp_from, 2 * (uint64_t)C.sweeps_done + 2 * j - 1);  // black half of S + j
   return (1.0 - wb) * (1.0 - wb) * A[2 * j - 1] + A[2 * j];
 }
 
@@ -129,7 +129,7 @@ __device__ void control(PhCtl &C, const PhInst &I, const double *A, const double
   bool stop = false;
   if (S >= 1 && C.sweeps_final < S) stop = finalize_sweep(C, I, S, R2[0], RL[0]);
   for (int j = 1; j < nsw && !stop; ++j) stop = finalize_sweep(C, I, S + j, R2[j], RL[j]);
-  const double wl = ph_omega_at(I, h0 + 2 * nsw - 1);
+  const double wl = ph_omega_at(I, C.ramp_from, h0 + 2 * nsw - 1);
   C.prev_black = (1.0 - wl) * (1.0 - wl) * A[2 * nsw - 1];
   if (stop) {
     C.halt_base = S;
@@ -303,7 +303,7 @@ __device__ __forceinline__ void sor_tile(const PhTile &T, const PhInst &I, const
   {
     const uint64_t h0 = 2 * (uint64_t)C.sweeps_done;
 #pragma unroll
-    for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(ph_omega_at(I, h0 + s) / I.dg64) : 0.0f;
+    for (int s = 0; s < K; ++s) cs[s] = s < k_run ? (float)(ph_omega_at(I, C.ramp_from, h0 + s) / I.dg64) : 0.0f;
   }
   // columns of the valid core, as 0/1 weights of the residual sums
   float cw[CPL];

@@ -445,19 +445,75 @@ bool warm_net_ready() { return !g_warm_params.empty(); }
 int warm_net_res() { return WARM_RES; }
 int warm_net_batch() { return WARM_BATCH; }
 
+// The solver's forward passes as hipGraphs: its ~20 small launches (about
+// 10 us each at the solver's batches) replayed from one graph launch. A key
+// (input, output, batch) runs directly the first time and is captured the
+// second (a batch of many images sees many keys once); the graphs hold kernel
+// nodes only, captured on a private default-priority stream. They refer to
+// the network's buffers, so they go when the network does.
+// PAINTOP_HIP_GRAPH=0 runs every pass directly.
+struct NetGraph {
+  const float *in;
+  float *out;
+  int batch;
+  hipGraphExec_t exec;
+};
+static std::vector<NetGraph> g_net_graphs;
+static hipStream_t g_net_cs = nullptr;
+static void net_graphs_clear() {
+  if (!g_net_graphs.empty()) (void)hipDeviceSynchronize();
+  for (NetGraph &g : g_net_graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g_net_graphs.clear();
+}
+
 // Inference for the solver: d_in [batch][res][res][4] -> d_out [batch][res][res].
 void warm_net_forward(const float *d_in, float *d_out, int batch, hipStream_t st) {
   if (g_warm_params.empty())
     throw fail(PH_CLASS_POLICY, "E_HIP_WARM_START_UNSET",
                "warm start requested but no network weights are set (ph_warm_start_set_weights)");
   if (!g_warm) {
+    net_graphs_clear();
     auto n = std::make_unique<UnNet>();
     n->create(WARM_BATCH, WARM_RES, g_warm_params.data(), 0);
     g_warm = std::move(n);
   }
-  g_warm->forward(d_in, g_warm->out, batch, st);
-  HIP_OK(hipMemcpyAsync(d_out, g_warm->out, (size_t)batch * WARM_RES * WARM_RES * sizeof(float),
-                        hipMemcpyDeviceToDevice, st));
+  const char *ge = getenv("PAINTOP_HIP_GRAPH");
+  if (ge && ge[0] == '0') {
+    g_warm->forward(d_in, d_out, batch, st);
+    return;
+  }
+  NetGraph *e = nullptr;
+  for (NetGraph &g : g_net_graphs)
+    if (g.in == d_in && g.out == d_out && g.batch == batch) e = &g;
+  if (!e) {
+    if (g_net_graphs.size() >= 32) {
+      HIP_OK(hipStreamSynchronize(st));
+      net_graphs_clear();
+    }
+    g_net_graphs.push_back(NetGraph{d_in, d_out, batch, nullptr});
+    g_warm->forward(d_in, d_out, batch, st);
+    return;
+  }
+  if (!e->exec) {
+    if (!g_net_cs) HIP_OK(hipStreamCreateWithFlags(&g_net_cs, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(g_net_cs, hipStreamCaptureModeRelaxed));
+    try {
+      g_warm->forward(d_in, d_out, batch, g_net_cs);
+    } catch (...) {
+      (void)hipStreamEndCapture(g_net_cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_OK(hipStreamEndCapture(g_net_cs, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t r = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_OK(r);
+    e->exec = ex;
+  }
+  HIP_OK(hipGraphLaunch(e->exec, st));
 }
 
 }  // namespace ph
@@ -599,6 +655,7 @@ extern "C" ph_status ph_warm_start_set_weights(const float *params, uint64_t n_p
   clear_error(err);
   try {
     if (params == nullptr || n_params == 0) {  // clear
+      net_graphs_clear();
       g_warm.reset();
       g_warm_params.clear();
       return PH_OK;
@@ -613,6 +670,7 @@ extern "C" ph_status ph_warm_start_set_weights(const float *params, uint64_t n_p
         throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "warm-start weights: value %llu not finite",
                    (unsigned long long)i);
     g_warm_params.assign(params, params + n_params);
+    net_graphs_clear();
     g_warm.reset();
     return PH_OK;
   } catch (const Fail &f) {

@@ -200,20 +200,35 @@ __global__ __launch_bounds__(64) void k_coarsen_groups(const PhInst *__restrict_
 }
 
 // Net input of a coarse correction, from an instance's plane: per block the
-// sum of g (the correction equation's rhs, Laplacian(e) = g) over interior
-// cells, zero boundary values, interior fraction and ring density, exactly as
-// k_coarsen lays them out.
+// sum of the correction problem's rhs over interior cells, zero boundary
+// values, interior fraction and ring density, exactly as k_coarsen lays them
+// out. The state is x + e (x FP64 with g = -r(x) stored FP32, e the FP32
+// correction in the buffer given); its residual is r = L e - g with
+// L e = nsum(e) - (4 + lambda) e, and the correction problem for the error
+// of x + e is L d = -r. Without e (WE false, or e = 0 after a refinement)
+// the rhs is g itself, bit for bit. With WE every cell's r is the sweeps'
+// FP32 residual (the per-sweep residual estimates use the same formula), and
+// rsq receives the sum of r^2 of each coarse cell (their total is
+// ||r(x + e)||^2: the grid covers every interior cell). Sums run in FP32 down
+// a column's f rows and in FP64 from there on.
 // Block (bx, cy) takes coarse cells cx0 = bx*cpb .. cx0+cpb-1 of coarse row cy.
 // Each thread sums whole plane columns over the row's f plane rows (in row
 // order; adjacent threads read adjacent columns), the column sums go to LDS,
 // and one thread per coarse cell adds its f columns in column order, so the
 // result is deterministic. Columns go through LDS in chunks of
 // CP_MAX_COLS; a coarse cell wider than that (cpb = 1) adds chunk by chunk.
-constexpr int CP_MAX_COLS = 2048;
-__device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int y0, int f,
-                                                    int res, int cpb, float *__restrict__ in4,
-                                                    int bx, int cy) {
-  __shared__ double s_rhs[CP_MAX_COLS];
+// (a block's columns, ncc * f, stay within 1024 for f <= 1024: see
+// coarsen_plane_cpb; 16 KB of LDS a block, so six blocks fit a CU)
+constexpr int CP_MAX_COLS = 1024;
+// plane rows a thread takes per batch of loads
+constexpr int CP_ROWS = 6;
+template <bool WE>
+__device__ __forceinline__ void coarsen_plane_block(const PhInst &I, const float *__restrict__ e,
+                                                    int x0, int y0, int f, int res, int cpb,
+                                                    float *__restrict__ in4,
+                                                    double *__restrict__ rsq, int bx, int cy) {
+  __shared__ float s_rhs[CP_MAX_COLS];
+  __shared__ float s_sq[WE ? CP_MAX_COLS : 1];
   __shared__ int s_int[CP_MAX_COLS], s_ring[CP_MAX_COLS];
   const int cx0 = bx * cpb;
   const int ncc = min(cpb, res - cx0);
@@ -221,7 +236,8 @@ __device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int
   const int pxb = x0 + cx0 * f - I.ox;  // plane column of the block's column 0
   const int pyb = y0 + cy * f - I.oy;
   const int ya = max(pyb, 1), yb = min(pyb + f, I.ph - 1);
-  double wr = 0.0, wi = 0.0, wn = 0.0;  // running sums of a wide cell (thread 0)
+  const float dgf = I.dg;
+  double wr = 0.0, wi = 0.0, wn = 0.0, wq = 0.0;  // running sums of a wide cell (thread 0)
   for (int j0 = 0; j0 < ncols; j0 += CP_MAX_COLS) {
     const int nj = min(CP_MAX_COLS, ncols - j0);
     // plane columns [c0, c1) of this chunk, inside the plane's non-padding
@@ -230,42 +246,139 @@ __device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int
     // ring test, the bytes left and right of the group its edges
     const int c0 = max(pxb + j0, 1), c1 = min(pxb + j0 + nj, I.pw - 1);
     for (int j = threadIdx.x; j < nj; j += 256) {
-      s_rhs[j] = 0.0;
+      s_rhs[j] = 0.0f;
+      if (WE) s_sq[j] = 0.0f;
       s_int[j] = 0;
       s_ring[j] = 0;
     }
     __syncthreads();
     const int ga = c0 & ~3;
-    for (int gx = ga + 4 * (int)threadIdx.x; gx < c1; gx += 4 * 256) {
-      double sr[4] = {0.0, 0.0, 0.0, 0.0};
+    const size_t pitch = (size_t)I.pitch;
+    const int lane = threadIdx.x & 63;
+    // A wave's 64 groups walk the rows together (lanes past c1 idle along,
+    // reading a fixed cell of row 1), so the values left and right of a group
+    // come from the lanes beside it; the wave's edge lanes and the group
+    // before c1 load them. Rows go in batches of CP_ROWS: the batch's g and e
+    // loads (a row of the group without interior cells is not read: e is 0
+    // there) are issued together with the next batch's classification, so a
+    // batch waits for one round of loads, not one per row.
+    for (int gb = ga + 4 * (int)(threadIdx.x & ~63); gb < c1; gb += 4 * 256) {
+      const int gx0 = gb + 4 * lane;
+      const bool act = gx0 < c1;
+      const int gx = act ? gx0 : ga;
+      const bool ld_l = lane == 0, ld_r = lane == 63 || gx0 + 4 >= c1;
+      const size_t dummy = pitch + gx;  // row 1: in the plane, in cache
+      // column sums over the coarse row's f plane rows, in row order (FP32:
+      // at most f terms of one cell's size)
+      float sr[4] = {0.f, 0.f, 0.f, 0.f}, sq[4] = {0.f, 0.f, 0.f, 0.f};
       int ni[4] = {0, 0, 0, 0}, nr[4] = {0, 0, 0, 0};
-      for (int py = ya; py < yb; ++py) {
-        const size_t p = (size_t)py * I.pitch + gx;
-        const uchar4 cc = *reinterpret_cast<const uchar4 *>(I.cls + p);
-        const uchar4 cu = *reinterpret_cast<const uchar4 *>(I.cls + p - I.pitch);
-        const uchar4 cd = *reinterpret_cast<const uchar4 *>(I.cls + p + I.pitch);
-        const unsigned char cl = I.cls[p - 1], cr = I.cls[p + 4];
-        const unsigned char c[6] = {cl, cc.x, cc.y, cc.z, cc.w, cr};
-        const unsigned char u[4] = {cu.x, cu.y, cu.z, cu.w}, d[4] = {cd.x, cd.y, cd.z, cd.w};
-        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (cc.x | cc.y | cc.z | cc.w) gv = *reinterpret_cast<const float4 *>(I.g + p);
-        const float gg[4] = {gv.x, gv.y, gv.z, gv.w};
+      // classification of rows py0 - 1 .. py0 + CP_ROWS of the batch at py0
+      // (rows clamped into the plane; a clamped row is never used)
+      unsigned cb[CP_ROWS + 2];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (c[k + 1]) {
-            ni[k] += 1;
-            sr[k] += (double)gg[k];
-          } else if (c[k] | c[k + 2] | u[k] | d[k]) {
-            nr[k] += 1;
+      for (int r = 0; r < CP_ROWS + 2; ++r)
+        cb[r] = *reinterpret_cast<const unsigned *>(
+            I.cls + (size_t)min(max(ya - 1 + r, 0), I.ph - 1) * pitch + gx);
+      for (int py0 = ya; py0 < yb; py0 += CP_ROWS) {
+        // the batch's g (rows py0 ..), e (rows py0 - 1 ..) and edges
+        float4 gb4[CP_ROWS], eb4[CP_ROWS + 2];
+        float elb[CP_ROWS], erb[CP_ROWS];
+        unsigned char clb[CP_ROWS], crb[CP_ROWS];
+#pragma unroll
+        for (int r = 0; r < CP_ROWS + 2; ++r) eb4[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < CP_ROWS; ++r) elb[r] = erb[r] = 0.f;
+#pragma unroll
+        for (int r = 0; r < CP_ROWS; ++r) {
+          const int py = py0 + r;
+          const size_t p = (size_t)min(py, I.ph - 2) * pitch + gx;
+          const bool any = act && py < yb && cb[r + 1] != 0;
+          gb4[r] = *reinterpret_cast<const float4 *>(I.g + (any ? p : dummy));
+          clb[r] = ld_l ? I.cls[act ? p - 1 : dummy] : 0;
+          crb[r] = ld_r ? I.cls[act ? p + 4 : dummy] : 0;
+          if (WE) {
+            elb[r] = ld_l ? e[any ? p - 1 : dummy] : 0.f;
+            erb[r] = ld_r ? e[any ? p + 4 : dummy] : 0.f;
           }
         }
+        if (WE) {
+#pragma unroll
+          for (int r = 0; r < CP_ROWS + 2; ++r) {
+            // e is 0 off the interior: a row of the group without interior
+            // cells reads as 0
+            const size_t pr = (size_t)min(max(py0 - 1 + r, 0), I.ph - 1) * pitch + gx;
+            const float4 v = *reinterpret_cast<const float4 *>(e + (cb[r] ? pr : dummy));
+            const float z = cb[r] ? 1.0f : 0.0f;
+            eb4[r] = make_float4(v.x * z, v.y * z, v.z * z, v.w * z);
+          }
+        }
+        // the next batch's classification (its first two rows are this
+        // batch's last two)
+        unsigned cn[CP_ROWS + 2];
+        cn[0] = cb[CP_ROWS];
+        cn[1] = cb[CP_ROWS + 1];
+#pragma unroll
+        for (int r = 2; r < CP_ROWS + 2; ++r)
+          cn[r] = *reinterpret_cast<const unsigned *>(
+              I.cls + (size_t)min(py0 + CP_ROWS - 1 + r, I.ph - 1) * pitch + gx);
+#pragma unroll
+        for (int r = 0; r < CP_ROWS; ++r) {
+          if (py0 + r >= yb) continue;  // uniform across the block
+          const unsigned cu = cb[r], cc = cb[r + 1], cd = cb[r + 2];
+          const float4 eu4 = eb4[r], ec4 = eb4[r + 1], ed4 = eb4[r + 2];
+          const unsigned cl_n =
+              (unsigned)__builtin_amdgcn_update_dpp(0, (int)cc, 0x138, 0xf, 0xf, false);
+          const unsigned cr_n =
+              (unsigned)__builtin_amdgcn_update_dpp(0, (int)cc, 0x130, 0xf, 0xf, false);
+          const unsigned char cl = ld_l ? clb[r] : (unsigned char)(cl_n >> 24);
+          const unsigned char cr = ld_r ? crb[r] : (unsigned char)(cr_n & 0xff);
+          float el = 0.f, erx = 0.f;
+          if (WE) {
+            const float el_n = __builtin_bit_cast(
+                float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, ec4.w), 0x138, 0xf,
+                                                   0xf, false));
+            const float er_n = __builtin_bit_cast(
+                float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, ec4.x), 0x130, 0xf,
+                                                   0xf, false));
+            el = ld_l ? elb[r] : el_n;
+            erx = ld_r ? erb[r] : er_n;
+          }
+          const uchar4 c4 = __builtin_bit_cast(uchar4, cc), u4 = __builtin_bit_cast(uchar4, cu),
+                       d4 = __builtin_bit_cast(uchar4, cd);
+          const unsigned char c[6] = {cl, c4.x, c4.y, c4.z, c4.w, cr};
+          const unsigned char u[4] = {u4.x, u4.y, u4.z, u4.w}, d[4] = {d4.x, d4.y, d4.z, d4.w};
+          const float gg[4] = {gb4[r].x, gb4[r].y, gb4[r].z, gb4[r].w};
+          const float er[6] = {el, ec4.x, ec4.y, ec4.z, ec4.w, erx};
+          const float eu[4] = {eu4.x, eu4.y, eu4.z, eu4.w}, ed[4] = {ed4.x, ed4.y, ed4.z, ed4.w};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (c[k + 1]) {
+              ni[k] += 1;
+              if (WE) {
+                // the sweeps' own FP32 residual of the cell (r = -g exactly for e = 0)
+                const float rr =
+                    fmaf(-dgf, er[k + 1], (((er[k] + er[k + 2]) + eu[k]) + ed[k]) - gg[k]);
+                sr[k] -= rr;
+                sq[k] = fmaf(rr, rr, sq[k]);
+              } else {
+                sr[k] += gg[k];
+              }
+            } else if (c[k] | c[k + 2] | u[k] | d[k]) {
+              nr[k] += 1;
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < CP_ROWS + 2; ++r) cb[r] = cn[r];
       }
+      if (!act) continue;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int px = gx + k;
         if (px < c0 || px >= c1) continue;
         const int j = px - pxb - j0;
         s_rhs[j] = sr[k];
+        if (WE) s_sq[j] = sq[k];
         s_int[j] = ni[k];
         s_ring[j] = nr[k];
       }
@@ -275,14 +388,16 @@ __device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int
       if (threadIdx.x == 0)
         for (int j = 0; j < nj; ++j) {
           wr += s_rhs[j];
+          if (WE) wq += s_sq[j];
           wi += s_int[j];
           wn += s_ring[j];
         }
     } else if ((int)threadIdx.x < ncc) {
       const int c = threadIdx.x;
-      double sr = 0.0, ni = 0.0, nr = 0.0;
+      double sr = 0.0, sq = 0.0, ni = 0.0, nr = 0.0;
       for (int j = c * f; j < c * f + f; ++j) {
         sr += s_rhs[j];
+        if (WE) sq += s_sq[j];
         ni += s_int[j];
         nr += s_ring[j];
       }
@@ -291,6 +406,7 @@ __device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int
       o[1] = 0.0f;
       o[2] = (float)(ni / ((double)f * f));
       o[3] = (float)(nr / f);
+      if (WE) rsq[(size_t)cy * res + cx0 + c] = sq;
     }
     __syncthreads();
   }
@@ -300,24 +416,42 @@ __device__ __forceinline__ void coarsen_plane_block(const PhInst &I, int x0, int
     o[1] = 0.0f;
     o[2] = (float)(wi / ((double)f * f));
     o[3] = (float)(wn / f);
+    if (WE) rsq[(size_t)cy * res + cx0] = wq;
   }
 }
 
+// The input from g alone (the plane right after a refinement, e = 0).
 __global__ __launch_bounds__(256) void k_coarsen_plane(PhInst I, int x0, int y0, int f, int res,
                                                        int cpb, float *__restrict__ in4) {
-  coarsen_plane_block(I, x0, y0, f, res, cpb, in4, blockIdx.x, blockIdx.y);
+  coarsen_plane_block<false>(I, nullptr, x0, y0, f, res, cpb, in4, nullptr, blockIdx.x,
+                             blockIdx.y);
 }
 
-// k_coarsen_plane for every correction of a round in one launch: blockIdx.z =
-// entry j, whose input goes to in4 + res*res*4*j. gridDim.x covers the entry
-// with the most blocks per coarse row; blocks past an entry's row end exit.
-__global__ __launch_bounds__(256) void k_coarsen_plane_list(const PhInst *__restrict__ insts,
-                                                            const PhCorrArgs *__restrict__ args,
-                                                            int res, float *__restrict__ in4) {
+// The input from the state x + e of a plane (e in the given buffer), with the
+// per-coarse-cell sums of r(x + e)^2.
+__global__ __launch_bounds__(256) void k_corr_input(PhInst I, const float *__restrict__ e, int x0,
+                                                    int y0, int f, int res, int cpb,
+                                                    float *__restrict__ in4,
+                                                    double *__restrict__ rsq) {
+  coarsen_plane_block<true>(I, e, x0, y0, f, res, cpb, in4, rsq, blockIdx.x, blockIdx.y);
+}
+
+// k_corr_input for every correction of a round in one launch: blockIdx.z =
+// entry j, whose input goes to in4 + res*res*4*j and its squared-residual
+// cells to rsq + res*res*j; its state is in the correction buffer its control
+// block names. gridDim.x covers the entry with the most blocks per coarse row;
+// blocks past an entry's row end exit.
+__global__ __launch_bounds__(256) void k_corr_input_list(const PhInst *__restrict__ insts,
+                                                         const PhCtl *__restrict__ ctl,
+                                                         const PhCorrArgs *__restrict__ args,
+                                                         int res, float *__restrict__ in4,
+                                                         double *__restrict__ rsq) {
   const PhCorrArgs A = args[blockIdx.z];
   if ((int)blockIdx.x * A.cpb >= res) return;
-  coarsen_plane_block(insts[A.slot], A.x0, A.y0, A.f, res, A.cpb,
-                      in4 + (size_t)res * res * 4 * blockIdx.z, blockIdx.x, blockIdx.y);
+  const PhInst &I = insts[A.slot];
+  coarsen_plane_block<true>(I, I.e[ctl[A.slot].cur_buf], A.x0, A.y0, A.f, res, A.cpb,
+                            in4 + (size_t)res * res * 4 * blockIdx.z,
+                            rsq + (size_t)res * res * blockIdx.z, blockIdx.x, blockIdx.y);
 }
 
 // Coarse cells per block of k_coarsen_plane: about one 4-column group per
@@ -396,17 +530,21 @@ __device__ __forceinline__ double warm_bilinear(const float *hc, int res, int x0
 // clamped to [0, max_step] (0: no usable correction). Input and prediction
 // carry the same normalisation, so alpha is scale-free.
 // Two kernels over (NORM_SLICES slices, corrections): per-slice sums of
-// G H and (A_c H) H, then one lane per correction adds the slices in order.
+// G H and (A_c H) H (and, given rsq, of the input's squared-residual cells),
+// then one lane per correction adds the slices in order: alpha, and the norm
+// ||r|| of the state the input was taken from.
 __global__ __launch_bounds__(256) void k_corr_alpha_part(const float *__restrict__ in4_all,
                                                          const float *__restrict__ h_all,
+                                                         const double *__restrict__ rsq_all,
                                                          int res, double *__restrict__ part) {
   const int sl = blockIdx.x, b = blockIdx.y;
   const int n = res * res, per = (n + NORM_SLICES - 1) / NORM_SLICES;
   const int i0 = sl * per, i1 = min(n, i0 + per);
   const float *in4 = in4_all + (size_t)b * n * 4;
   const float *hc = h_all + (size_t)b * n;
-  double s1 = 0.0, s2 = 0.0;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (int c = i0 + threadIdx.x; c < i1; c += 256) {
+    if (rsq_all) s3 += rsq_all[(size_t)b * n + c];
     const int cx = c % res, cy = c / res;
     if (cx < 1 || cy < 1 || cx >= res - 1 || cy >= res - 1) continue;
     if (!(in4[(size_t)c * 4 + 2] >= 1.0f)) continue;
@@ -418,34 +556,41 @@ __global__ __launch_bounds__(256) void k_corr_alpha_part(const float *__restrict
     s1 = fma((double)in4[(size_t)c * 4], h, s1);
     s2 = fma(ah, h, s2);
   }
-  __shared__ double red[2][256];
+  __shared__ double red[3][256];
   red[0][threadIdx.x] = s1;
   red[1][threadIdx.x] = s2;
+  red[2][threadIdx.x] = s3;
   __syncthreads();
   for (int k = 128; k >= 1; k >>= 1) {
     if ((int)threadIdx.x < k) {
       red[0][threadIdx.x] += red[0][threadIdx.x + k];
       red[1][threadIdx.x] += red[1][threadIdx.x + k];
+      red[2][threadIdx.x] += red[2][threadIdx.x + k];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[((size_t)b * NORM_SLICES + sl) * 2] = red[0][0];
-    part[((size_t)b * NORM_SLICES + sl) * 2 + 1] = red[1][0];
+    double *o = part + ((size_t)b * NORM_SLICES + sl) * 3;
+    o[0] = red[0][0];
+    o[1] = red[1][0];
+    o[2] = red[2][0];
   }
 }
 
 __global__ void k_corr_alpha(const double *__restrict__ part, int n, double max_step,
-                             double *__restrict__ alpha) {
+                             double *__restrict__ alpha, double *__restrict__ rnorm) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
-  double s1 = 0.0, s2 = 0.0;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (int k = 0; k < NORM_SLICES; ++k) {
-    s1 += part[((size_t)b * NORM_SLICES + k) * 2];
-    s2 += part[((size_t)b * NORM_SLICES + k) * 2 + 1];
+    const double *o = part + ((size_t)b * NORM_SLICES + k) * 3;
+    s1 += o[0];
+    s2 += o[1];
+    s3 += o[2];
   }
   const double a = s1 / s2;
   alpha[b] = (s2 < 0.0 && isfinite(a)) ? fmin(fmax(a, 0.0), max_step) : 0.0;
+  if (rnorm) rnorm[b] = sqrt(s3);
 }
 
 // x = P + scale * bilinear(h) at the interior cells of every listed network
@@ -518,6 +663,46 @@ extern "C" ph_status ph_warm_coarsen_device(const float *d_source, const float *
                        d_anchor, d_mask, d_solution, (int)width, (int)height, (int)channels,
                        (int)channel, lambda, x0, y0, factor, res, (int)mode, d_in,
                        d_solution ? d_target : nullptr, d_solution ? d_weight : nullptr);
+    HIP_OK(hipGetLastError());
+    return PH_OK;
+  } catch (const Fail &f) {
+    put_error(err, f);
+    return f.code == "E_GPU_UNAVAILABLE" ? PH_UNAVAILABLE : PH_ERROR;
+  }
+}
+
+extern "C" ph_status ph_correction_input_device(const float *d_e, const float *d_g,
+                                                const uint8_t *d_cls, int32_t pitch, int32_t pw,
+                                                int32_t ph, int32_t ox, int32_t oy, int32_t x0,
+                                                int32_t y0, int32_t factor, int32_t res,
+                                                double lambda, float *d_in, double *d_rsq,
+                                                void *stream, ph_error *err) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  clear_error(err);
+  try {
+    ensure_device();
+    if (!d_e || !d_g || !d_cls || !d_in || !d_rsq)
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "null buffer");
+    if (pw < 3 || ph < 3 || pitch < pw || pitch % 4 != 0 || factor < 1 || res < 1 ||
+        res > 4096 || (uintptr_t)d_g % 16 != 0 || (uintptr_t)d_e % 16 != 0 ||
+        (uintptr_t)d_cls % 4 != 0 || !(lambda >= 0.0))
+      throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT",
+                 "correction_input: plane %dx%d pitch %d, factor %d, resolution %d (pitch a "
+                 "multiple of 4, e and g 16-byte and cls 4-byte aligned, lambda >= 0)",
+                 pw, ph, pitch, factor, res);
+    PhInst I{};
+    I.g = const_cast<float *>(d_g);
+    I.cls = d_cls;
+    I.pitch = pitch;
+    I.pw = pw;
+    I.ph = ph;
+    I.ox = ox;
+    I.oy = oy;
+    I.dg64 = 4.0 + lambda;
+    I.dg = (float)I.dg64;
+    const int cpb = coarsen_plane_cpb(factor);
+    hipLaunchKernelGGL(k_corr_input, dim3((res + cpb - 1) / cpb, res), dim3(256), 0,
+                       (hipStream_t)stream, I, d_e, x0, y0, factor, res, cpb, d_in, d_rsq);
     HIP_OK(hipGetLastError());
     return PH_OK;
   } catch (const Fail &f) {

@@ -49,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "ph_unet_gather_device",
     "ph_assert_no_change_outside_mask_device",
     "ph_coarsen_plane_device",
+    "ph_correction_input_device",
 )
 
 
@@ -221,6 +222,10 @@ def lib():
     L.ph_coarsen_plane_device.argtypes = ([vp, vp] + [ctypes.c_int32] * 9 +
                                           [vp, vp, ctypes.POINTER(PhError)])
     L.ph_coarsen_plane_device.restype = ctypes.c_int
+    L.ph_correction_input_device.argtypes = ([vp, vp, vp] + [ctypes.c_int32] * 9 +
+                                             [ctypes.c_double, vp, vp, vp,
+                                              ctypes.POINTER(PhError)])
+    L.ph_correction_input_device.restype = ctypes.c_int
     L.ph_gauss_noise_device.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, vp,
                                         ctypes.POINTER(PhError)]
     L.ph_gauss_noise_device.restype = ctypes.c_int
@@ -435,6 +440,18 @@ def coarsen_plane_device(d_g: int, d_cls: int, pitch: int, pw: int, ph: int, ox:
     err = PhError()
     st = lib().ph_coarsen_plane_device(d_g, d_cls, pitch, pw, ph, ox, oy, x0, y0, factor, res,
                                        d_in, stream or None, ctypes.byref(err))
+    _check(st, err)
+
+
+def correction_input_device(d_e: int, d_g: int, d_cls: int, pitch: int, pw: int, ph: int,
+                            ox: int, oy: int, x0: int, y0: int, factor: int, res: int,
+                            lam: float, d_in: int, d_rsq: int, stream: int = 0):
+    """The correction-round input from a plane's state x + e
+    (ph_correction_input_device)."""
+    err = PhError()
+    st = lib().ph_correction_input_device(d_e, d_g, d_cls, pitch, pw, ph, ox, oy, x0, y0, factor,
+                                          res, lam, d_in, d_rsq, stream or None,
+                                          ctypes.byref(err))
     _check(st, err)
 
 

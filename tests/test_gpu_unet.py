@@ -205,6 +205,61 @@ def test_coarsen_plane_matches_numpy(gpu, pw, ph, pitch, f, res, x0, y0):
             assert np.allclose(got[cy, cx], exp, rtol=1e-5, atol=1e-5), (cy, cx, got[cy, cx], exp)
 
 
+@pytest.mark.parametrize("pw,ph,pitch,f,res,x0,y0,lam", [
+    (90, 70, 128, 5, 16, -7, -3, 0.0),      # grid larger than the plane, negative offsets
+    (257, 200, 320, 30, 8, 3, 11, 0.0),     # 8 cells per block
+    (300, 60, 320, 300, 1, -20, -5, 0.25),  # f > 256: one coarse cell per block
+    (64, 64, 64, 1, 64, 0, 0, 0.0),         # f = 1, pitch == pw
+])
+def test_correction_input_matches_numpy(gpu, pw, ph, pitch, f, res, x0, y0, lam):
+    """The correction rounds' input from a plane's state x + e
+    (ph_correction_input_device) against numpy in float64: per coarse cell the
+    sum of -r = g - (nsum(e) - (4 + lambda) e) over interior cells and of r^2,
+    with e = 0 off the interior (the device forms r in FP32, as the sweeps do:
+    within FP32 rounding); and with e = 0 the input equals the g-only input
+    (ph_coarsen_plane_device) bit for bit."""
+    from paintop_amd.device import DeviceArray
+
+    rng = np.random.default_rng(pw * 37 + f)
+    ox, oy = 5, -2
+    cls = (rng.random((ph, pitch)) < 0.6).astype(np.uint8)
+    cls[0, :] = cls[-1, :] = 0
+    cls[:, 0] = 0
+    cls[:, pw - 1:] = 0
+    g = np.where(cls > 0, rng.standard_normal((ph, pitch)), np.nan).astype(np.float32)
+    e = np.where(cls > 0, rng.standard_normal((ph, pitch)), 0.0).astype(np.float32)
+    dg_, dc, de = DeviceArray.from_numpy(g), DeviceArray.from_numpy(cls), DeviceArray.from_numpy(e)
+    din = DeviceArray((res, res, 4), np.float32)
+    drsq = DeviceArray((res, res), np.float64)
+    gpu.correction_input_device(de.ptr, dg_.ptr, dc.ptr, pitch, pw, ph, ox, oy, x0, y0, f, res,
+                                lam, din.ptr, drsq.ptr)
+    got, rsq = din.numpy(), drsq.numpy()
+    e64 = e.astype(np.float64)
+    ep = np.pad(e64, 1)
+    nsum = ((ep[1:-1, :-2] + ep[1:-1, 2:]) + ep[:-2, 1:-1]) + ep[2:, 1:-1]
+    valid = np.zeros_like(cls, dtype=bool)
+    valid[1:ph - 1, 1:pw - 1] = True
+    inside = (cls > 0) & valid
+    r = np.where(inside, nsum - (4.0 + lam) * e64 - g.astype(np.float64), 0.0)
+    pad = np.pad(cls, 1)
+    near = (pad[:-2, 1:-1] | pad[2:, 1:-1] | pad[1:-1, :-2] | pad[1:-1, 2:]) > 0
+    ring = (cls == 0) & near & valid
+    for cy in range(res):
+        for cx in range(res):
+            ys = slice(max(0, y0 + cy * f - oy), max(0, min(ph, y0 + cy * f + f - oy)))
+            xs = slice(max(0, x0 + cx * f - ox), max(0, min(pitch, x0 + cx * f + f - ox)))
+            exp = [-r[ys, xs].sum(), 0.0, inside[ys, xs].sum() / f ** 2, ring[ys, xs].sum() / f]
+            assert np.allclose(got[cy, cx], exp, rtol=1e-5, atol=1e-4), (cy, cx, got[cy, cx], exp)
+            assert np.isclose(rsq[cy, cx], (r[ys, xs] ** 2).sum(), rtol=1e-4, atol=1e-5)
+    # e = 0: the state's input is the g-only input, bit for bit
+    dz = DeviceArray.from_numpy(np.zeros_like(e))
+    din0 = DeviceArray((res, res, 4), np.float32)
+    gpu.correction_input_device(dz.ptr, dg_.ptr, dc.ptr, pitch, pw, ph, ox, oy, x0, y0, f, res,
+                                lam, din.ptr, drsq.ptr)
+    gpu.coarsen_plane_device(dg_.ptr, dc.ptr, pitch, pw, ph, ox, oy, x0, y0, f, res, din0.ptr)
+    assert din.numpy().tobytes() == din0.numpy().tobytes()
+
+
 def test_bad_weights_fall_back_to_the_plain_start(gpu, monkeypatch):
     """Injected bad weights: the network's start is worse than the anchor
     start, so the solver keeps the plain start and the result is bit-identical
