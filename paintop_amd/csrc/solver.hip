@@ -38,6 +38,7 @@
 #include "../../include/paintop_hip.h"
 #include "host_common.h"
 #include "ph_internal.h"
+#include "sor_tuning.h"
 
 namespace ph {
 
@@ -826,22 +827,32 @@ struct ScatterL {
   }
 };
 
+// Half-sweeps per launch and tile rows of a call: the caller's, or the tuning
+// table's row for the call's work (sum over images of channels x interior
+// bounding-box area). One wave per tile streams its hb core rows and 2K halo
+// rows per launch in periods of U = K + PF + 1 row steps; two waves fit a
+// SIMD (2048 on the chip), so a launch takes about ceil(live tiles / 2048)
+// rounds of one tile pass: large calls want tall tiles and K = 16 (fewer HBM
+// round trips and halo rows per sweep), small ones short tiles and K = 8 (a
+// lone tile's pass is serial, and shorter with fewer half-sweeps).
+// (One table serves plain calls and calls with the learned start: a learned
+// start that loses its race leaves the plain call's result bit for bit only
+// under the same launch boundaries.)
+static void sweep_config(const ph_solve_desc *d, double work, int &K, int &hb) {
+  const PhSorTune *row = &PH_SOR_TUNING[0];
+  while (work > row->w_max) ++row;
+  K = d->half_sweeps ? (int)d->half_sweeps : row->k;
+  if (d->tile_rows) hb = (int)d->tile_rows;
+  else if (K == row->k) hb = row->hb;
+  else {
+    // a caller's K without tile rows: the ring period U, twice it for large calls
+    const int U = K + prefetch_rows(K) + 1;
+    hb = (work >= 3.0e7 ? 2 : 1) * ((16 + U - 1) / U) * U;
+  }
+}
+
 static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipStream_t st) {
   const int nitem = (int)items.size();
-  const int K = d->half_sweeps ? (int)d->half_sweeps : 16;
-  const int WV = 256 - 2 * K;
-  const int PF = prefetch_rows(K);
-  // Tile height: one wave per tile streams its hb core rows and 2K halo rows
-  // per launch in periods of U = K + PF + 1 row steps; with hb a multiple of
-  // U the last period ends on the last row (otherwise it runs past it). Two
-  // waves fit a SIMD (2048 on the chip), so a launch takes about
-  // ceil(live tiles / 2048) rounds of one tile pass, and fewer, taller tiles
-  // hide latency worse: 2U rows for large solves, U for small ones.
-  const int U = K + PF + 1;
-  uint64_t work_px = 0;
-  for (const ItemIn &it : items) work_px += (uint64_t)it.W * it.H * it.C;
-  const int hb = d->tile_rows ? (int)d->tile_rows
-                               : (work_px >= (32ull << 20) ? 2 : 1) * ((16 + U - 1) / U) * U;
   const bool timing = (d->flags & PH_FLAG_TIMING) != 0;
   const bool tuned = d->omega <= 0.0;
   const bool want_warm =
@@ -858,11 +869,22 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     HIP_OK(hipEventRecord(ev0, st));
   }
   {
+    // (the sweep configuration is chosen after the classification: the
+    // workspace covers every configuration the call may choose)
+    auto item_need_any = [&](const ItemIn &it) {
+      size_t m = 0;
+      for (const PhSorTune &r : PH_SOR_TUNING) {
+        int k = 0, h = 0;
+        sweep_config(d, r.w_max, k, h);
+        m = std::max(m, item_need(d, it.W, it.H, it.C, k, h));
+      }
+      return m;
+    };
     size_t need = 128 * 1024 + (size_t)nitem * 64;
-    for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
+    for (const ItemIn &it : items) need += item_need_any(it);
     // the plain twins of the warm-start race
     if (race_any)
-      for (const ItemIn &it : items) need += item_need(d, it.W, it.H, it.C, K, hb);
+      for (const ItemIn &it : items) need += item_need_any(it);
     if (want_warm) {
       const size_t R2 = (size_t)warm_net_res() * warm_net_res();
       size_t nch = 0;
@@ -905,7 +927,6 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     rep->reserved_ = 0;
     rep->sweep_ms = 0.0;
     rep->device_ms = 0.0;
-    rep->half_sweeps_used = (uint32_t)K;
     rep->omega_used = d->omega;
   }
 
@@ -947,6 +968,17 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   HIP_OK(hipStreamSynchronize(st));
   for (int i = 0; i < nitem; ++i)
     if ((size_t)iw[i].in.W * iw[i].in.H == 0) bbox_h[4 * i + 2] = -1;
+  // the sweep configuration for the call's work
+  double work = 0.0;
+  for (int i = 0; i < nitem; ++i)
+    if (bbox_h[4 * i + 2] >= 0)
+      work += (double)iw[i].in.C * (bbox_h[4 * i + 2] - bbox_h[4 * i] + 1) *
+              (bbox_h[4 * i + 3] - bbox_h[4 * i + 1] + 1);
+  int K = 16, hb = 40;
+  sweep_config(d, work, K, hb);
+  const int WV = 256 - 2 * K;
+  const int PF = prefetch_rows(K);
+  for (int i = 0; i < nitem; ++i) iw[i].in.rep->half_sweeps_used = (uint32_t)K;
 
   phase("classify");
   // --- 2. planes, tile occupancy, coarse masks for the tuner (one readback)

@@ -222,3 +222,50 @@ def test_integration_rust_binding_matches_the_header():
     for c, r in (("ph_solve_desc", "PhSolveDesc"), ("ph_report", "PhReport"),
                  ("ph_batch_item", "PhBatchItem")):
         assert c_fields(c) == rust_fields(r), (c, c_fields(c), rust_fields(r))
+
+
+def test_tuner_rows_ignore_timing_noise():
+    """The sweep-configuration tuner's table (paintop_amd/tune.py): a
+    configuration within the slack of a problem's fastest counts as equal, so
+    a near-tie does not split a row; rows end at the geometric mean of the
+    neighbouring problems' work; the last row is unbounded."""
+    from paintop_amd import tune
+
+    n = len(tune.CONFIGS)
+    works = [1e5, 4e5, 1.6e6, 6.4e6, 2.6e7]
+
+    def t(best, near=()):
+        ts = [10.0] * n
+        ts[best] = 5.0
+        for c in near:
+            ts[c] = 5.1  # within the slack
+        return ts
+
+    # config 4 is fastest or within 3 % of it up to 6.4e6; config 0 above
+    times = [t(3, near=(4,)), t(4), t(4, near=(5,)), t(5, near=(4,)), t(0)]
+    rows = tune.table_rows(works, times)
+    assert [(k, hb) for _, k, hb in rows] == [tune.CONFIGS[4], tune.CONFIGS[0]]
+    assert rows[0][0] == pytest.approx((6.4e6 * 2.6e7) ** 0.5)
+    assert rows[-1][0] == 1e300
+    # input order does not matter
+    perm = [3, 0, 4, 2, 1]
+    assert tune.table_rows([works[i] for i in perm], [times[i] for i in perm]) == rows
+    # a real split stays
+    rows2 = tune.table_rows(works[:2], [t(3), t(5)])
+    assert [(k, hb) for _, k, hb in rows2] == [tune.CONFIGS[3], tune.CONFIGS[5]]
+
+
+def test_tuning_header_matches_the_tuner_format():
+    """The committed table (paintop_amd/csrc/sor_tuning.h) has its rows in
+    increasing work bounds, the last unbounded, with configurations the
+    sweep kernel is built for (K in 8, 12, 16; even tile rows >= 16)."""
+    import re
+
+    text = open(os.path.join(ROOT, "paintop_amd", "csrc", "sor_tuning.h")).read()
+    for name in ("PH_SOR_TUNING",):
+        body = re.search(name + r"\[\] = \{(.*?)\};", text, re.S).group(1)
+        rows = [(float(a), int(b), int(c)) for a, b, c in
+                re.findall(r"\{([0-9.e+]+), (\d+), (\d+)\}", body)]
+        assert rows and rows[-1][0] >= 1e299
+        assert all(rows[i][0] < rows[i + 1][0] for i in range(len(rows) - 1))
+        assert all(k in (8, 12, 16) and hb % 2 == 0 and 16 <= hb <= 1024 for _, k, hb in rows)
