@@ -155,6 +155,15 @@ ph_status ph_probe(ph_device_info *out, ph_error *err);
 /* Force the probe (and every solve) to report PH_UNAVAILABLE. For tests. */
 void ph_force_unavailable(int32_t forced);
 
+/* The sweep configuration (half-sweeps per launch, tile rows) a call with
+ * half_sweeps = tile_rows = 0 takes for its work: the sum over its images of
+ * channels x the area of the interior's bounding box. No counterpart in the
+ * reference (its solver has no launch configuration); an executor that
+ * gathers several nodes into one batched call passes each group's own
+ * configuration, so every image is solved as it would be alone. Host-only:
+ * no device needed. */
+void ph_sweep_config(double work, uint32_t *half_sweeps, uint32_t *tile_rows);
+
 /* One op invocation on host buffers in the ResourceValue layout
  * (paintop-core/src/executor/value.rs:35-85): row-major, channel-interleaved
  * f32, source/target (guidance/anchor) width*height*channels, mask

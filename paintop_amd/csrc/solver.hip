@@ -2315,6 +2315,14 @@ extern "C" void ph_force_unavailable(int32_t forced) {
   g_forced = forced ? 1 : 0;
 }
 
+extern "C" void ph_sweep_config(double work, uint32_t *half_sweeps, uint32_t *tile_rows) {
+  ph_solve_desc d{};
+  int K = 0, hb = 0;
+  sweep_config(&d, work, K, hb);
+  if (half_sweeps) *half_sweeps = (uint32_t)K;
+  if (tile_rows) *tile_rows = (uint32_t)hb;
+}
+
 extern "C" ph_status ph_probe(ph_device_info *out, ph_error *err) {
   std::lock_guard<std::mutex> lk(g_mu);
   clear_error(err);

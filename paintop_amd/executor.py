@@ -175,6 +175,32 @@ def dispatch_node(node_id: str, op_id: str, manifest: dict, impls: ImplRegistry,
     return outputs
 
 
+def dispatch_batch(node_ids: list, op_id: str, manifest: dict, impl, sel: BackendSelection,
+                   calls: list, trace: list) -> list:
+    """Several nodes of one op, selected to the same implementation, in one
+    batched call (the hip backend's ``compute_batch``): the per-node trace
+    events of ``dispatch_node`` plus one ``dispatch_batched`` event naming the
+    nodes; every node's outputs are checked against the manifest."""
+    for nid in node_ids:
+        trace.append({"event": "implementation_selected", "node": nid,
+                      "implementation": str(sel.impl_id), "fell_back": sel.fell_back})
+        trace.append({"event": "dispatch_started", "node": nid})
+        trace.append({"event": "cache_lookup", "node": nid, "outcome": "bypass"})
+    trace.append({"event": "dispatch_batched", "nodes": list(node_ids),
+                  "implementation": str(sel.impl_id)})
+    t0 = time.perf_counter()
+    outputs = impl.compute_batch(calls)
+    elapsed = (time.perf_counter() - t0) * 1e3
+    for nid, out in zip(node_ids, outputs):
+        for port in [o["name"] for o in manifest.get("outputs", [])]:
+            if port not in out:
+                raise PaintopError(ErrorClass.EXECUTION, E_OUTPUT_NOT_PRODUCED,
+                                   f"node {nid} did not produce declared output `{port}`")
+        trace.append({"event": "dispatch_completed", "node": nid, "elapsed_ms": elapsed,
+                      "batched": len(node_ids)})
+    return outputs
+
+
 def register_hip_backend(impls: ImplRegistry, **tuning) -> None:
     """Register the hip implementations of both Poisson ops under ``hip.poisson@1``
     (the line domains/repair.rs:23-34 would add next to the reference ones)."""

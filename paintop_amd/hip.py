@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "ph_assert_no_change_outside_mask_device",
     "ph_coarsen_plane_device",
     "ph_correction_input_device",
+    "ph_sweep_config",
 )
 
 
@@ -226,6 +227,9 @@ def lib():
                                              [ctypes.c_double, vp, vp, vp,
                                               ctypes.POINTER(PhError)])
     L.ph_correction_input_device.restype = ctypes.c_int
+    L.ph_sweep_config.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_uint32),
+                                  ctypes.POINTER(ctypes.c_uint32)]
+    L.ph_sweep_config.restype = None
     L.ph_gauss_noise_device.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, vp,
                                         ctypes.POINTER(PhError)]
     L.ph_gauss_noise_device.restype = ctypes.c_int
@@ -441,6 +445,15 @@ def coarsen_plane_device(d_g: int, d_cls: int, pitch: int, pw: int, ph: int, ox:
     st = lib().ph_coarsen_plane_device(d_g, d_cls, pitch, pw, ph, ox, oy, x0, y0, factor, res,
                                        d_in, stream or None, ctypes.byref(err))
     _check(st, err)
+
+
+def sweep_config(work: float) -> tuple[int, int]:
+    """(half-sweeps per launch, tile rows) a call of this work takes by default
+    (ph_sweep_config): work = sum over its images of channels x the interior's
+    bounding-box area."""
+    k, hb = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    lib().ph_sweep_config(float(work), ctypes.byref(k), ctypes.byref(hb))
+    return int(k.value), int(hb.value)
 
 
 def correction_input_device(d_e: int, d_g: int, d_cls: int, pitch: int, pw: int, ph: int,

@@ -140,6 +140,19 @@ def check_shapes(source: ImageDescriptor, target: ImageDescriptor, mask_extent, 
     return target
 
 
+def interior_bbox_area(mask: np.ndarray) -> int:
+    """Area of the bounding box of the interior cells (mask > 0.5 and not on
+    the image border, poisson.rs:329-355): the work measure of the solver's
+    sweep configuration, per channel."""
+    h, w = mask.shape
+    inside = np.asarray(mask[1:h - 1, 1:w - 1]) > 0.5
+    ys = np.flatnonzero(inside.any(axis=1))
+    xs = np.flatnonzero(inside.any(axis=0))
+    if len(ys) == 0:
+        return 0
+    return int((ys[-1] - ys[0] + 1) * (xs[-1] - xs[0] + 1))
+
+
 def solver_data(kind: str, r: hip.SolveResult) -> SolverData:
     """poisson_blend.rs:249-266."""
     return SolverData(kind=kind, steps=r.iterations, stability_number=0.0, stability_limit=0.0,
@@ -194,8 +207,8 @@ class _HipPoissonOp:
                                f"{self.op_name} requires a `{port}` input value")
         return inputs[port]
 
-    def compute(self, inputs: dict, params: dict) -> dict:
-        """OpImplementation::compute (op_impl.rs:51-70) for this op."""
+    def _prepare(self, inputs: dict, params: dict):
+        """Resolved controls, the three input values and the output descriptor."""
         max_it, tol, lam = self._resolve(params)
         a, b, m = (self._input(inputs, p) for p in self.ports)
         if not isinstance(a.descriptor, ImageDescriptor):
@@ -208,6 +221,73 @@ class _HipPoissonOp:
             raise PaintopError(ErrorClass.TYPE, self.input_code,
                                f"{self.op_name} `mask` input must be a mask resource")
         out_desc = check_shapes(a.descriptor, b.descriptor, m.descriptor.extent, self.op_name)
+        return (max_it, tol, lam), (a, b, m), out_desc
+
+    def _outputs(self, out: np.ndarray, res: "hip.SolveResult", out_desc) -> dict:
+        self.last_result = res
+        samples = out.reshape(-1)
+        all_finite = bool(np.isfinite(samples).all())
+        try:
+            candidate = ResourceValue(out_desc, out_desc.channels, samples)
+        except PaintopError:
+            raise PaintopError(ErrorClass.EXECUTION, self.buffer_code,
+                               f"{self.op_name} produced a candidate buffer of unexpected length "
+                               f"{samples.size}") from None
+        report = Report(extent=out_desc.extent, channels=out_desc.channels, all_finite=all_finite,
+                        solver=solver_data(self.kind, res))
+        return {"candidate": candidate, "report": ResourceValue.report_value(report)}
+
+    def batch_key(self, inputs: dict, params: dict):
+        """What a node must share with others to be solved in one batched call
+        with each image's result the one it gets alone: the resolved controls,
+        and the sweep configuration its own work takes (an image's result
+        depends on its launch boundaries). None: this node is solved alone (the
+        learned start gates on the call's largest image; split components)."""
+        if self.warm_start or self.split_components:
+            return None
+        controls, (_, _, m), out_desc = self._prepare(inputs, params)
+        k, hb = (self.half_sweeps, 0) if self.half_sweeps else hip.sweep_config(
+            out_desc.channels * interior_bbox_area(m.array()[..., 0]))
+        return (self.op_id, controls, self.omega, k, hb)
+
+    def compute_batch(self, calls: list) -> list:
+        """The nodes of one batch key in one batched call
+        (ph_poisson_solve_batch_device), each with the outputs its own compute
+        gives."""
+        from .device import DeviceArray
+
+        keys = {self.batch_key(i, p) for i, p in calls}
+        if len(keys) != 1 or None in keys:
+            raise PaintopError(ErrorClass.EXECUTION, "E_HIP_BATCH",
+                               f"{self.op_name}: nodes of different batch keys in one call")
+        (_, (max_it, tol, lam), omega, k, hb), = keys
+        prepared = [self._prepare(i, p) for i, p in calls]
+        arrays, items = [], []
+        try:
+            for _, (a, b, m), od in prepared:
+                ext = od.extent
+                s_ = DeviceArray.from_numpy(np.ascontiguousarray(a.array(), dtype=np.float32))
+                t_ = DeviceArray.from_numpy(np.ascontiguousarray(b.array(), dtype=np.float32))
+                m_ = DeviceArray.from_numpy(np.ascontiguousarray(m.array()[..., 0],
+                                                                 dtype=np.float32))
+                o_ = DeviceArray((ext.height, ext.width, od.channels), np.float32)
+                arrays += [s_, t_, m_, o_]
+                items.append((ext.width, ext.height, od.channels, s_.ptr, t_.ptr, m_.ptr, o_.ptr))
+            desc = hip.make_desc(0, 0, 1, max_it, tol, lam, op=self.ph_op, omega=omega,
+                                 half_sweeps=k, tile_rows=hb)
+            try:
+                results = hip.poisson_solve_batch_device(desc, items)
+            except hip.HipCallError as e:
+                raise PaintopError(ErrorClass(e.error_class), e.code, e.message) from None
+            outs = [arrays[4 * j + 3].numpy() for j in range(len(calls))]
+        finally:
+            for arr in arrays:
+                arr.free()
+        return [self._outputs(o, r, od) for o, r, (_, _, od) in zip(outs, results, prepared)]
+
+    def compute(self, inputs: dict, params: dict) -> dict:
+        """OpImplementation::compute (op_impl.rs:51-70) for this op."""
+        (max_it, tol, lam), (a, b, m), out_desc = self._prepare(inputs, params)
         channels = out_desc.channels
         ext = out_desc.extent
         desc = hip.make_desc(ext.width, ext.height, channels, max_it, tol, lam, op=self.ph_op,
@@ -225,18 +305,7 @@ class _HipPoissonOp:
         except hip.HipCallError as e:
             cls = ErrorClass(e.error_class)
             raise PaintopError(cls, e.code, e.message) from None
-        self.last_result = res
-        samples = out.reshape(-1)
-        all_finite = bool(np.isfinite(samples).all())
-        try:
-            candidate = ResourceValue(out_desc, channels, samples)
-        except PaintopError:
-            raise PaintopError(ErrorClass.EXECUTION, self.buffer_code,
-                               f"{self.op_name} produced a candidate buffer of unexpected length "
-                               f"{samples.size}") from None
-        report = Report(extent=ext, channels=channels, all_finite=all_finite,
-                        solver=solver_data(self.kind, res))
-        return {"candidate": candidate, "report": ResourceValue.report_value(report)}
+        return self._outputs(out, res, out_desc)
 
 
 class PoissonBlend(_HipPoissonOp):

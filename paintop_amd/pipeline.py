@@ -31,7 +31,8 @@ from dataclasses import dataclass, field
 from typing import Optional
 
 from .errors import E_BACKEND_UNSUPPORTED, ErrorClass, PaintopError
-from .executor import BackendId, BackendPolicy, ImplId, ImplRegistry, dispatch_node
+from .executor import (BackendId, BackendPolicy, ImplId, ImplRegistry, dispatch_batch,
+                       dispatch_node, select_backend)
 
 # Names a plan may list in policy.execution.allowed_backends (IR_SPEC §15) and
 # the backends each admits: an exact backend id, or a whole backend family.
@@ -142,13 +143,34 @@ def resolve_reference(ref: str, inputs: dict, node_outputs: dict):
                        f"resource reference `{ref}` is not available")
 
 
+def _batch_key(impl, ins: dict, params: dict):
+    """The node's batch key when its implementation batches, else None."""
+    key = getattr(impl, "batch_key", None)
+    if key is None or not hasattr(impl, "compute_batch"):
+        return None
+    try:
+        return key(ins, params)
+    except PaintopError:
+        return None  # dispatched alone, where compute reports the error
+
+
 def run_parsed_plan(plan: dict, impls: ImplRegistry, manifests: dict,
-                    inputs: Optional[dict] = None) -> RunOutcome:
+                    inputs: Optional[dict] = None, batch: bool = True) -> RunOutcome:
     """Run every node of ``plan`` in its listed (topological-compatible) order
-    under the node's policy, then resolve the exports."""
+    under the node's policy, then resolve the exports.
+
+    Executor-level batching (``batch``): a run of consecutive nodes of one op
+    whose implementation batches (the hip Poisson ops), selected to the same
+    implementation, with equal batch keys and no input of one taken from
+    another, goes to the device as one batched call
+    (``executor.dispatch_batch``). The batch key makes each node's result the
+    one it gets alone, bit for bit, so batching changes only the launches.
+    The reference dispatches node by node (executor/mod.rs:175-193)."""
     inputs = inputs or {}
     node_outputs, trace, selected = {}, [], {}
-    for node in plan.get("nodes", []):
+    nodes = plan.get("nodes", [])
+
+    def prepare(node):
         nid, op_id = node["id"], node["op"]
         manifest = manifests.get(op_id)
         if manifest is None:
@@ -156,11 +178,45 @@ def run_parsed_plan(plan: dict, impls: ImplRegistry, manifests: dict,
                                f"node {nid}: operation {op_id} is not registered")
         ins = {port: resolve_reference(ref, inputs, node_outputs)
                for port, ref in (node.get("in") or {}).items()}
+        return nid, op_id, manifest, ins, node.get("params") or {}
+
+    i = 0
+    while i < len(nodes):
+        nid, op_id, manifest, ins, params = prepare(nodes[i])
         policy = node_policy(plan, op_id, manifest)
-        node_outputs[nid] = dispatch_node(nid, op_id, manifest, impls, policy, ins,
-                                          node.get("params") or {}, trace)
-        selected[nid] = next(e["implementation"] for e in reversed(trace)
-                             if e["event"] == "implementation_selected" and e["node"] == nid)
+        group = [(nid, ins, params)]
+        if batch:
+            sel = select_backend(op_id, manifest, impls, policy)
+            impl = impls.get_backend(op_id, sel.impl_id.backend_id())
+            key = _batch_key(impl, ins, params)
+            j = i + 1
+            while key is not None and j < len(nodes) and nodes[j]["op"] == op_id:
+                refs = (nodes[j].get("in") or {}).values()
+                if any(r.startswith("node:") and r[5:].partition("/")[0] in
+                       {g[0] for g in group} for r in refs):
+                    break
+                try:
+                    nj, _, _, ins_j, params_j = prepare(nodes[j])
+                except PaintopError:
+                    break
+                if select_backend(op_id, manifest, impls,
+                                  node_policy(plan, op_id, manifest)) != sel or \
+                        _batch_key(impl, ins_j, params_j) != key:
+                    break
+                group.append((nj, ins_j, params_j))
+                j += 1
+        if len(group) > 1:
+            outs = dispatch_batch([g[0] for g in group], op_id, manifest, impl, sel,
+                                  [(g[1], g[2]) for g in group], trace)
+            for (gid, _, _), out in zip(group, outs):
+                node_outputs[gid] = out
+                selected[gid] = str(sel.impl_id)
+        else:
+            node_outputs[nid] = dispatch_node(nid, op_id, manifest, impls, policy, ins, params,
+                                              trace)
+            selected[nid] = next(e["implementation"] for e in reversed(trace)
+                                 if e["event"] == "implementation_selected" and e["node"] == nid)
+        i += len(group)
     exports = {}
     for eid, spec in (plan.get("exports") or {}).items():
         ref = spec["resource"] if isinstance(spec, dict) else spec

@@ -127,3 +127,61 @@ def test_cache_keeps_hip_and_reference_entries_apart(gpu, oracle):
     ref_c = r1["node_outputs"]["blend"]["candidate"].samples
     assert hip_c.tobytes() == h1["node_outputs"]["blend"]["candidate"].samples.tobytes()
     assert hip_c.tobytes() != ref_c.tobytes()  # bounded, not exact: distinct entries
+
+
+def test_plan_of_many_blends_runs_as_one_batched_call(gpu, oracle):
+    """Executor-level batching on the device: a plan of five independent
+    blend nodes (uneven C4-style images, two of them reading the same inputs,
+    one RGBA star) plus a screened node runs its blends as one batched call
+    (ph_poisson_solve_batch_device, each group passing its own sweep
+    configuration) and gives every node's candidate and report bit for bit as
+    the same plan run node by node; each is within the bounded tier of the
+    reference run."""
+    impls, manifests = _registry()
+    ins, nodes = {}, []
+    for i in range(4):
+        rng = SplitMix64(700 + i)
+        s = math.exp(rng.uniform(math.log(32.0), math.log(160.0)))
+        asp = rng.uniform(0.5, 2.0)
+        w, h = max(8, round(s * math.sqrt(asp))), max(8, round(s / math.sqrt(asp)))
+        verts = random_polygon(w / 2, h / 2, 0.45 * w, 0.45 * h, 6 + i, rng)
+        m = oracle.polygon_mask(w, h, verts)
+        src = oracle.fbm(w, h, 2.0, 20_000 + 2 * i, 6) * np.float32(0.5) + np.float32(0.5)
+        tgt = oracle.fbm(w, h, 2.0, 20_001 + 2 * i, 6) * np.float32(0.5) + np.float32(0.5)
+        ins.update({f"s{i}": image(w, h, src), f"t{i}": image(w, h, tgt), f"m{i}": mask(w, h, m)})
+    n = 96
+    src, tgt = (np.stack([oracle.fbm(n, n, 1.5, k + c, 6) * np.float32(0.5) + np.float32(0.5)
+                          for c in range(3)] + [np.ones((n, n), np.float32)], axis=-1)
+                for k in (30_100, 30_200))
+    m4 = oracle.polygon_mask(n, n, star_polygon(n / 2, n / 2, 36.0, 0.5, 6, 0.1, 301))
+    ins.update({"s4": image(n, n, src, "Rgba"), "t4": image(n, n, tgt, "Rgba"),
+                "m4": mask(n, n, m4)})
+    for i, srcs in enumerate([0, 1, 2, 3, 4, 0]):
+        nodes.append({"id": f"b{i}", "op": ops.POISSON_BLEND_OP_ID,
+                      "in": {"source": f"input:s{srcs}", "target": f"input:t{srcs}",
+                             "mask": f"input:m{srcs}"}, "params": TIGHT})
+    nodes.append({"id": "scr", "op": ops.SCREENED_POISSON_OP_ID,
+                  "in": {"guidance": "input:s0", "anchor": "input:t0", "mask": "input:m0"},
+                  "params": {"tolerance": 1e-9, "max_iterations": 100000, "lambda": 0.25}})
+    plan = {"paintop": "1.0", "policy": {"execution": {"allowed_backends": ["hip"]}},
+            "inputs": {k: {} for k in ins}, "nodes": nodes,
+            "exports": {nd["id"]: {"resource": f"node:{nd['id']}/candidate"} for nd in nodes}}
+    out = pipeline.run_parsed_plan(plan, impls, manifests, ins)
+    batched = [e["nodes"] for e in out.trace if e["event"] == "dispatch_batched"]
+    print(f"[plan] batched dispatches {batched}", flush=True)
+    assert batched and sum(len(b) for b in batched) >= 2
+    assert all(s == "hip.poisson@1" for s in out.selected.values())
+    one = pipeline.run_parsed_plan(plan, impls, manifests, ins, batch=False)
+    assert not [e for e in one.trace if e["event"] == "dispatch_batched"]
+    for nd in nodes:
+        nid = nd["id"]
+        assert out.exports[nid].array().tobytes() == one.exports[nid].array().tobytes(), nid
+        ra = out.node_outputs[nid]["report"].as_report().solver
+        rb = one.node_outputs[nid]["report"].as_report().solver
+        assert (ra.iterations, ra.final_residual, ra.residual_history) == \
+            (rb.iterations, rb.final_residual, rb.residual_history), nid
+    ref_plan = dict(plan, policy={"execution": {"allowed_backends": ["cpu-reference"]}})
+    ref = pipeline.run_parsed_plan(ref_plan, impls, manifests, ins)
+    for nd in nodes:
+        em = D.ErrorMap.compute(ref.exports[nd["id"]], out.exports[nd["id"]])
+        assert em.within(D.BOUNDED), (nd["id"], em)

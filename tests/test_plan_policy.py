@@ -271,3 +271,113 @@ def test_hip_availability_from_the_probe():
     assert gate.reason is not None and "E_GPU_UNAVAILABLE" in gate.reason
     assert gate.unavailable(HIP) is not None
     assert gate.unavailable(BackendId("cpu", "optimized")) is None
+
+
+# ---- executor-level batching -----------------------------------------------------
+
+
+class _BatchingHip:
+    """A hip stand-in that batches: its batch key is the resolved controls
+    (None for a node carrying the `solo` marker param), compute_batch records
+    each call's node count and returns what compute returns per node."""
+
+    def __init__(self):
+        self.batches = []
+        self.singles = 0
+
+    def batch_key(self, inputs, params):
+        if params.get("_solo"):
+            return None
+        p = {k: v for k, v in params.items() if k != "_solo"}
+        return tuple(sorted(ops.resolve_blend_controls(p)))
+
+    def compute(self, inputs, params):
+        from tests.oracle_ops import PoissonBlendOracle
+
+        self.singles += 1
+        p = {k: v for k, v in params.items() if k != "_solo"}
+        return PoissonBlendOracle().compute(inputs, p)
+
+    def compute_batch(self, calls):
+        from tests.oracle_ops import PoissonBlendOracle
+
+        self.batches.append(len(calls))
+        return [PoissonBlendOracle().compute(i, {k: v for k, v in p.items() if k != "_solo"})
+                for i, p in calls]
+
+
+def _multi_plan(nodes, allowed=("hip", "cpu-reference")):
+    return {"paintop": "1.0", "name": "many-blends",
+            "inputs": {"source": {"kind": "image"}, "target": {"kind": "image"},
+                       "mask": {"kind": "mask"}, "source2": {"kind": "image"}},
+            "nodes": nodes,
+            "exports": {n["id"]: {"resource": f"node:{n['id']}/candidate", "kind": "image"}
+                        for n in nodes},
+            "policy": {"execution": {"deadline_ms": 10000, "allowed_backends": list(allowed)}}}
+
+
+def _blend(nid, src="input:source", params=None):
+    return {"id": nid, "op": ops.POISSON_BLEND_OP_ID,
+            "in": {"source": src, "target": "input:target", "mask": "input:mask"},
+            "params": params or {"tolerance": 1e-6, "max_iterations": 3000}}
+
+
+FAKE_PROBE = {"name": "test double", "arch": "gfx950", "compute_units": 256,
+              "runtime_version": 0, "driver_version": 0, "hbm_bytes": 0}
+
+
+def test_consecutive_independent_nodes_go_in_one_batch(oracle, monkeypatch):
+    """run_parsed_plan gathers a run of consecutive nodes of one op with equal
+    batch keys and no input from each other into one compute_batch call; a
+    node reading another's output, a node with other controls, a node that
+    batches alone and a node whose policy selects another implementation each
+    end the run. Outputs equal the unbatched run's bit for bit."""
+    from paintop_amd import hip
+
+    monkeypatch.setattr(hip, "probe", lambda: dict(FAKE_PROBE))  # no device here
+    impls, manifests = reference_registry()
+    fake = _BatchingHip()
+    impls.register_backend(ops.POISSON_BLEND_OP_ID, ImplId.parse(ops.HIP_IMPL_ID), fake)
+    ins = small_inputs(24)
+    ins["source2"] = image(24, 24, ins["source"].array()[..., 0] * np.float32(0.5))
+    nodes = [
+        _blend("a"), _blend("b", "input:source2"), _blend("c"),           # one batch of 3
+        _blend("d", "node:c/candidate"),                                  # reads c: new run
+        _blend("e"),                                                      # batch with d
+        _blend("f", params={"tolerance": 1e-5, "max_iterations": 3000}),  # other controls
+        _blend("g", params={"tolerance": 1e-6, "max_iterations": 3000, "_solo": True}),
+        _blend("h"), _blend("i"),                                         # batch of 2
+    ]
+    out = pipeline.run_parsed_plan(_multi_plan(nodes), impls, manifests, ins)
+    assert fake.batches == [3, 2, 2] and fake.singles == 2
+    batched = [e["nodes"] for e in out.trace if e["event"] == "dispatch_batched"]
+    assert batched == [["a", "b", "c"], ["d", "e"], ["h", "i"]]
+    assert set(out.selected.values()) == {ops.HIP_IMPL_ID}
+    sel = [e["node"] for e in out.trace if e["event"] == "implementation_selected"]
+    assert sel == list("abcdefghi")
+    # the same plan node by node
+    impls2, _ = reference_registry()
+    fake2 = _BatchingHip()
+    impls2.register_backend(ops.POISSON_BLEND_OP_ID, ImplId.parse(ops.HIP_IMPL_ID), fake2)
+    out1 = pipeline.run_parsed_plan(_multi_plan(nodes), impls2, manifests, ins, batch=False)
+    assert fake2.batches == [] and fake2.singles == 9
+    for nid in "abcdefghi":
+        assert out.exports[nid].array().tobytes() == out1.exports[nid].array().tobytes()
+
+
+def test_batching_respects_the_selected_implementation(oracle):
+    """A plan that allows only the reference never batches (the oracle does
+    not batch), and a hip node is not batched with one the policy sends to
+    another implementation."""
+    impls, manifests = reference_registry()
+    fake = _BatchingHip()
+    impls.register_backend(ops.POISSON_BLEND_OP_ID, ImplId.parse(ops.HIP_IMPL_ID), fake)
+    ins = small_inputs(16)
+    ins["source2"] = ins["source"]
+    nodes = [_blend("a"), _blend("b")]
+    out = pipeline.run_parsed_plan(_multi_plan(nodes, allowed=("cpu-reference",)), impls,
+                                   manifests, ins)
+    assert fake.batches == [] and fake.singles == 0
+    assert not [e for e in out.trace if e["event"] == "dispatch_batched"]
+    assert set(out.selected.values()) == {"cpu.reference.poisson_blend@1"} or \
+        all(s.startswith("cpu.reference") for s in out.selected.values())
