@@ -192,19 +192,41 @@ __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__res
     counts[G.coarse_off + (int64_t)cy * G.cw + c] = cnt[c];
 }
 
+// Tree sum over the 256 threads of a block, in a fixed order (thread 0).
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  return red[0];
+}
+
 // Fill the planes of the listed images' channel instances, one block per plane
 // row (spans: slot = the first of the image's C consecutive descriptors, one
 // per channel, a descriptor with no planes (x == nullptr) is skipped; item =
-// the image). Each pixel's channels are read together, so the interleaved
-// source and target are read once per image, not once per channel.
+// the image; out = the first descriptor's first row in row_sums, the
+// channels' rows following each other, ph apart). Each pixel's channels are
+// read together, so the interleaved source and target are read once per
+// image, not once per channel.
 // b = fma(4, -c, ((l + r) + u) + d) of the source (poisson.rs:364-373),
 // b' = fma(lambda, -anchor, b) (poisson.rs:319); x starts at the anchor.
+// The anchor start's residual comes in the same pass: at an interior cell
+// r = fma(4 + lambda, -x, ((x_l + x_r) + x_u) + x_d) - b' with x the anchor
+// at the cell and its four neighbours (all in the image: interior cells are
+// never on its border), the FP64 residual's own operation order
+// (poisson.rs:305-325); g = -r (FP32) there and NaN elsewhere, and the row's
+// sum of r^2 goes to row_sums (each thread's cells in column order, then the
+// block's fixed tree).
 // CC = C for 1..4 (the descriptors' pointers then sit in registers), 0 for
-// any C (read per cell).
+// any C (read per cell; the residual then takes a pass per channel).
 template <int CC>
 __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict__ insts,
-                                                        const PhSpan *__restrict__ spans, int n,
-                                                        const PhItemIO *__restrict__ io) {
+                                                       const PhSpan *__restrict__ spans, int n,
+                                                       const PhItemIO *__restrict__ io,
+                                                       double *__restrict__ row_sums) {
   const int b = xcd_row_group(blockIdx.x, gridDim.x);  // reads the source rows around b
   const PhSpan S = spans[ph_span_find(spans, n, b)];
   const PhItemIO &G = io[S.item];
@@ -218,6 +240,9 @@ __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict
   constexpr int R = CC ? CC : 1;
   double *xs[R], *bps[R];
   float *e0s[R], *e1s[R], *gs[R];
+  double rs[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) rs[c] = 0.0;
   if (CC) {
 #pragma unroll
     for (int c = 0; c < R; ++c) {
@@ -230,6 +255,22 @@ __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict
     }
   }
   const double lambda = I0.lambda;
+  const double dg = I0.dg64;
+  // b' of channel c at interior image cell q, and its anchor residual
+  auto bprime = [&](size_t q, int c, double a) {
+    const double cc = (double)src[q * C + c];
+    const double l = (double)src[(q - 1) * C + c];
+    const double r = (double)src[(q + 1) * C + c];
+    const double u = (double)src[(q - W) * C + c];
+    const double d = (double)src[(q + W) * C + c];
+    const double bb = fma(4.0, -cc, ((l + r) + u) + d);
+    return fma(lambda, -a, bb);
+  };
+  auto anchor_r = [&](size_t q, int c, double a, double bp) {
+    const double xl = (double)tgt[(q - 1) * C + c], xr = (double)tgt[(q + 1) * C + c];
+    const double xu = (double)tgt[(q - W) * C + c], xd = (double)tgt[(q + W) * C + c];
+    return fma(dg, -a, ((xl + xr) + xu) + xd) - bp;
+  };
   for (int px = threadIdx.x; px < I0.pitch; px += 256) {
     const size_t p = (size_t)py * I0.pitch + px;
     const int ix = I0.ox + px, iy = I0.oy + py;
@@ -243,17 +284,17 @@ __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict
       double *x = CC ? xs[c] : insts[S.slot + c].x;
       if (!x) continue;
       double a = 0.0, bp = 0.0;
+      float gv = __builtin_nanf("");
       if (inside) {
         a = (double)tgt[q * C + c];
         if (in) {
           // interior cells are never on the image border: four in-image neighbours
-          const double cc = (double)src[q * C + c];
-          const double l = (double)src[(q - 1) * C + c];
-          const double r = (double)src[(q + 1) * C + c];
-          const double u = (double)src[(q - W) * C + c];
-          const double d = (double)src[(q + W) * C + c];
-          const double bb = fma(4.0, -cc, ((l + r) + u) + d);
-          bp = fma(lambda, -a, bb);
+          bp = bprime(q, c, a);
+          if (CC) {
+            const double r = anchor_r(q, c, a, bp);
+            gv = (float)(-r);
+            rs[c] = fma(r, r, rs[c]);
+          }
         }
       }
       x[p] = a;
@@ -261,27 +302,58 @@ __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict
         bps[c][p] = bp;
         e0s[c][p] = 0.0f;
         e1s[c][p] = 0.0f;
-        gs[c][p] = __builtin_nanf("");
+        gs[c][p] = gv;
       } else {
         const PhInst &I = insts[S.slot + c];
         I.bp[p] = bp;
         I.e[0][p] = 0.0f;
         I.e[1][p] = 0.0f;
-        I.g[p] = __builtin_nanf("");
+        I.g[p] = gv;
       }
+    }
+  }
+  if (CC) {
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      const double t = block_sum(rs[c]);
+      if (threadIdx.x == 0 && xs[c]) row_sums[S.out + (size_t)c * I0.ph + py] = t;
+      __syncthreads();
+    }
+  } else {
+    // any C: the residual in a pass per channel over the row (its g written
+    // by this thread in the pass above, so no other thread's writes are read)
+    for (int c = 0; c < C; ++c) {
+      const PhInst &I = insts[S.slot + c];
+      if (!I.x) continue;
+      double sr = 0.0;
+      for (int px = threadIdx.x; px < I0.pitch; px += 256) {
+        const size_t p = (size_t)py * I0.pitch + px;
+        const int ix = I0.ox + px, iy = I0.oy + py;
+        const bool inside = px < I0.pw && ix >= 0 && ix < W && iy >= 0 && iy < H;
+        if (!inside) continue;
+        const size_t q = (size_t)iy * W + ix;
+        if (!cls_img[q]) continue;
+        const double a = (double)tgt[q * C + c];
+        const double r = anchor_r(q, c, a, bprime(q, c, a));
+        I.g[p] = (float)(-r);
+        sr = fma(r, r, sr);
+      }
+      const double t = block_sum(sr);
+      if (threadIdx.x == 0) row_sums[S.out + (size_t)c * I0.ph + py] = t;
+      __syncthreads();
     }
   }
 }
 template __global__ void k_assemble_spans<0>(const PhInst *, const PhSpan *, int,
-                                             const PhItemIO *);
+                                             const PhItemIO *, double *);
 template __global__ void k_assemble_spans<1>(const PhInst *, const PhSpan *, int,
-                                             const PhItemIO *);
+                                             const PhItemIO *, double *);
 template __global__ void k_assemble_spans<2>(const PhInst *, const PhSpan *, int,
-                                             const PhItemIO *);
+                                             const PhItemIO *, double *);
 template __global__ void k_assemble_spans<3>(const PhInst *, const PhSpan *, int,
-                                             const PhItemIO *);
+                                             const PhItemIO *, double *);
 template __global__ void k_assemble_spans<4>(const PhInst *, const PhSpan *, int,
-                                             const PhItemIO *);
+                                             const PhItemIO *, double *);
 
 // g = -r(x) in FP64, stored FP32 (NaN at non-interior cells), and this
 // thread's part of the row's sum of r^2 (the block adds the parts in a fixed
@@ -327,18 +399,6 @@ __device__ __forceinline__ double residual_row(const PhInst &I, int py) {
     }
   }
   return s;
-}
-
-// Tree sum over the 256 threads of a block, in a fixed order (thread 0).
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double red[256];
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  return red[0];
 }
 
 // Fixed-order sum of nrows row sums (thread 0): thread t takes a contiguous

@@ -52,7 +52,7 @@ __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 
                                      int *);
 __global__ void k_coarse_count_all(const PhItemGeo *, const PhSpan *, int, int *);
 template <int CC>
-__global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *);
+__global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *, double *);
 __global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
 __global__ void k_residual_spans(const PhInst *, const PhSpan *, int, double *);
 __global__ void k_residual_norm_spans(const PhSpan *, const double *, double *);
@@ -574,6 +574,22 @@ struct SweepGraphs {
 };
 static SweepGraphs g_graphs;
 
+// A second stream of default priority (the learned start's network pass runs
+// on it beside the solve's stream) and the two events that order them.
+static hipStream_t side_stream() {
+  static hipStream_t s = nullptr;
+  if (!s) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+static std::pair<hipEvent_t, hipEvent_t> side_events() {
+  static hipEvent_t a = nullptr, b = nullptr;
+  if (!a) {
+    HIP_OK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+  }
+  return {a, b};
+}
+
 // PAINTOP_HIP_TRACE=1: the warm-start race's events on stderr.
 static bool trace_on() {
   const char *e = getenv("PAINTOP_HIP_TRACE");
@@ -815,8 +831,8 @@ static void launch_by_channels(int cc, A... args) {
 template <int CC>
 struct AssembleL {
   static void go(dim3 g, hipStream_t st, const PhInst *i, const PhSpan *sp, int n,
-                 const PhItemIO *io) {
-    hipLaunchKernelGGL(k_assemble_spans<CC>, g, dim3(256), 0, st, i, sp, n, io);
+                 const PhItemIO *io, double *row_sums) {
+    hipLaunchKernelGGL(k_assemble_spans<CC>, g, dim3(256), 0, st, i, sp, n, io, row_sums);
   }
 };
 template <int CC>
@@ -1187,8 +1203,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     inst_all[k] = inst[k].I;
     inst_rows.add(k, inst[k].I.ph, inst[k].item, 0, k);
   }
+  // (an image's entry carries its first instance's first row in the
+  // instances' row sums: the assembly takes the anchor residual too)
   for (int i = 0; i < nitem; ++i)
-    if (iw[i].any) img_planes.add(iw[i].inst0, iw[i].phh, i);
+    if (iw[i].any) img_planes.add(iw[i].inst0, iw[i].phh, i, 0, inst_rows.h[iw[i].inst0].row0);
   if (ninst) {
     for (int i = 0; i < nitem; ++i) io[i].cls_plane = iw[i].cls_plane;
     double *inst_sums_d = g_pool.take<double>((size_t)inst_rows.rows);
@@ -1199,9 +1217,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     img_planes.upload(img_planes_d, st);
     launch_by_channels<AssembleL>(common_channels(img_planes, iw), dim3(img_planes.rows), st,
                                   (const PhInst *)inst_all_d, (const PhSpan *)img_planes_d,
-                                  (int)img_planes.h.size(), (const PhItemIO *)io_d);
-    hipLaunchKernelGGL(k_residual_spans, dim3(inst_rows.rows), dim3(256), 0, st, inst_all_d,
-                       inst_rows_d, ninst, inst_sums_d);
+                                  (int)img_planes.h.size(), (const PhItemIO *)io_d, inst_sums_d);
     hipLaunchKernelGGL(k_residual_norm_spans, dim3(ninst), dim3(256), 0, st, inst_rows_d,
                        inst_sums_d, resv_d);
     HIP_OK(hipGetLastError());
@@ -1304,34 +1320,10 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
   SpanTable warm_rows, warm_groups, cand_rows, anchor_rows;
   if (!cand.empty()) {
     const int nw = (int)cand.size();
-    // the anchor-start state of every candidate, copied before its start
-    // overwrites it: the plain twin of the race, or the way back to the anchor
-    // start for a start rejected outright (the source term b' is shared)
-    for (int j = 0; j < nw; ++j) {
-      const InstWork &X = inst[cand[j]];
-      const size_t plane = (size_t)X.I.pitch * X.I.ph;
-      PhInst A = X.I;
-      A.e[0] = g_pool.take<float>(plane);
-      A.e[1] = g_pool.take<float>(plane);
-      A.g = g_pool.take<float>(plane);
-      A.x = g_pool.take<double>(plane);
-      anchor_from.push_back(X.I);
-      anchor_to.push_back(A);
-      anchor_rows.add(j, X.I.ph, X.item);
-    }
-    {
-      PhInst *from_d = g_pool.take<PhInst>(nw);
-      PhInst *to_d = g_pool.take<PhInst>(nw);
-      PhSpan *rows_d = g_pool.take<PhSpan>(nw);
-      HIP_OK(hipMemcpyAsync(from_d, anchor_from.data(), nw * sizeof(PhInst),
-                            hipMemcpyHostToDevice, st));
-      HIP_OK(hipMemcpyAsync(to_d, anchor_to.data(), nw * sizeof(PhInst), hipMemcpyHostToDevice,
-                            st));
-      anchor_rows.upload(rows_d, st);
-      hipLaunchKernelGGL(k_copy_anchor_spans, dim3(anchor_rows.rows), dim3(256), 0, st, from_d,
-                         to_d, rows_d, nw);
-      HIP_OK(hipGetLastError());
-    }
+    hipStream_t side = nullptr;
+    PhCorrArgs *warm_args_d = nullptr;
+    PhSpan *warm_spans_d = nullptr;
+    std::function<void()> launch_net;  // the side stream's work, enqueued after the copy
     if (!net_list.empty()) {
       const int nn = (int)net_list.size();
       for (int k : net_list) {
@@ -1364,27 +1356,84 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         }
       }
       group0.push_back((int)warm_groups.h.size());
-      PhCorrArgs *warm_args_d = g_pool.take<PhCorrArgs>(nn);
-      PhSpan *warm_spans_d = g_pool.take<PhSpan>(nn);
+      warm_args_d = g_pool.take<PhCorrArgs>(nn);
+      warm_spans_d = g_pool.take<PhSpan>(nn);
       PhSpan *warm_groups_d = g_pool.take<PhSpan>(nn);
       HIP_OK(hipMemcpyAsync(warm_args_d, warm_args.data(), nn * sizeof(PhCorrArgs),
                             hipMemcpyHostToDevice, st));
       warm_rows.upload(warm_spans_d, st);
       warm_groups.upload(warm_groups_d, st);
-      for (int j0 = 0, bi = 0; j0 < nn; j0 += NNB, ++bi) {
-        const int nb = std::min(NNB, nn - j0);
-        hipLaunchKernelGGL(k_coarsen_groups, dim3(NR * NR, group0[bi + 1] - group0[bi]), dim3(64),
-                           0, st, inst_all_d, warm_args_d, warm_groups_d + group0[bi], io_d,
-                           d->lambda, NR, j0, net_in);
-        hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
+      // the inputs (HBM-bound) here; the network passes (latency-bound: small
+      // launches) on a second stream beside the copy of the twins' anchor
+      // state (HBM-bound) that follows. One network batch's input buffer
+      // holds NNB inputs, so batches past the first wait for the previous pass.
+      side = side_stream();
+      const int nbatch = (nn + NNB - 1) / NNB;
+      if (nbatch == 1) {
+        hipLaunchKernelGGL(k_coarsen_groups, dim3(NR * NR, group0[1] - group0[0]), dim3(64), 0, st,
+                           inst_all_d, warm_args_d, warm_groups_d, io_d, d->lambda, NR, 0, net_in);
+        hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nn), dim3(256), 0, st, net_in, NR,
                            net_part);
-        hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, st, net_in, NR,
-                           net_part, net_scale + j0);
+        hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nn), dim3(256), 0, st, net_in, NR,
+                           net_part, net_scale);
         HIP_OK(hipGetLastError());
-        warm_net_forward(net_in, net_h + NR2 * j0, nb, st);
       }
+      launch_net = [=, &side]() {
+        HIP_OK(hipStreamWaitEvent(side, side_events().first, 0));
+        for (int j0 = 0, bi = 0; j0 < nn; j0 += NNB, ++bi) {
+          const int nb = std::min(NNB, nn - j0);
+          if (nbatch > 1) {
+            hipLaunchKernelGGL(k_coarsen_groups, dim3(NR * NR, group0[bi + 1] - group0[bi]),
+                               dim3(64), 0, side, inst_all_d, warm_args_d,
+                               warm_groups_d + group0[bi], io_d, d->lambda, NR, j0, net_in);
+            hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, side,
+                               net_in, NR, net_part);
+            hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, side,
+                               net_in, NR, net_part, net_scale + j0);
+            HIP_OK(hipGetLastError());
+          }
+          warm_net_forward(net_in, net_h + NR2 * j0, nb, side);
+        }
+        HIP_OK(hipEventRecord(side_events().second, side));
+      };
+    }
+    // the anchor-start state of every candidate, copied before its start
+    // overwrites it: the plain twin of the race, or the way back to the anchor
+    // start for a start rejected outright (the source term b' is shared)
+    for (int j = 0; j < nw; ++j) {
+      const InstWork &X = inst[cand[j]];
+      const size_t plane = (size_t)X.I.pitch * X.I.ph;
+      PhInst A = X.I;
+      A.e[0] = g_pool.take<float>(plane);
+      A.e[1] = g_pool.take<float>(plane);
+      A.g = g_pool.take<float>(plane);
+      A.x = g_pool.take<double>(plane);
+      anchor_from.push_back(X.I);
+      anchor_to.push_back(A);
+      anchor_rows.add(j, X.I.ph, X.item);
+    }
+    {
+      PhInst *from_d = g_pool.take<PhInst>(nw);
+      PhInst *to_d = g_pool.take<PhInst>(nw);
+      PhSpan *rows_d = g_pool.take<PhSpan>(nw);
+      HIP_OK(hipMemcpyAsync(from_d, anchor_from.data(), nw * sizeof(PhInst),
+                            hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(to_d, anchor_to.data(), nw * sizeof(PhInst), hipMemcpyHostToDevice,
+                            st));
+      anchor_rows.upload(rows_d, st);
+      // the side stream starts behind every upload, beside the copy
+      if (launch_net) HIP_OK(hipEventRecord(side_events().first, st));
+      hipLaunchKernelGGL(k_copy_anchor_spans, dim3(anchor_rows.rows), dim3(256), 0, st, from_d,
+                         to_d, rows_d, nw);
+      HIP_OK(hipGetLastError());
+    }
+    if (launch_net) launch_net();
+    if (!net_list.empty()) {
+      // the predictions join the solve's stream, then every start is applied
+      HIP_OK(hipStreamWaitEvent(st, side_events().second, 0));
       hipLaunchKernelGGL(k_warm_apply_spans, dim3(warm_rows.rows), dim3(256), 0, st, inst_all_d,
-                         warm_spans_d, nn, warm_args_d, io_d, net_h, net_scale, NR);
+                         warm_spans_d, (int)net_list.size(), warm_args_d, io_d, net_h, net_scale,
+                         NR);
       HIP_OK(hipGetLastError());
     }
     // the caller's start fields, then the starts' FP64 residuals in three launches
