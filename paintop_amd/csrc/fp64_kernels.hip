@@ -145,17 +145,31 @@ __global__ void k_tile_occupancy_all(const PhItemGeo *__restrict__ items,
   }
 }
 
-// Interior-cell count of each f x f block of an image's region (the coarse
-// mask the SOR-factor tuner works on), for every image of a call in one
-// launch: one block per coarse row of every listed image (spans: slot =
-// image, nrows = its coarse rows). The 4 waves share the coarse row's f image rows and
-// read along them (coalesced), each lane adds its columns over its rows, and
-// the sums meet in LDS counters (integer adds: the counts do not depend on
-// the order). cw <= PH_COARSE_MAX.
+// splitmix64's output function (Steele, Lea and Flood 2014): a bijective mix
+// of a 64-bit integer.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// The coarse mask the SOR-factor tuner works on, for every image of a call in
+// one launch: a block of f x f cells of an image's region is inside when at
+// least half of its cells are interior. One block per coarse row of every
+// listed image (spans: slot = image, nrows = its coarse rows). The 4 waves
+// share the coarse row's f image rows and read along them (coalesced), each
+// lane adds its columns over its rows, and the sums meet in LDS counters
+// (integer adds: the counts do not depend on the order). Writes the row's
+// mask bytes and row_hash[block] = the sum mod 2^64 of mix64(index) over the
+// row's inside cells (index = cy * cw + c within the image's mask), so an
+// image's key (k_coarse_hash_items) is independent of the summation order.
+// cw <= PH_COARSE_MAX.
 constexpr int PH_COARSE_MAX = 256;
 __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__restrict__ items,
                                                           const PhSpan *__restrict__ spans,
-                                                          int n, int *__restrict__ counts) {
+                                                          int n, uint8_t *__restrict__ mask,
+                                                          uint64_t *__restrict__ row_hash) {
   const PhSpan S = spans[ph_span_find(spans, n, blockIdx.x)];
   const PhItemGeo G = items[S.slot];
   const int cy = blockIdx.x - S.row0;
@@ -188,8 +202,40 @@ __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__res
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < G.cw; c += 256)
-    counts[G.coarse_off + (int64_t)cy * G.cw + c] = cnt[c];
+  uint64_t h = 0;
+  const int full = G.cf * G.cf;
+  for (int c = threadIdx.x; c < G.cw; c += 256) {
+    const bool in = 2 * cnt[c] >= full;
+    const int64_t i = (int64_t)cy * G.cw + c;
+    mask[G.coarse_off + i] = in ? 1 : 0;
+    if (in) h += mix64((uint64_t)i);
+  }
+  __shared__ uint64_t hs[256];
+  hs[threadIdx.x] = h;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) hs[threadIdx.x] += hs[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) row_hash[blockIdx.x] = hs[0];
+}
+
+// Per listed image (one block per span entry of k_coarse_count_all's table):
+// hash[e] = the sum mod 2^64 of its rows' row_hash.
+__global__ __launch_bounds__(256) void k_coarse_hash_items(const PhSpan *__restrict__ spans,
+                                                           const uint64_t *__restrict__ row_hash,
+                                                           uint64_t *__restrict__ hash) {
+  const PhSpan S = spans[blockIdx.x];
+  uint64_t h = 0;
+  for (int r = threadIdx.x; r < S.nrows; r += 256) h += row_hash[S.row0 + r];
+  __shared__ uint64_t hs[256];
+  hs[threadIdx.x] = h;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) hs[threadIdx.x] += hs[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) hash[blockIdx.x] = hs[0];
 }
 
 // Tree sum over the 256 threads of a block, in a fixed order (thread 0).

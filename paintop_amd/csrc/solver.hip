@@ -50,7 +50,9 @@ __global__ void k_classify_spans(const PhSpan *, int, const PhItemIO *, int2 *);
 __global__ void k_bbox_spans(const PhSpan *, const int2 *, int *);
 __global__ void k_tile_occupancy_all(const PhItemGeo *, const int *, const int2 *, int, int, int,
                                      int *);
-__global__ void k_coarse_count_all(const PhItemGeo *, const PhSpan *, int, int *);
+__global__ void k_coarse_count_all(const PhItemGeo *, const PhSpan *, int, uint8_t *,
+                                   uint64_t *);
+__global__ void k_coarse_hash_items(const PhSpan *, const uint64_t *, uint64_t *);
 template <int CC>
 __global__ void k_assemble_spans(const PhInst *, const PhSpan *, int, const PhItemIO *, double *);
 __global__ void k_apply_correction_spans(const PhInst *, const PhSpan *, int);
@@ -304,7 +306,10 @@ static void validate(const ph_solve_desc *d, bool check_shape = true) {
 // half of its f x f cells are), by inverse iteration with conjugate gradients,
 // and scaled by 1/f^2. On a 4096^2 star domain a <= 128^2 coarse grid lands
 // within a few percent of the fine eigenvalue, on the low side (a slightly
-// large omega, where SOR degrades gently). Results are cached per coarse mask.
+// large omega, where SOR degrades gently). Results are cached per coarse mask,
+// keyed by its size, factor and a 64-bit hash the device computes with the
+// mask (the sum of a mix of every inside cell's index), so a cached mask is
+// never copied to the host.
 struct TuneKey {
   int cw, ch, f;
   uint64_t hash;
@@ -690,6 +695,7 @@ struct ItemWork {
   double n_interior = 0;
   int cf = 1, cw = 0, ch = 0;
   size_t coarse_off = 0;
+  int coarse_entry = -1;  // entry in the coarse-mask launch, or -1: no mask
   bool precise = false;
   double omega = 1.0;
   uint32_t stall_window = PH_STALL_WINDOW;
@@ -718,17 +724,6 @@ static double mu_bound(double n_interior, int bw, int bh) {
   return std::min(std::max(mu_disc, mu_rect), 4.0);
 }
 
-// The coarse mask of interior counts (a block is inside when at least half of
-// its cells are interior) and its cache key.
-static TuneKey coarse_mask(const int *counts, int f, int cw, int ch, std::vector<uint8_t> &in) {
-  in.assign((size_t)cw * ch, 0);
-  uint64_t hsh = 1469598103934665603ull;  // FNV-1a of the coarse mask
-  for (size_t i = 0; i < in.size(); ++i) {
-    in[i] = 2 * counts[i] >= f * f ? 1 : 0;
-    hsh = (hsh ^ in[i]) * 1099511628211ull;
-  }
-  return TuneKey{cw, ch, f, hsh};
-}
 
 // PAINTOP_HIP_TUNE_CACHE=0: every solve pays the tuner (the cost a new mask
 // sees), e.g. to time it.
@@ -1021,7 +1016,11 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     for (int m = 0; m < nblocks; ++m)
       for (int n = 0; n < nstrips; ++n) all_origins.push_back(make_int2(n * WV, m * hb));
     w.origins.assign(all_origins.begin() + w.origin_off, all_origins.end());
-    if (tuned) {
+    // (a coarse mask only where the interior's bounding box could hold the
+    // precise tuner's minimum of interior cells)
+    const double bbox_area = (double)(bbox_h[4 * i + 2] - bbox_h[4 * i] + 1) *
+                             (bbox_h[4 * i + 3] - bbox_h[4 * i + 1] + 1);
+    if (tuned && bbox_area >= PRECISE_TUNE_MIN_CELLS) {
       w.cf = std::max(1, (std::max(w.rw, w.rh) + 127) / 128);
       w.cw = (w.rw + w.cf - 1) / w.cf;
       w.ch = (w.rh + w.cf - 1) / w.cf;
@@ -1029,11 +1028,12 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       coarse_total += (size_t)w.cw * w.ch;
     }
   }
-  std::vector<int> counts(2 * all_origins.size()), coarse(coarse_total);
+  std::vector<int> counts(2 * all_origins.size());
+  std::vector<uint64_t> coarse_hash;  // per coarse-mask image (ItemWork::coarse_entry)
+  uint8_t *coarse_d = coarse_total ? g_pool.take<uint8_t>(coarse_total) : nullptr;
   if (!all_origins.empty()) {
     int2 *origins_d = g_pool.take<int2>(all_origins.size());
     int *counts_d = g_pool.take<int>(2 * all_origins.size());
-    int *coarse_d = coarse_total ? g_pool.take<int>(coarse_total) : nullptr;
     // one launch each for every image: per-image geometry and the image of
     // every tile origin
     std::vector<PhItemGeo> geo(nitem);
@@ -1050,12 +1050,13 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       G.oy = w.oy;
       G.rx0 = w.rx0;
       G.ry0 = w.ry0;
-      if (w.any && tuned) {
+      if (w.any && w.ch > 0) {
         G.cf = w.cf;
         G.cw = w.cw;
         G.ch = w.ch;
         G.coarse_off = (int64_t)w.coarse_off;
-        if (w.ch > 0) coarse_rows.add(i, w.ch, i);
+        iw[i].coarse_entry = (int)coarse_rows.h.size();
+        coarse_rows.add(i, w.ch, i);
       }
       if (w.any)
         for (size_t k = 0; k < w.origins.size(); ++k) origin_item[w.origin_off + k] = i;
@@ -1070,17 +1071,24 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
                           hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_tile_occupancy_all, dim3(all_origins.size()), dim3(256), 0, st, geo_d,
                        oitem_d, origins_d, K, WV, hb, counts_d);
+    uint64_t *hash_d = nullptr;
     if (coarse_rows.rows > 0) {
-      PhSpan *coarse_rows_d = g_pool.take<PhSpan>(coarse_rows.h.size());
+      const int ne = (int)coarse_rows.h.size();
+      PhSpan *coarse_rows_d = g_pool.take<PhSpan>(ne);
+      uint64_t *row_hash_d = g_pool.take<uint64_t>(coarse_rows.rows);
+      hash_d = g_pool.take<uint64_t>(ne);
       coarse_rows.upload(coarse_rows_d, st);
       hipLaunchKernelGGL(k_coarse_count_all, dim3(coarse_rows.rows), dim3(256), 0, st, geo_d,
-                         coarse_rows_d, (int)coarse_rows.h.size(), coarse_d);
+                         coarse_rows_d, ne, coarse_d, row_hash_d);
+      hipLaunchKernelGGL(k_coarse_hash_items, dim3(ne), dim3(256), 0, st, coarse_rows_d,
+                         row_hash_d, hash_d);
+      coarse_hash.resize(ne);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * sizeof(int),
                           hipMemcpyDeviceToHost, st));
-    if (coarse_total)
-      HIP_OK(hipMemcpyAsync(coarse.data(), coarse_d, coarse.size() * sizeof(int),
+    if (hash_d)
+      HIP_OK(hipMemcpyAsync(coarse_hash.data(), hash_d, coarse_hash.size() * sizeof(uint64_t),
                             hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
   }
@@ -1108,18 +1116,27 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     for (int i = 0; i < nitem; ++i) {
       ItemWork &w = iw[i];
       if (!w.any) continue;
-      if (w.n_interior < PRECISE_TUNE_MIN_CELLS) {
+      if (w.n_interior < PRECISE_TUNE_MIN_CELLS || w.coarse_entry < 0) {
         mu_of[i] = mu_bound(w.n_interior, w.rw, w.rh);
         continue;
       }
       Job j;
       j.item = i;
-      j.key = coarse_mask(coarse.data() + w.coarse_off, w.cf, w.cw, w.ch, j.in);
+      j.key = TuneKey{w.cw, w.ch, w.cf, coarse_hash[w.coarse_entry]};
       j.mu = -1.0;
       if (tune_lookup(j.key, mu_of[i])) continue;
       jobs.push_back(std::move(j));
     }
     if (!jobs.empty()) {
+      // the masks not in the cache, from the device (one readback)
+      std::vector<uint8_t> masks(coarse_total);
+      HIP_OK(hipMemcpyAsync(masks.data(), coarse_d, coarse_total, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (Job &j : jobs) {
+        const ItemWork &w = iw[j.item];
+        j.in.assign(masks.begin() + w.coarse_off,
+                    masks.begin() + w.coarse_off + (size_t)w.cw * w.ch);
+      }
       const int nth = (int)std::min<size_t>(jobs.size(),
                                             std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
       std::atomic<size_t> next{0};
