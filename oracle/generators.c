@@ -19,7 +19,11 @@ uint64_t or_mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-/* noise.rs:86-109: fold (gx, gy) into the seed, top 24 bits -> [-1, 1]. */
+/* noise.rs:86-109: fold (gx, gy) into the seed, top 24 bits -> [-1, 1].
+ * Parity unpinned for the two fold multipliers: the reference holds no fbm
+ * golden values, and its fbm tests (range, position purity, value noise equal
+ * to the lattice value at integer points) hold for any odd multipliers. The
+ * finalizer (or_mix64) is pinned by SplitMix64's published vectors. */
 double or_lattice_value(int64_t gx, int64_t gy, uint64_t seed) {
   uint64_t ux = (uint64_t)gx;
   uint64_t uy = (uint64_t)gy;

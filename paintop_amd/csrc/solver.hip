@@ -65,7 +65,7 @@ __global__ void k_scatter_spans(const PhInst *, const PhSpan *, int, const PhIte
 __global__ void k_gather_hist(const PhHistCopy *, double *);
 __global__ void k_coarsen(const float *, const float *, const float *, const float *, int, int, int,
                           int, double, int, int, int, int, int, float *, float *, float *);
-__global__ void k_coarsen_groups(const PhInst *, const PhCorrArgs *, const PhSpan *,
+__global__ void k_coarsen_groups(const PhInst *, const PhCorrArgs *, const PhSpan *, int,
                                  const PhItemIO *, double, int, int, float *);
 __global__ void k_warm_apply_spans(const PhInst *, const PhSpan *, int, const PhCorrArgs *,
                                    const PhItemIO *, const float *, const float *, int);
@@ -78,6 +78,7 @@ __global__ void k_corr_apply_spans(const PhInst *, const PhCtl *, const PhSpan *
 __global__ void k_put_values(const PhValue *, int);
 __global__ void k_copy_anchor_spans(const PhInst *, const PhInst *, const PhSpan *, int);
 int coarsen_plane_cpb(int f);
+int coarsen_blocks(int f, int res);
 __global__ void k_in_norm_part(const float *, int, double *);
 __global__ void k_in_norm_apply(float *, int, const double *, float *);
 __global__ void k_corr_alpha_part(const float *, const float *, const double *, int, double *);
@@ -492,16 +493,10 @@ static uint32_t stall_window_for(double omega, bool tuned) {
   return (uint32_t)std::max<double>(PH_STALL_WINDOW, std::min(w, 1e6));
 }
 
-// One sweep launch: the tiles, then the per-instance control.
-static int sor_waves() {
-  const char *e = getenv("PAINTOP_HIP_SOR_WAVES");
-  const int w = e ? atoi(e) : 1;
-  return (w == 2 || w == 4 || w == 8) ? w : 1;
-}
+// One sweep launch: the tiles (one wave each), then the per-instance control.
 static void launch_sor(int K, int PF, int ntiles, int ninst, hipStream_t st, const PhTile *tiles,
                        const PhInst *insts, PhCtl *ctl, double *partials, int hb) {
-  const int W = sor_waves();
-  const dim3 grid((ntiles + W - 1) / W), block(64 * W), cgrid(ninst), cblock(64 * K);
+  const dim3 grid(ntiles), block(64), cgrid(ninst), cblock(64 * K);
 #define PH_SOR(KK, PP)                                                                      \
   hipLaunchKernelGGL((k_sor<KK, PP>), grid, block, 0, st, tiles, ntiles, insts, ctl, partials, hb)
   switch (K * 8 + PF) {
@@ -1359,18 +1354,22 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         warm_rows.add(k, inst[k].I.ph, inst[k].item);
       }
       // the coarsening takes an image's channels together: groups of up to 4
-      // consecutive entries of one image inside one network batch
+      // consecutive entries of one image inside one network batch, each with
+      // its blocks (row0: the group's first block within its batch's launch)
       warm_groups.clear();
-      std::vector<int> group0;  // first group of each batch
+      std::vector<int> group0, blocks0;  // per batch: first group, blocks of its launch
       for (int j0 = 0; j0 < nn; j0 += NNB) {
         group0.push_back((int)warm_groups.h.size());
         const int j1 = std::min(nn, j0 + NNB);
+        int nblk = 0;
         for (int j = j0; j < j1;) {
           int e = j + 1;
           while (e < j1 && e - j < 4 && warm_args[e].item == warm_args[j].item) ++e;
-          warm_groups.h.push_back(PhSpan{j, 0, e - j, 0, 0, warm_args[j].item, nullptr, 0.0});
+          warm_groups.h.push_back(PhSpan{j, nblk, e - j, 0, 0, warm_args[j].item, nullptr, 0.0});
+          nblk += coarsen_blocks(warm_args[j].f, NR);
           j = e;
         }
+        blocks0.push_back(nblk);
       }
       group0.push_back((int)warm_groups.h.size());
       warm_args_d = g_pool.take<PhCorrArgs>(nn);
@@ -1387,8 +1386,9 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
       side = side_stream();
       const int nbatch = (nn + NNB - 1) / NNB;
       if (nbatch == 1) {
-        hipLaunchKernelGGL(k_coarsen_groups, dim3(NR * NR, group0[1] - group0[0]), dim3(64), 0, st,
-                           inst_all_d, warm_args_d, warm_groups_d, io_d, d->lambda, NR, 0, net_in);
+        hipLaunchKernelGGL(k_coarsen_groups, dim3(blocks0[0]), dim3(256), 0, st, inst_all_d,
+                           warm_args_d, warm_groups_d, group0[1] - group0[0], io_d, d->lambda, NR,
+                           0, net_in);
         hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nn), dim3(256), 0, st, net_in, NR,
                            net_part);
         hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nn), dim3(256), 0, st, net_in, NR,
@@ -1400,9 +1400,9 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         for (int j0 = 0, bi = 0; j0 < nn; j0 += NNB, ++bi) {
           const int nb = std::min(NNB, nn - j0);
           if (nbatch > 1) {
-            hipLaunchKernelGGL(k_coarsen_groups, dim3(NR * NR, group0[bi + 1] - group0[bi]),
-                               dim3(64), 0, side, inst_all_d, warm_args_d,
-                               warm_groups_d + group0[bi], io_d, d->lambda, NR, j0, net_in);
+            hipLaunchKernelGGL(k_coarsen_groups, dim3(blocks0[bi]), dim3(256), 0, side,
+                               inst_all_d, warm_args_d, warm_groups_d + group0[bi],
+                               group0[bi + 1] - group0[bi], io_d, d->lambda, NR, j0, net_in);
             hipLaunchKernelGGL(k_in_norm_part, dim3(PH_NET_SLICES, nb), dim3(256), 0, side,
                                net_in, NR, net_part);
             hipLaunchKernelGGL(k_in_norm_apply, dim3(PH_NET_SLICES, nb), dim3(256), 0, side,

@@ -21,116 +21,181 @@ namespace ph {
 
 constexpr float WARM_MASK_THRESHOLD = 0.5f;  // poisson.rs:329
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 __device__ __forceinline__ bool warm_interior(const float *mask, int W, int H, int ix, int iy) {
   return ix >= 1 && ix < W - 1 && iy >= 1 && iy < H - 1 &&
          mask[(size_t)iy * W + ix] > WARM_MASK_THRESHOLD;
 }
 
-// One wave per coarse cell; lanes stride the block's columns, sums in a fixed
-// order (lane-local rows, then the butterfly), so results are deterministic.
+// Coarse cells per block of the coarsening kernels: whole cells of one
+// coarse row, f columns each, at most 256 columns (one per thread); a cell
+// wider than 256 columns takes a block of its own.
+__host__ __device__ __forceinline__ int coarsen_cells_per_block(int f) {
+  return f >= 256 ? 1 : 256 / f;
+}
+__host__ __device__ __forceinline__ int coarsen_blocks_per_row(int f, int res) {
+  const int c = coarsen_cells_per_block(f);
+  return (res + c - 1) / c;
+}
+
+// Sums of one coarse cell: interior and ring counts, and per channel the rhs
+// over interior cells, (anchor - P) over ring cells and (solution - P) over
+// both (the training target).
+template <int NE>
+struct CoarseSums {
+  int n_int = 0, n_ring = 0;
+  double rhs[NE] = {}, ring[NE] = {}, t[NE] = {};
+};
+
+// The f x f blocks (cells) cell0 .. cell0 + ncell - 1 of coarse row cy of the
+// region [x0, x0 + res*f) x [y0, y0 + res*f), for channels ch[0 .. ne) (ne <= NE)
+// of an image; thread c < ncell returns cell c's sums. Thread t sums plane
+// column t of the block down the cells' f rows (in row order), the column sums
+// meet in LDS, and thread c adds its cell's f columns in column order: a fixed
+// order, the same for every launch shape, so results are deterministic and
+// a channel's sums do not depend on which other channels share the block.
 // Ring cells are the non-interior cells with an interior 4-neighbour: the
 // Dirichlet data of the solve (poisson.rs:374-392 reads them as neighbours).
 // mode 0 (warm start): P = source, the rhs is lambda (P - anchor).
 // mode 1 (correction problem): P = 0 and the rhs is the 5-point Laplacian of
 // the source (poisson.rs:364-373): a blend with a zero anchor is the problem
 // Laplacian(u) = rhs, u = 0 on the ring, that a coarse correction solves.
-__device__ __forceinline__ void coarsen_cell(const float *__restrict__ src,
-                                             const float *__restrict__ anc,
-                                             const float *__restrict__ mask,
-                                             const float *__restrict__ sol, int W, int H, int C,
-                                             int ch, double lambda, int x0, int y0, int f,
-                                             int res, int mode, float *__restrict__ in4,
-                                             float *__restrict__ tgt, float *__restrict__ wt,
-                                             int c) {
-  const int cx = c % res, cy = c / res;
-  const int lane = threadIdx.x;
-  double s_rhs = 0.0, s_ring = 0.0, n_int = 0.0, n_ring = 0.0, s_t = 0.0;
-  // the block's f*f cells, row-major, strided over the lanes (every lane busy
-  // for any f)
-  for (int i = lane; i < f * f; i += 64) {
-    const int dy = i / f, dx = i - dy * f;
-    const int iy = y0 + cy * f + dy;
-    const int ix = x0 + cx * f + dx;
-    if (iy < 0 || iy >= H || ix < 0 || ix >= W) continue;
-    {
-      const size_t q = (size_t)iy * W + ix;
-      const double S = (double)src[q * C + ch], A = (double)anc[q * C + ch];
-      const double P = mode ? 0.0 : S;
-      if (warm_interior(mask, W, H, ix, iy)) {
-        n_int += 1.0;
-        if (mode) {
-          // interior cells are never on the image border: four in-image neighbours
-          const double l = src[(q - 1) * C + ch], r = src[(q + 1) * C + ch];
-          const double u = src[(q - W) * C + ch], d = src[(q + W) * C + ch];
-          s_rhs += fma(4.0, -S, ((l + r) + u) + d);
-        } else {
-          s_rhs += lambda * (P - A);
+template <int NE, bool TGT>
+__device__ __forceinline__ CoarseSums<NE> coarsen_block(
+    const float *__restrict__ src, const float *__restrict__ anc, const float *__restrict__ mask,
+    const float *__restrict__ sol, int W, int H, int C, const int (&ch)[NE], int ne,
+    double lambda, int x0, int y0, int f, int mode, int cy, int cell0, int ncell) {
+  __shared__ int s_ni[256], s_nr[256];
+  __shared__ double s_rhs[NE][256], s_ring[NE][256], s_t[TGT ? NE : 1][256];
+  CoarseSums<NE> cell;
+  const int t = threadIdx.x;
+  const int ncol = ncell * f;
+  const int ix0 = x0 + cell0 * f, iy0 = y0 + cy * f;
+  const int dy0 = max(0, -iy0), dy1 = min(f, H - iy0);
+  for (int c0 = 0; c0 < ncol; c0 += 256) {
+    const int col = c0 + t;
+    const int ix = ix0 + col;
+    CoarseSums<NE> cs;
+    if (col < ncol && ix >= 0 && ix < W) {
+      for (int dy = dy0; dy < dy1; ++dy) {
+        const int iy = iy0 + dy;
+        const size_t q = (size_t)iy * W + ix;
+        if (warm_interior(mask, W, H, ix, iy)) {
+          cs.n_int += 1;
+#pragma unroll
+          for (int e = 0; e < NE; ++e)
+            if (e < ne) {
+              const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
+              const double P = mode ? 0.0 : S;
+              if (mode) {
+                // interior cells are never on the image border: four in-image neighbours
+                const double l = src[(q - 1) * C + ch[e]], r = src[(q + 1) * C + ch[e]];
+                const double u = src[(q - W) * C + ch[e]], d = src[(q + W) * C + ch[e]];
+                cs.rhs[e] += fma(4.0, -S, ((l + r) + u) + d);
+              } else {
+                cs.rhs[e] += lambda * (P - A);
+              }
+              if (TGT) cs.t[e] += (double)sol[q * C + ch[e]] - P;
+            }
+        } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
+                   warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
+          cs.n_ring += 1;
+#pragma unroll
+          for (int e = 0; e < NE; ++e)
+            if (e < ne) {
+              const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
+              const double P = mode ? 0.0 : S;
+              cs.ring[e] += A - P;
+              if (TGT) cs.t[e] += A - P;
+            }
         }
-        if (sol) s_t += (double)sol[q * C + ch] - P;
-      } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
-                 warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
-        n_ring += 1.0;
-        s_ring += A - P;
-        if (sol) s_t += A - P;
       }
     }
+    s_ni[t] = cs.n_int;
+    s_nr[t] = cs.n_ring;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      s_rhs[e][t] = cs.rhs[e];
+      s_ring[e][t] = cs.ring[e];
+      if (TGT) s_t[TGT ? e : 0][t] = cs.t[e];
+    }
+    __syncthreads();
+    if (t < ncell) {
+      const int a = max(t * f, c0) - c0, b = min(t * f + f, c0 + 256) - c0;
+      for (int j = a; j < b; ++j) {
+        cell.n_int += s_ni[j];
+        cell.n_ring += s_nr[j];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          cell.rhs[e] += s_rhs[e][j];
+          cell.ring[e] += s_ring[e][j];
+          if (TGT) cell.t[e] += s_t[TGT ? e : 0][j];
+        }
+      }
+    }
+    __syncthreads();
   }
-  s_rhs = wave_sum_d(s_rhs);
-  s_ring = wave_sum_d(s_ring);
-  n_int = wave_sum_d(n_int);
-  n_ring = wave_sum_d(n_ring);
-  s_t = wave_sum_d(s_t);
-  if (lane == 0) {
-    const double inv = 1.0 / ((double)f * f);
-    in4[(size_t)c * 4 + 0] = (float)s_rhs;  // f^2 * mean
-    in4[(size_t)c * 4 + 1] = n_ring > 0.0 ? (float)(s_ring / n_ring) : 0.0f;
-    in4[(size_t)c * 4 + 2] = (float)(n_int * inv);
-    in4[(size_t)c * 4 + 3] = (float)(n_ring / f);
-    const double nt = n_int + n_ring;
-    if (tgt) tgt[c] = nt > 0.0 ? (float)(s_t / nt) : 0.0f;
-    if (wt) wt[c] = n_int > 0.0 ? 1.0f : (n_ring > 0.0 ? 0.5f : 0.0f);
-  }
+  return cell;
 }
 
-__global__ __launch_bounds__(64) void k_coarsen(const float *__restrict__ src,
-                                                const float *__restrict__ anc,
-                                                const float *__restrict__ mask,
-                                                const float *__restrict__ sol, int W, int H, int C,
-                                                int ch, double lambda, int x0, int y0, int f,
-                                                int res, int mode, float *__restrict__ in4,
-                                                float *__restrict__ tgt, float *__restrict__ wt) {
-  coarsen_cell(src, anc, mask, sol, W, H, C, ch, lambda, x0, y0, f, res, mode, in4, tgt, wt,
-               blockIdx.x);
+// The four input channels of one cell (laid out [res][res][4]).
+template <int NE>
+__device__ __forceinline__ void coarsen_put(float *__restrict__ o, const CoarseSums<NE> &s, int e,
+                                            int f) {
+  const double n_int = s.n_int, n_ring = s.n_ring;
+  o[0] = (float)s.rhs[e];  // f^2 * mean
+  o[1] = n_ring > 0.0 ? (float)(s.ring[e] / n_ring) : 0.0f;
+  o[2] = (float)(n_int * (1.0 / ((double)f * f)));
+  o[3] = (float)(n_ring / f);
+}
+
+// One channel: grid res * coarsen_blocks_per_row(f, res) blocks of 256.
+__global__ __launch_bounds__(256) void k_coarsen(const float *__restrict__ src,
+                                                 const float *__restrict__ anc,
+                                                 const float *__restrict__ mask,
+                                                 const float *__restrict__ sol, int W, int H,
+                                                 int C, int ch, double lambda, int x0, int y0,
+                                                 int f, int res, int mode, float *__restrict__ in4,
+                                                 float *__restrict__ tgt, float *__restrict__ wt) {
+  const int nbx = coarsen_blocks_per_row(f, res), cpb = coarsen_cells_per_block(f);
+  const int cy = blockIdx.x / nbx, cell0 = (blockIdx.x - cy * nbx) * cpb;
+  const int ncell = min(cpb, res - cell0);
+  const int chs[1] = {ch};
+  CoarseSums<1> s;
+  if (sol)
+    s = coarsen_block<1, true>(src, anc, mask, sol, W, H, C, chs, 1, lambda, x0, y0, f, mode, cy,
+                               cell0, ncell);
+  else
+    s = coarsen_block<1, false>(src, anc, mask, nullptr, W, H, C, chs, 1, lambda, x0, y0, f,
+                                mode, cy, cell0, ncell);
+  if ((int)threadIdx.x >= ncell) return;
+  const size_t c = (size_t)cy * res + cell0 + threadIdx.x;
+  coarsen_put(in4 + c * 4, s, 0, f);
+  if (tgt) {
+    const double nt = (double)s.n_int + s.n_ring;
+    tgt[c] = nt > 0.0 ? (float)(s.t[0] / nt) : 0.0f;
+  }
+  if (wt) wt[c] = s.n_int > 0 ? 1.0f : (s.n_ring > 0 ? 0.5f : 0.0f);
 }
 
 // The warm-start inputs (mode 0) of every network instance of a batch in one
-// launch, the channels of an image together: blockIdx.y = group, a run of up
-// to 4 consecutive entries args[slot .. slot + nrows) of one image (item), so
-// each pixel's channels and its interior and ring tests are read once for the
-// group. Per channel the sums run in k_coarsen's order (lane-local, then the
-// butterfly), so the inputs equal k_coarsen's. Entry j's input goes to
-// in4 + res*res*4*(net - j0).
-__global__ __launch_bounds__(64) void k_coarsen_groups(const PhInst *__restrict__ insts,
-                                                       const PhCorrArgs *__restrict__ args,
-                                                       const PhSpan *__restrict__ groups,
-                                                       const PhItemIO *__restrict__ io,
-                                                       double lambda, int res, int j0,
-                                                       float *__restrict__ in4) {
-  const PhSpan Gs = groups[blockIdx.y];
+// launch, the channels of an image together: a group (span entry) is a run of
+// up to 4 consecutive entries args[slot .. slot + nrows) of one image (item),
+// so each pixel's channels and its interior and ring tests are read once for
+// the group; its blocks are row0 .. row0 + res * coarsen_blocks_per_row(f, res) - 1.
+// A channel's inputs equal k_coarsen's bit for bit (the same sums in the same
+// order). Entry j's input goes to in4 + res*res*4*(net - j0).
+__global__ __launch_bounds__(256) void k_coarsen_groups(const PhInst *__restrict__ insts,
+                                                        const PhCorrArgs *__restrict__ args,
+                                                        const PhSpan *__restrict__ groups,
+                                                        int ngroups,
+                                                        const PhItemIO *__restrict__ io,
+                                                        double lambda, int res, int j0,
+                                                        float *__restrict__ in4) {
+  const PhSpan Gs = groups[ph_span_find(groups, ngroups, blockIdx.x)];
   const PhItemIO &G = io[Gs.item];
   const int ne = Gs.nrows;
   const PhCorrArgs A0 = args[Gs.slot];
-  const int x0 = A0.x0, y0 = A0.y0, f = A0.f;
-  const int W = G.W, H = G.H, C = G.C;
-  const float *__restrict__ src = G.src;
-  const float *__restrict__ anc = G.tgt;
-  const float *__restrict__ mask = G.mask;
+  const int f = A0.f;
   int ch[4] = {0, 0, 0, 0}, net[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int e = 0; e < 4; ++e)
@@ -139,64 +204,18 @@ __global__ __launch_bounds__(64) void k_coarsen_groups(const PhInst *__restrict_
       ch[e] = insts[A.slot].channel;
       net[e] = A.net;
     }
-  const int c = blockIdx.x;
-  const int cx = c % res, cy = c / res;
-  const int lane = threadIdx.x;
-  // a coarse cell wholly outside the image (small images leave most of the
-  // grid empty) has no interior or ring cell: its inputs are all zero
-  if (x0 + cx * f >= W || y0 + cy * f >= H || x0 + cx * f + f <= 0 || y0 + cy * f + f <= 0) {
-    if (lane < ne) {
-      const int nl = lane == 0 ? net[0] : lane == 1 ? net[1] : lane == 2 ? net[2] : net[3];
-      float *o = in4 + ((size_t)(nl - j0) * res * res + c) * 4;
-      o[0] = 0.0f;
-      o[1] = 0.0f;
-      o[2] = 0.0f;
-      o[3] = 0.0f;
-    }
-    return;
-  }
-  double s_rhs[4] = {0.0, 0.0, 0.0, 0.0}, s_ring[4] = {0.0, 0.0, 0.0, 0.0};
-  double n_int = 0.0, n_ring = 0.0;
-  for (int i = lane; i < f * f; i += 64) {
-    const int dy = i / f, dx = i - dy * f;
-    const int iy = y0 + cy * f + dy;
-    const int ix = x0 + cx * f + dx;
-    if (iy < 0 || iy >= H || ix < 0 || ix >= W) continue;
-    const size_t q = (size_t)iy * W + ix;
-    if (warm_interior(mask, W, H, ix, iy)) {
-      n_int += 1.0;
+  const int nbx = coarsen_blocks_per_row(f, res), cpb = coarsen_cells_per_block(f);
+  const int local = blockIdx.x - Gs.row0;
+  const int cy = local / nbx, cell0 = (local - cy * nbx) * cpb;
+  const int ncell = min(cpb, res - cell0);
+  const CoarseSums<4> s = coarsen_block<4, false>(G.src, G.tgt, G.mask, nullptr, G.W, G.H, G.C,
+                                                  ch, ne, lambda, A0.x0, A0.y0, f, 0, cy, cell0,
+                                                  ncell);
+  if ((int)threadIdx.x >= ncell) return;
+  const size_t c = (size_t)cy * res + cell0 + threadIdx.x;
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (e < ne) {
-          const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
-          s_rhs[e] += lambda * (S - A);
-        }
-    } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
-               warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
-      n_ring += 1.0;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (e < ne) {
-          const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
-          s_ring[e] += A - S;
-        }
-    }
-  }
-  n_int = wave_sum_d(n_int);
-  n_ring = wave_sum_d(n_ring);
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    if (e >= ne) break;
-    const double sr = wave_sum_d(s_rhs[e]);
-    const double sg = wave_sum_d(s_ring[e]);
-    if (lane == 0) {
-      float *o = in4 + ((size_t)(net[e] - j0) * res * res + c) * 4;
-      o[0] = (float)sr;  // f^2 * mean
-      o[1] = n_ring > 0.0 ? (float)(sg / n_ring) : 0.0f;
-      o[2] = (float)(n_int * (1.0 / ((double)f * f)));
-      o[3] = (float)(n_ring / f);
-    }
-  }
+  for (int e = 0; e < 4; ++e)
+    if (e < ne) coarsen_put(in4 + ((size_t)(net[e] - j0) * res * res + c) * 4, s, e, f);
 }
 
 // Net input of a coarse correction, from an instance's plane: per block the
@@ -459,6 +478,7 @@ __global__ __launch_bounds__(256) void k_corr_input_list(const PhInst *__restric
 // 256 cells (one finishing thread per cell). The result does not depend on it:
 // a cell's columns are always added in column order by one thread.
 int coarsen_plane_cpb(int f) { return std::max(1, std::min(256, 1024 / f)); }
+int coarsen_blocks(int f, int res) { return res * coarsen_blocks_per_row(f, res); }
 
 // The problems are linear: each net input is scaled so that its rhs and
 // boundary channels have unit RMS over the grid (scale[b] = that RMS, 1 when
@@ -659,7 +679,8 @@ extern "C" ph_status ph_warm_coarsen_device(const float *d_source, const float *
                  factor, res, (int)mode);
     if (d_solution && (!d_target || !d_weight))
       throw fail(PH_CLASS_SCHEMA, "E_HIP_ARGUMENT", "coarsen: target/weight buffers required");
-    hipLaunchKernelGGL(k_coarsen, dim3(res * res), dim3(64), 0, (hipStream_t)stream, d_source,
+    hipLaunchKernelGGL(k_coarsen, dim3(res * coarsen_blocks_per_row(factor, res)), dim3(256), 0,
+                       (hipStream_t)stream, d_source,
                        d_anchor, d_mask, d_solution, (int)width, (int)height, (int)channels,
                        (int)channel, lambda, x0, y0, factor, res, (int)mode, d_in,
                        d_solution ? d_target : nullptr, d_solution ? d_weight : nullptr);

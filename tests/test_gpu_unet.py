@@ -117,18 +117,24 @@ def test_training_fits_a_fixed_batch(gpu):
     net.close()
 
 
+@pytest.mark.parametrize("H,W,R,f,x0,y0", [
+    (70, 90, 16, 5, -7, -3),      # several whole cells per block
+    (40, 50, 64, 1, -5, -2),      # one pixel per cell, a block spans the coarse row
+    (130, 150, 8, 33, 3, -4),     # 7 cells per block: a row in two blocks, the last one partial
+    (400, 500, 2, 300, -20, -10),  # cells wider than a block: columns in chunks of 256
+])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_coarsen_matches_numpy(gpu, mode):
+def test_coarsen_matches_numpy(gpu, mode, H, W, R, f, x0, y0):
     from paintop_amd.device import DeviceArray
     from tests.helpers import classify
 
     rng = np.random.default_rng(2)
-    H, W, C, R, f = 70, 90, 3, 16, 5
+    C = 3
     s = rng.random((H, W, C)).astype(np.float32)
     t = rng.random((H, W, C)).astype(np.float32)
     u = rng.random((H, W, C)).astype(np.float32)
     m = (rng.random((H, W)) > 0.4).astype(np.float32)
-    x0, y0, ch, lam = -7, -3, 1, 0.25
+    ch, lam = 1, 0.25
     ds, dt, dm, du = _dev(s), _dev(t), _dev(m), _dev(u)
     din = DeviceArray((R, R, 4), np.float32)
     dtg = DeviceArray((R, R), np.float32)
