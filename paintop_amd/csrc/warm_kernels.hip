@@ -26,11 +26,16 @@ __device__ __forceinline__ bool warm_interior(const float *mask, int W, int H, i
          mask[(size_t)iy * W + ix] > WARM_MASK_THRESHOLD;
 }
 
-// Coarse cells per block of the coarsening kernels: whole cells of one
-// coarse row, f columns each, at most 256 columns (one per thread); a cell
-// wider than 256 columns takes a block of its own.
+// Cells of the coarsening kernels' blocks (256 threads). Below
+// COARSEN_WAVE_F a block takes whole cells of one coarse row, f columns
+// each, at most 256 columns (one thread per column); from COARSEN_WAVE_F on
+// a cell has f^2 >= 256 pixels and takes a wave of its own (4 cells a block),
+// whose lanes stride its pixels: a thread's chain of dependent loads stays
+// short either way (a column of 30 rows per thread ran 25% slower than a wave
+// per 30 x 30 cell).
+constexpr int COARSEN_WAVE_F = 16;
 __host__ __device__ __forceinline__ int coarsen_cells_per_block(int f) {
-  return f >= 256 ? 1 : 256 / f;
+  return f >= COARSEN_WAVE_F ? 4 : 256 / f;
 }
 __host__ __device__ __forceinline__ int coarsen_blocks_per_row(int f, int res) {
   const int c = coarsen_cells_per_block(f);
@@ -41,100 +46,154 @@ __host__ __device__ __forceinline__ int coarsen_blocks_per_row(int f, int res) {
 // over interior cells, (anchor - P) over ring cells and (solution - P) over
 // both (the training target).
 template <int NE>
+struct CoarseChannels {
+  int c[NE];
+};
+
+template <int NE>
 struct CoarseSums {
   int n_int = 0, n_ring = 0;
   double rhs[NE] = {}, ring[NE] = {}, t[NE] = {};
 };
 
-// The f x f blocks (cells) cell0 .. cell0 + ncell - 1 of coarse row cy of the
-// region [x0, x0 + res*f) x [y0, y0 + res*f), for channels ch[0 .. ne) (ne <= NE)
-// of an image; thread c < ncell returns cell c's sums. Thread t sums plane
-// column t of the block down the cells' f rows (in row order), the column sums
-// meet in LDS, and thread c adds its cell's f columns in column order: a fixed
-// order, the same for every launch shape, so results are deterministic and
-// a channel's sums do not depend on which other channels share the block.
+// Adds pixel (ix, iy) (inside the image) to s, for channels ch[0 .. NE).
 // Ring cells are the non-interior cells with an interior 4-neighbour: the
 // Dirichlet data of the solve (poisson.rs:374-392 reads them as neighbours).
 // mode 0 (warm start): P = source, the rhs is lambda (P - anchor).
 // mode 1 (correction problem): P = 0 and the rhs is the 5-point Laplacian of
 // the source (poisson.rs:364-373): a blend with a zero anchor is the problem
 // Laplacian(u) = rhs, u = 0 on the ring, that a coarse correction solves.
-template <int NE, bool TGT>
-__device__ __forceinline__ CoarseSums<NE> coarsen_block(
-    const float *__restrict__ src, const float *__restrict__ anc, const float *__restrict__ mask,
-    const float *__restrict__ sol, int W, int H, int C, const int (&ch)[NE], int ne,
-    double lambda, int x0, int y0, int f, int mode, int cy, int cell0, int ncell) {
-  __shared__ int s_ni[256], s_nr[256];
-  __shared__ double s_rhs[NE][256], s_ring[NE][256], s_t[TGT ? NE : 1][256];
-  CoarseSums<NE> cell;
+template <int NE, bool TGT, int MODE>
+__device__ __forceinline__ void coarsen_pixel(CoarseSums<NE> &cs, const float *__restrict__ src,
+                                              const float *__restrict__ anc,
+                                              const float *__restrict__ mask,
+                                              const float *__restrict__ sol, int W, int H, int C,
+                                              CoarseChannels<NE> ch, double lambda, int ix,
+                                              int iy) {
+  const size_t q = (size_t)iy * W + ix;
+  const bool in = warm_interior(mask, W, H, ix, iy);
+  const bool rg = !in && (warm_interior(mask, W, H, ix - 1, iy) ||
+                          warm_interior(mask, W, H, ix + 1, iy) ||
+                          warm_interior(mask, W, H, ix, iy - 1) ||
+                          warm_interior(mask, W, H, ix, iy + 1));
+  cs.n_int += in ? 1 : 0;
+  cs.n_ring += rg ? 1 : 0;
+  if (!(in || rg)) return;
+  // every sum takes a term (0 where the pixel does not count: the sums start
+  // at +0, so an added +0 changes no bit); adds into one array or the other
+  // under a branch would be merged into one add through a selected address,
+  // which puts the sums in scratch memory
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const double S = (double)src[q * C + ch.c[e]], A = (double)anc[q * C + ch.c[e]];
+    const double P = MODE ? 0.0 : S;
+    double vr = 0.0;
+    if constexpr (MODE) {
+      if (in) {
+        // interior cells are never on the image border: four in-image neighbours
+        const double l = src[(q - 1) * C + ch.c[e]], r = src[(q + 1) * C + ch.c[e]];
+        const double u = src[(q - W) * C + ch.c[e]], d = src[(q + W) * C + ch.c[e]];
+        vr = fma(4.0, -S, ((l + r) + u) + d);
+      }
+    } else {
+      vr = lambda * (P - A);
+    }
+    cs.rhs[e] += in ? vr : 0.0;
+    cs.ring[e] += rg ? A - P : 0.0;
+    if (TGT) cs.t[e] += in ? (double)sol[q * C + ch.c[e]] - P : A - P;
+  }
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// LDS of a coarsening block: the column sums of its 256 threads (ints: the
+// two counts; doubles: up to 3 sums for each of up to 4 channels).
+struct CoarseLds {
+  int n[2][256];
+  double d[12][256];
+};
+
+// The f x f blocks (cells) cell0 .. cell0 + ncell - 1 of coarse row cy of the
+// region [x0, x0 + res*f) x [y0, y0 + res*f), for channels ch[0 .. NE) of an
+// image; put(c, sums) receives cell c's sums, once per cell, from one thread.
+// f < COARSEN_WAVE_F: thread t sums plane column t of the block down the
+// cells' f rows (in row order), the column sums meet in LDS, and thread c
+// adds its cell's f columns in column order. Otherwise wave w takes cell w:
+// lane l adds the cell's pixels l, l + 64, ... (row-major), and a fixed
+// butterfly adds the lanes. Either order is fixed for a given f, the same for
+// every launch of either kernel, so results are deterministic and a channel's
+// sums do not depend on which other channels share the block.
+template <int NE, bool TGT, int MODE, class Put>
+__device__ __forceinline__ void coarsen_block(
+    CoarseLds &lds, const float *__restrict__ src, const float *__restrict__ anc,
+    const float *__restrict__ mask, const float *__restrict__ sol, int W, int H, int C,
+    CoarseChannels<NE> ch, double lambda, int x0, int y0, int f, int cy, int cell0, int ncell,
+    const Put &put) {
   const int t = threadIdx.x;
-  const int ncol = ncell * f;
   const int ix0 = x0 + cell0 * f, iy0 = y0 + cy * f;
-  const int dy0 = max(0, -iy0), dy1 = min(f, H - iy0);
-  for (int c0 = 0; c0 < ncol; c0 += 256) {
-    const int col = c0 + t;
-    const int ix = ix0 + col;
+  if (f >= COARSEN_WAVE_F) {
+    const int w = t >> 6, lane = t & 63;
     CoarseSums<NE> cs;
-    if (col < ncol && ix >= 0 && ix < W) {
-      for (int dy = dy0; dy < dy1; ++dy) {
-        const int iy = iy0 + dy;
-        const size_t q = (size_t)iy * W + ix;
-        if (warm_interior(mask, W, H, ix, iy)) {
-          cs.n_int += 1;
-#pragma unroll
-          for (int e = 0; e < NE; ++e)
-            if (e < ne) {
-              const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
-              const double P = mode ? 0.0 : S;
-              if (mode) {
-                // interior cells are never on the image border: four in-image neighbours
-                const double l = src[(q - 1) * C + ch[e]], r = src[(q + 1) * C + ch[e]];
-                const double u = src[(q - W) * C + ch[e]], d = src[(q + W) * C + ch[e]];
-                cs.rhs[e] += fma(4.0, -S, ((l + r) + u) + d);
-              } else {
-                cs.rhs[e] += lambda * (P - A);
-              }
-              if (TGT) cs.t[e] += (double)sol[q * C + ch[e]] - P;
-            }
-        } else if (warm_interior(mask, W, H, ix - 1, iy) || warm_interior(mask, W, H, ix + 1, iy) ||
-                   warm_interior(mask, W, H, ix, iy - 1) || warm_interior(mask, W, H, ix, iy + 1)) {
-          cs.n_ring += 1;
-#pragma unroll
-          for (int e = 0; e < NE; ++e)
-            if (e < ne) {
-              const double S = (double)src[q * C + ch[e]], A = (double)anc[q * C + ch[e]];
-              const double P = mode ? 0.0 : S;
-              cs.ring[e] += A - P;
-              if (TGT) cs.t[e] += A - P;
-            }
-        }
+    if (w < ncell) {
+      for (int i = lane; i < f * f; i += 64) {
+        const int dy = i / f, dx = i - dy * f;
+        const int iy = iy0 + dy, ix = ix0 + w * f + dx;
+        if (iy < 0 || iy >= H || ix < 0 || ix >= W) continue;
+        coarsen_pixel<NE, TGT, MODE>(cs, src, anc, mask, sol, W, H, C, ch, lambda, ix, iy);
       }
     }
-    s_ni[t] = cs.n_int;
-    s_nr[t] = cs.n_ring;
+    cs.n_int = wave_sum_i(cs.n_int);
+    cs.n_ring = wave_sum_i(cs.n_ring);
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      s_rhs[e][t] = cs.rhs[e];
-      s_ring[e][t] = cs.ring[e];
-      if (TGT) s_t[TGT ? e : 0][t] = cs.t[e];
+      cs.rhs[e] = wave_sum_d(cs.rhs[e]);
+      cs.ring[e] = wave_sum_d(cs.ring[e]);
+      if (TGT) cs.t[e] = wave_sum_d(cs.t[e]);
     }
-    __syncthreads();
-    if (t < ncell) {
-      const int a = max(t * f, c0) - c0, b = min(t * f + f, c0 + 256) - c0;
-      for (int j = a; j < b; ++j) {
-        cell.n_int += s_ni[j];
-        cell.n_ring += s_nr[j];
+    if (lane == 0 && w < ncell) put(w, cs);
+    return;
+  }
+  const int ncol = ncell * f;  // <= 256
+  const int dy0 = max(0, -iy0), dy1 = min(f, H - iy0);
+  {
+    const int ix = ix0 + t;
+    CoarseSums<NE> cs;
+    if (t < ncol && ix >= 0 && ix < W)
+      for (int dy = dy0; dy < dy1; ++dy)
+        coarsen_pixel<NE, TGT, MODE>(cs, src, anc, mask, sol, W, H, C, ch, lambda, ix, iy0 + dy);
+    lds.n[0][t] = cs.n_int;
+    lds.n[1][t] = cs.n_ring;
 #pragma unroll
-        for (int e = 0; e < NE; ++e) {
-          cell.rhs[e] += s_rhs[e][j];
-          cell.ring[e] += s_ring[e][j];
-          if (TGT) cell.t[e] += s_t[TGT ? e : 0][j];
-        }
+    for (int e = 0; e < NE; ++e) {
+      lds.d[3 * e][t] = cs.rhs[e];
+      lds.d[3 * e + 1][t] = cs.ring[e];
+      if (TGT) lds.d[3 * e + 2][t] = cs.t[e];
+    }
+  }
+  __syncthreads();
+  if (t < ncell) {
+    CoarseSums<NE> cell;
+    for (int j = t * f; j < t * f + f; ++j) {
+      cell.n_int += lds.n[0][j];
+      cell.n_ring += lds.n[1][j];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        cell.rhs[e] += lds.d[3 * e][j];
+        cell.ring[e] += lds.d[3 * e + 1][j];
+        if (TGT) cell.t[e] += lds.d[3 * e + 2][j];
       }
     }
-    __syncthreads();
+    put(t, cell);
   }
-  return cell;
 }
 
 // The four input channels of one cell (laid out [res][res][4]).
@@ -148,6 +207,30 @@ __device__ __forceinline__ void coarsen_put(float *__restrict__ o, const CoarseS
   o[3] = (float)(n_ring / f);
 }
 
+template <bool TGT, int MODE>
+__device__ __forceinline__ void coarsen_one(CoarseLds &lds, const float *__restrict__ src,
+                                            const float *__restrict__ anc,
+                                            const float *__restrict__ mask,
+                                            const float *__restrict__ sol, int W, int H, int C,
+                                            int ch, double lambda, int x0, int y0, int f, int res,
+                                            float *__restrict__ in4, float *__restrict__ tgt,
+                                            float *__restrict__ wt) {
+  const int nbx = coarsen_blocks_per_row(f, res), cpb = coarsen_cells_per_block(f);
+  const int cy = blockIdx.x / nbx, cell0 = (blockIdx.x - cy * nbx) * cpb;
+  const int ncell = min(cpb, res - cell0);
+  const CoarseChannels<1> chs = {{ch}};
+  coarsen_block<1, TGT, MODE>(lds, src, anc, mask, sol, W, H, C, chs, lambda, x0, y0, f, cy,
+                              cell0, ncell, [&](int k, const CoarseSums<1> &s) {
+    const size_t c = (size_t)cy * res + cell0 + k;
+    coarsen_put(in4 + c * 4, s, 0, f);
+    if (TGT) {
+      const double nt = (double)s.n_int + s.n_ring;
+      if (tgt) tgt[c] = nt > 0.0 ? (float)(s.t[0] / nt) : 0.0f;
+      if (wt) wt[c] = s.n_int > 0 ? 1.0f : (s.n_ring > 0 ? 0.5f : 0.0f);
+    }
+  });
+}
+
 // One channel: grid res * coarsen_blocks_per_row(f, res) blocks of 256.
 __global__ __launch_bounds__(256) void k_coarsen(const float *__restrict__ src,
                                                  const float *__restrict__ anc,
@@ -156,25 +239,17 @@ __global__ __launch_bounds__(256) void k_coarsen(const float *__restrict__ src,
                                                  int C, int ch, double lambda, int x0, int y0,
                                                  int f, int res, int mode, float *__restrict__ in4,
                                                  float *__restrict__ tgt, float *__restrict__ wt) {
-  const int nbx = coarsen_blocks_per_row(f, res), cpb = coarsen_cells_per_block(f);
-  const int cy = blockIdx.x / nbx, cell0 = (blockIdx.x - cy * nbx) * cpb;
-  const int ncell = min(cpb, res - cell0);
-  const int chs[1] = {ch};
-  CoarseSums<1> s;
-  if (sol)
-    s = coarsen_block<1, true>(src, anc, mask, sol, W, H, C, chs, 1, lambda, x0, y0, f, mode, cy,
-                               cell0, ncell);
+  __shared__ CoarseLds lds;
+  if (sol && mode)
+    coarsen_one<true, 1>(lds, src, anc, mask, sol, W, H, C, ch, lambda, x0, y0, f, res, in4, tgt, wt);
+  else if (sol)
+    coarsen_one<true, 0>(lds, src, anc, mask, sol, W, H, C, ch, lambda, x0, y0, f, res, in4, tgt, wt);
+  else if (mode)
+    coarsen_one<false, 1>(lds, src, anc, mask, nullptr, W, H, C, ch, lambda, x0, y0, f, res, in4,
+                          nullptr, nullptr);
   else
-    s = coarsen_block<1, false>(src, anc, mask, nullptr, W, H, C, chs, 1, lambda, x0, y0, f,
-                                mode, cy, cell0, ncell);
-  if ((int)threadIdx.x >= ncell) return;
-  const size_t c = (size_t)cy * res + cell0 + threadIdx.x;
-  coarsen_put(in4 + c * 4, s, 0, f);
-  if (tgt) {
-    const double nt = (double)s.n_int + s.n_ring;
-    tgt[c] = nt > 0.0 ? (float)(s.t[0] / nt) : 0.0f;
-  }
-  if (wt) wt[c] = s.n_int > 0 ? 1.0f : (s.n_ring > 0 ? 0.5f : 0.0f);
+    coarsen_one<false, 0>(lds, src, anc, mask, nullptr, W, H, C, ch, lambda, x0, y0, f, res, in4,
+                          nullptr, nullptr);
 }
 
 // The warm-start inputs (mode 0) of every network instance of a batch in one
@@ -184,6 +259,35 @@ __global__ __launch_bounds__(256) void k_coarsen(const float *__restrict__ src,
 // the group; its blocks are row0 .. row0 + res * coarsen_blocks_per_row(f, res) - 1.
 // A channel's inputs equal k_coarsen's bit for bit (the same sums in the same
 // order). Entry j's input goes to in4 + res*res*4*(net - j0).
+template <int NE>
+__device__ __forceinline__ void coarsen_group(CoarseLds &lds, const PhInst *__restrict__ insts,
+                                              const PhCorrArgs *__restrict__ args, const PhSpan &Gs,
+                                              const PhItemIO &G, double lambda, int res, int j0,
+                                              float *__restrict__ in4) {
+  const PhCorrArgs A0 = args[Gs.slot];
+  const int f = A0.f;
+  CoarseChannels<NE> ch;
+  int net[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const PhCorrArgs A = args[Gs.slot + e];
+    ch.c[e] = insts[A.slot].channel;
+    net[e] = A.net;
+  }
+  const int nbx = coarsen_blocks_per_row(f, res), cpb = coarsen_cells_per_block(f);
+  const int local = blockIdx.x - Gs.row0;
+  const int cy = local / nbx, cell0 = (local - cy * nbx) * cpb;
+  const int ncell = min(cpb, res - cell0);
+  coarsen_block<NE, false, 0>(lds, G.src, G.tgt, G.mask, nullptr, G.W, G.H, G.C, ch, lambda,
+                              A0.x0, A0.y0, f, cy, cell0, ncell,
+                              [&](int k, const CoarseSums<NE> &s) {
+    const size_t c = (size_t)cy * res + cell0 + k;
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+      coarsen_put(in4 + ((size_t)(net[e] - j0) * res * res + c) * 4, s, e, f);
+  });
+}
+
 __global__ __launch_bounds__(256) void k_coarsen_groups(const PhInst *__restrict__ insts,
                                                         const PhCorrArgs *__restrict__ args,
                                                         const PhSpan *__restrict__ groups,
@@ -191,31 +295,15 @@ __global__ __launch_bounds__(256) void k_coarsen_groups(const PhInst *__restrict
                                                         const PhItemIO *__restrict__ io,
                                                         double lambda, int res, int j0,
                                                         float *__restrict__ in4) {
+  __shared__ CoarseLds lds;
   const PhSpan Gs = groups[ph_span_find(groups, ngroups, blockIdx.x)];
   const PhItemIO &G = io[Gs.item];
-  const int ne = Gs.nrows;
-  const PhCorrArgs A0 = args[Gs.slot];
-  const int f = A0.f;
-  int ch[4] = {0, 0, 0, 0}, net[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    if (e < ne) {
-      const PhCorrArgs A = args[Gs.slot + e];
-      ch[e] = insts[A.slot].channel;
-      net[e] = A.net;
-    }
-  const int nbx = coarsen_blocks_per_row(f, res), cpb = coarsen_cells_per_block(f);
-  const int local = blockIdx.x - Gs.row0;
-  const int cy = local / nbx, cell0 = (local - cy * nbx) * cpb;
-  const int ncell = min(cpb, res - cell0);
-  const CoarseSums<4> s = coarsen_block<4, false>(G.src, G.tgt, G.mask, nullptr, G.W, G.H, G.C,
-                                                  ch, ne, lambda, A0.x0, A0.y0, f, 0, cy, cell0,
-                                                  ncell);
-  if ((int)threadIdx.x >= ncell) return;
-  const size_t c = (size_t)cy * res + cell0 + threadIdx.x;
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    if (e < ne) coarsen_put(in4 + ((size_t)(net[e] - j0) * res * res + c) * 4, s, e, f);
+  switch (Gs.nrows) {  // the group's channels, uniform across the block
+    case 1: coarsen_group<1>(lds, insts, args, Gs, G, lambda, res, j0, in4); break;
+    case 2: coarsen_group<2>(lds, insts, args, Gs, G, lambda, res, j0, in4); break;
+    case 3: coarsen_group<3>(lds, insts, args, Gs, G, lambda, res, j0, in4); break;
+    default: coarsen_group<4>(lds, insts, args, Gs, G, lambda, res, j0, in4); break;
+  }
 }
 
 // Net input of a coarse correction, from an instance's plane: per block the

@@ -118,10 +118,11 @@ def test_training_fits_a_fixed_batch(gpu):
 
 
 @pytest.mark.parametrize("H,W,R,f,x0,y0", [
-    (70, 90, 16, 5, -7, -3),      # several whole cells per block
-    (40, 50, 64, 1, -5, -2),      # one pixel per cell, a block spans the coarse row
-    (130, 150, 8, 33, 3, -4),     # 7 cells per block: a row in two blocks, the last one partial
-    (400, 500, 2, 300, -20, -10),  # cells wider than a block: columns in chunks of 256
+    (70, 90, 16, 5, -7, -3),       # a thread per column, several whole cells per block
+    (40, 50, 64, 1, -5, -2),       # one pixel per cell, a block spans the coarse row
+    (300, 310, 32, 12, -40, 9),    # 21 cells per block: a row in two blocks, the last partial
+    (130, 150, 6, 20, 3, -4),      # a wave per cell: 4 cells per block, the last block partial
+    (400, 500, 2, 300, -20, -10),  # a wave per cell wider than 256 columns
 ])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_coarsen_matches_numpy(gpu, mode, H, W, R, f, x0, y0):
