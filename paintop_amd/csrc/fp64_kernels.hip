@@ -156,13 +156,14 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 // The coarse mask the SOR-factor tuner works on, for every image of a call in
 // one launch: a block of f x f cells of an image's region is inside when at
-// least half of its cells are interior. One block per coarse row of every
-// listed image (spans: slot = image, nrows = its coarse rows). The 4 waves
+// least half of its cells are interior. PH_COARSE_PARTS blocks per coarse row
+// of every listed image, each for a contiguous part of the row's cells
+// (spans: slot = image, nrows = PH_COARSE_PARTS x its coarse rows). The 4 waves
 // share the coarse row's f image rows and read along them (coalesced), each
 // lane adds its columns over its rows, and the sums meet in LDS counters
 // (integer adds: the counts do not depend on the order). Writes the row's
 // mask bytes and row_hash[block] = the sum mod 2^64 of mix64(index) over the
-// row's inside cells (index = cy * cw + c within the image's mask), so an
+// part's inside cells (index = cy * cw + c within the image's mask), so an
 // image's key (k_coarse_hash_items) is independent of the summation order.
 // cw <= PH_COARSE_MAX.
 constexpr int PH_COARSE_MAX = 256;
@@ -172,11 +173,13 @@ __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__res
                                                           uint64_t *__restrict__ row_hash) {
   const PhSpan S = spans[ph_span_find(spans, n, blockIdx.x)];
   const PhItemGeo G = items[S.slot];
-  const int cy = blockIdx.x - S.row0;
+  const int local = blockIdx.x - S.row0;
+  const int cy = local / PH_COARSE_PARTS, part = local - cy * PH_COARSE_PARTS;
+  const int c_lo = part * G.cw / PH_COARSE_PARTS, c_hi = (part + 1) * G.cw / PH_COARSE_PARTS;
   __shared__ int cnt[PH_COARSE_MAX];
-  for (int c = threadIdx.x; c < G.cw; c += 256) cnt[c] = 0;
+  for (int c = threadIdx.x; c < c_hi - c_lo; c += 256) cnt[c] = 0;
   __syncthreads();
-  const int ncols = G.cw * G.cf;
+  const int xs = c_lo * G.cf, ncols = (c_hi - c_lo) * G.cf;
   // wave w takes the image rows w, w + 4, ...; its lanes take columns
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   constexpr int J = 16;  // columns per lane per chunk
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__res
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const int x = x0 + j * 64 + lane;
-        const int ix = G.rx0 + x;
+        const int ix = G.rx0 + xs + x;
         if (x < ncols && ix >= 0 && ix < G.w) n[j] += row[ix];
       }
     }
@@ -204,9 +207,9 @@ __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__res
   __syncthreads();
   uint64_t h = 0;
   const int full = G.cf * G.cf;
-  for (int c = threadIdx.x; c < G.cw; c += 256) {
+  for (int c = threadIdx.x; c < c_hi - c_lo; c += 256) {
     const bool in = 2 * cnt[c] >= full;
-    const int64_t i = (int64_t)cy * G.cw + c;
+    const int64_t i = (int64_t)cy * G.cw + c_lo + c;
     mask[G.coarse_off + i] = in ? 1 : 0;
     if (in) h += mix64((uint64_t)i);
   }

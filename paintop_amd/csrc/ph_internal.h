@@ -138,6 +138,9 @@ constexpr int PH_NET_SLICES = 16;
 
 // Per-image geometry for the set-up kernels that serve every image of a call
 // in one launch (tile occupancy, tuner coarse masks).
+// Blocks per coarse row of the SOR-factor tuner's coarse-mask launch.
+constexpr int PH_COARSE_PARTS = 4;
+
 struct PhItemGeo {
   const uint8_t *cls;   // image classification, w x h
   int32_t w, h;
