@@ -156,9 +156,9 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 // The coarse mask the SOR-factor tuner works on, for every image of a call in
 // one launch: a block of f x f cells of an image's region is inside when at
-// least half of its cells are interior. PH_COARSE_PARTS blocks per coarse row
+// least half of its cells are interior. ph_coarse_parts blocks per coarse row
 // of every listed image, each for a contiguous part of the row's cells
-// (spans: slot = image, nrows = PH_COARSE_PARTS x its coarse rows). The 4 waves
+// (spans: slot = image, nrows = the parts x its coarse rows). The 4 waves
 // share the coarse row's f image rows and read along them (coalesced), each
 // lane adds its columns over its rows, and the sums meet in LDS counters
 // (integer adds: the counts do not depend on the order). Writes the row's
@@ -174,8 +174,9 @@ __global__ __launch_bounds__(256) void k_coarse_count_all(const PhItemGeo *__res
   const PhSpan S = spans[ph_span_find(spans, n, blockIdx.x)];
   const PhItemGeo G = items[S.slot];
   const int local = blockIdx.x - S.row0;
-  const int cy = local / PH_COARSE_PARTS, part = local - cy * PH_COARSE_PARTS;
-  const int c_lo = part * G.cw / PH_COARSE_PARTS, c_hi = (part + 1) * G.cw / PH_COARSE_PARTS;
+  const int np = ph_coarse_parts(G.cw, G.cf);
+  const int cy = local / np, part = local - cy * np;
+  const int c_lo = part * G.cw / np, c_hi = (part + 1) * G.cw / np;
   __shared__ int cnt[PH_COARSE_MAX];
   for (int c = threadIdx.x; c < c_hi - c_lo; c += 256) cnt[c] = 0;
   __syncthreads();

@@ -138,8 +138,14 @@ constexpr int PH_NET_SLICES = 16;
 
 // Per-image geometry for the set-up kernels that serve every image of a call
 // in one launch (tile occupancy, tuner coarse masks).
-// Blocks per coarse row of the SOR-factor tuner's coarse-mask launch.
+// Blocks per coarse row of the SOR-factor tuner's coarse-mask launch: one per
+// started 1024 image columns of the row, at most 4 (a narrow row is one
+// block's work).
 constexpr int PH_COARSE_PARTS = 4;
+__host__ __device__ inline int ph_coarse_parts(int cw, int cf) {
+  const int p = (cw * cf + 1023) / 1024;
+  return p < 1 ? 1 : (p > PH_COARSE_PARTS ? PH_COARSE_PARTS : p);
+}
 
 struct PhItemGeo {
   const uint8_t *cls;   // image classification, w x h

@@ -1033,7 +1033,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
     // every tile origin
     std::vector<PhItemGeo> geo(nitem);
     std::vector<int> origin_item(all_origins.size());
-    SpanTable coarse_rows;  // PH_COARSE_PARTS blocks per coarse row of every tuned image
+    SpanTable coarse_rows;  // ph_coarse_parts blocks per coarse row of every tuned image
     for (int i = 0; i < nitem; ++i) {
       const ItemWork &w = iw[i];
       PhItemGeo &G = geo[i];
@@ -1051,7 +1051,7 @@ static void solve_items(const ph_solve_desc *d, std::vector<ItemIn> &items, hipS
         G.ch = w.ch;
         G.coarse_off = (int64_t)w.coarse_off;
         iw[i].coarse_entry = (int)coarse_rows.h.size();
-        coarse_rows.add(i, w.ch * PH_COARSE_PARTS, i);
+        coarse_rows.add(i, w.ch * ph_coarse_parts(w.cw, w.cf), i);
       }
       if (w.any)
         for (size_t k = 0; k < w.origins.size(); ++k) origin_item[w.origin_off + k] = i;
