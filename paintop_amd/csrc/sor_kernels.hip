@@ -389,19 +389,29 @@ __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict
                                                         int ninst, int ntiles) {
   __shared__ double A[K];
   __shared__ double R2[K / 2], RL[K / 2];
+  // (the grid has one block per instance: blockIdx.x < ninst)
   const int a = blockIdx.x;
-  if (a >= ninst) return;
   PhCtl &C = ctl[a];
-  if (C.halt != PH_RUN) return;  // uniform across the block
   const PhInst &I = insts[a];
+  // the halt state and the instance's tile range in one memory round trip:
+  // one test on all of them keeps both loads ahead of the branch
+  const uint32_t halt = C.halt;
+  const int tb = I.tile_begin, te = I.tile_end;
+  if (a >= ninst || halt != PH_RUN || tb > te) return;  // uniform across the block
   const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // the rules run on register copies (a chain of dependent global loads and
-  // stores would dominate this kernel otherwise), loaded before the sums
-  PhCtl c;
-  PhInst in;
-  if (threadIdx.x == 0) {
-    c = C;
-    in = I;
+  // The rules run on copies of the control block and the instance (a chain of
+  // dependent global loads and stores would dominate this kernel otherwise).
+  // Wave 0 loads them, one dword per lane, together with its tiles' sums, so
+  // the whole block waits for one memory round trip; they meet in LDS.
+  constexpr int NC = (int)(sizeof(PhCtl) / 4), NI = (int)(sizeof(PhInst) / 4);
+  static_assert(NC <= 64 && NI <= 64 && sizeof(PhCtl) % 4 == 0 && sizeof(PhInst) % 4 == 0,
+                "one dword per lane");
+  __shared__ __align__(16) uint32_t cpy_c[NC];
+  __shared__ __align__(16) uint32_t cpy_i[NI];
+  uint32_t wc = 0, wi = 0;
+  if (s == 0) {
+    if (lane < NC) wc = reinterpret_cast<const uint32_t *>(&C)[lane];
+    if (lane < NI) wi = reinterpret_cast<const uint32_t *>(&I)[lane];
   }
   // wave s adds the tiles' sums of half-sweep s in tile order: each lane loads
   // its tiles in batches of 16 (one memory latency for up to 1024 tiles), then
@@ -409,9 +419,8 @@ __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict
   // squares unchanged)
   double v = 0.0;
   const double *ps = partials + (size_t)s * ntiles;
-  const int te = I.tile_end;
   constexpr int B = 16;
-  for (int t = I.tile_begin + lane; t < te; t += 64 * B) {
+  for (int t = tb + lane; t < te; t += 64 * B) {
     double q[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) q[k] = t + 64 * k < te ? ps[t + 64 * k] : 0.0;
@@ -420,16 +429,24 @@ __global__ __launch_bounds__(64 * K) void k_sor_control(const PhInst *__restrict
   }
   v = wave_sum(v);
   if (lane == 0) A[s] = v;
+  if (s == 0) {
+    if (lane < NC) cpy_c[lane] = wc;
+    if (lane < NI) cpy_i[lane] = wi;
+  }
   __syncthreads();
+  const PhCtl &Cs = *reinterpret_cast<const PhCtl *>(cpy_c);
+  const PhInst &Is = *reinterpret_cast<const PhInst *>(cpy_i);
   // the sweeps' residuals in parallel (lane j: sweep S + j)
   if (threadIdx.x < K / 2) {
     const int j = threadIdx.x;
-    const double r2 = sweep_res2<K>(C, I, A, j);
+    const double r2 = sweep_res2<K>(Cs, Is, A, j);
     R2[j] = r2;
-    RL[j] = sqrt(r2) / C.r0;
+    RL[j] = sqrt(r2) / Cs.r0;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    PhCtl c = Cs;
+    PhInst in = Is;
     control<K>(c, in, A, R2, RL);
     C = c;
   }
