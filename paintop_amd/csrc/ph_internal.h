@@ -153,7 +153,7 @@ struct PhItemGeo {
   int32_t ox, oy;       // plane origin in image coordinates
   int32_t rx0, ry0;     // region origin (interior box plus ring)
   int32_t cf, cw, ch;   // tuner coarse grid: factor and size (0 x 0: none)
-  int64_t coarse_off;   // offset of the image's coarse counts
+  int64_t coarse_off;   // offset of the image's coarse mask (bytes)
 };
 
 // One entry of a row-batched launch (many instances or images in one grid,
