@@ -215,7 +215,10 @@ struct UnNet {
   void conv(const UnConvArgs &A, hipStream_t st) {
     const size_t np = (size_t)A.n * A.h * A.w;
     const int MB = A.cout / 16;
-    if (A.cout % 16 != 0 || (A.ca % 4) || (A.cb % 4) || (A.ca + A.cb != 4 && (A.ca + A.cb) % 16))
+    // (a 32-wide K chunk's lane group reads 8 channels of one tensor: from 16
+    // input channels on, the skip/coarser boundary ca is a multiple of 8)
+    if (A.cout % 16 != 0 || (A.ca % 4) || (A.cb % 4) || (A.ca + A.cb != 4 && (A.ca + A.cb) % 16) ||
+        (A.ca + A.cb >= 16 && A.ca % 8))
       throw fail(PH_CLASS_EXECUTION, "E_HIP_INTERNAL", "conv shape %d+%d -> %d", A.ca, A.cb,
                  A.cout);
     // few pixels (the solver's batches of a handful of 128x128 grids below the

@@ -1,5 +1,5 @@
 // unet_kernels.hip -- the warm-start network's kernels: BF16 3x3 convolutions
-// on the CDNA matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation) for
+// on the CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation) for
 // the forward pass and both gradients, plus the small elementwise kernels
 // around them (average pooling, gradient routing through skip/upsample
 // connections, the FP32 1x1 output head, the loss, Adam).
@@ -14,6 +14,9 @@
 // v_mfma_f32_16x16x16_bf16 operand layout (lane l, i = 0..3):
 //   A[m = l%16][k = 4*(l/16)+i], B[k = 4*(l/16)+i][n = l%16], C[m = 4*(l/16)+i][n = l%16]
 // so a lane holds four consecutive output channels of one pixel: one 8-byte store.
+// The forward / input-gradient convolutions take K in chunks of 32 with gfx950's
+// v_mfma_f32_16x16x32_bf16 (lane l holds k = 8*(l/16)+i, i = 0..7, of A and B:
+// one 16-byte load each; C as above), and a 16-wide chunk where cin % 32 == 16.
 //
 // The input of a decoder convolution is concat(skip, up2(coarser)) with a
 // nearest-neighbour x2 upsample; both are read in place (channel ci < ca from
@@ -49,6 +52,19 @@ __device__ __forceinline__ void st4(uint16_t *p, bf16x4 v) { *(gbf4 *)p = v; }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8m __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(1))) const bf16x8 gcbf8;
+__device__ __forceinline__ bf16x8 ld8(const uint16_t *p) { return *(gcbf8 *)p; }
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+  return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+// gfx950's double-K form: the products of a 32-wide K chunk in one instruction
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8m, a),
+                                                 __builtin_bit_cast(bf16x8m, b), c, 0, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -88,10 +104,11 @@ __global__ __launch_bounds__(256) void k_conv3(UnConvArgs A) {
 
   const uint16_t *wrow[MB];
 #pragma unroll
-  for (int mb = 0; mb < MB; ++mb) wrow[mb] = A.wgt + (size_t)(mb * 16 + col) * A.kp + grp * 4;
+  for (int mb = 0; mb < MB; ++mb) wrow[mb] = A.wgt + (size_t)(mb * 16 + col) * A.kp;
 
   if (cin >= 16) {
-    // cin is a multiple of 16: one tap per group of 16-wide K chunks
+    // cin is a multiple of 16: each tap's K in 32-wide chunks, then a 16-wide
+    // one when cin % 32 == 16 (the skip/coarser boundary ca is a multiple of 8)
     for (int t = 0; t < 9; ++t) {
       const int dy = (t / 3 - 1) * A.dil, dx = (t % 3 - 1) * A.dil;
       const uint16_t *sa[NB];
@@ -102,43 +119,75 @@ __global__ __launch_bounds__(256) void k_conv3(UnConvArgs A) {
         const int yy = py[nb] + dy, xx = px[nb] + dx;
         ok[nb] = pv[nb] && yy >= 0 && yy < A.h && xx >= 0 && xx < A.w;
         const int ys = ok[nb] ? yy : 0, xs = ok[nb] ? xx : 0;
-        sa[nb] = A.a + ((size_t)(pimg[nb] * A.h + ys) * A.w + xs) * A.ca + grp * 4;
-        sb[nb] = A.cb ? A.b + ((size_t)(pimg[nb] * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb +
-                            grp * 4 - A.ca
+        sa[nb] = A.a + ((size_t)(pimg[nb] * A.h + ys) * A.w + xs) * A.ca;
+        sb[nb] = A.cb ? A.b + ((size_t)(pimg[nb] * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb - A.ca
                       : nullptr;
       }
       const int kt = t * cin;
-      for (int c0 = 0; c0 < cin; c0 += 16) {
+      int c0 = 0;
+      for (; c0 + 32 <= cin; c0 += 32) {
+        const int c = c0 + grp * 8;
+        bf16x8 bv[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          bv[nb] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          if (ok[nb]) bv[nb] = (c < A.ca) ? ld8(sa[nb] + c) : ld8(sb[nb] + c);
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          const bf16x8 av = ld8(wrow[mb] + kt + c);
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma32(av, bv[nb], acc[mb][nb]);
+        }
+      }
+      if (c0 < cin) {
+        const int c = c0 + grp * 4;
         bf16x4 bv[NB];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
           bv[nb] = bf16x4{0, 0, 0, 0};
-          if (ok[nb]) bv[nb] = (c0 + grp * 4 < A.ca) ? ld4(sa[nb] + c0) : ld4(sb[nb] + c0);
+          if (ok[nb]) bv[nb] = (c < A.ca) ? ld4(sa[nb] + c) : ld4(sb[nb] + c);
         }
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
-          const bf16x4 av = ld4(wrow[mb] + kt + c0);
+          const bf16x4 av = ld4(wrow[mb] + kt + c);
 #pragma unroll
           for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma16(av, bv[nb], acc[mb][nb]);
         }
       }
     }
   } else {
-    // cin == 4 (the network input): one 16-wide K chunk spans four taps
-    for (int k0 = 0; k0 < A.kp; k0 += 16) {
-      const int t = (k0 >> 2) + grp;
+    // cin == 4 (the network input): k = 4 * tap + channel; a 32-wide K chunk
+    // spans eight taps (two per lane group), a 16-wide one four (kp is a
+    // multiple of 16: 48 = 32 + 16)
+    auto tap4 = [&](int t, int nb) -> bf16x4 {
       const int dy = (t / 3 - 1) * A.dil, dx = (t % 3 - 1) * A.dil;
-      bf16x4 bv[NB];
+      const int yy = py[nb] + dy, xx = px[nb] + dx;
+      if (t < 9 && pv[nb] && yy >= 0 && yy < A.h && xx >= 0 && xx < A.w)
+        return ld4(A.a + ((size_t)(pimg[nb] * A.h + yy) * A.w + xx) * 4);
+      return bf16x4{0, 0, 0, 0};
+    };
+    int k0 = 0;
+    for (; k0 + 32 <= A.kp; k0 += 32) {
+      const int t = (k0 >> 2) + 2 * grp;
+      bf16x8 bv[NB];
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        bv[nb] = bf16x4{0, 0, 0, 0};
-        const int yy = py[nb] + dy, xx = px[nb] + dx;
-        if (t < 9 && pv[nb] && yy >= 0 && yy < A.h && xx >= 0 && xx < A.w)
-          bv[nb] = ld4(A.a + ((size_t)(pimg[nb] * A.h + yy) * A.w + xx) * 4);
-      }
+      for (int nb = 0; nb < NB; ++nb) bv[nb] = cat8(tap4(t, nb), tap4(t + 1, nb));
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
-        const bf16x4 av = ld4(wrow[mb] + k0);
+        const bf16x8 av = ld8(wrow[mb] + k0 + grp * 8);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma32(av, bv[nb], acc[mb][nb]);
+      }
+    }
+    if (k0 < A.kp) {
+      const int t = (k0 >> 2) + grp;
+      bf16x4 bv[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) bv[nb] = tap4(t, nb);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const bf16x4 av = ld4(wrow[mb] + k0 + grp * 4);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma16(av, bv[nb], acc[mb][nb]);
       }
@@ -200,44 +249,44 @@ __global__ __launch_bounds__(256) void k_conv3_sk(UnConvArgs A) {
   for (int mb = 0; mb < MB; ++mb) acc[mb] = f32x4{0.f, 0.f, 0.f, 0.f};
   const uint16_t *wrow[MB];
 #pragma unroll
-  for (int mb = 0; mb < MB; ++mb) wrow[mb] = A.wgt + (size_t)(mb * 16 + col) * A.kp + grp * 4;
+  for (int mb = 0; mb < MB; ++mb) wrow[mb] = A.wgt + (size_t)(mb * 16 + col) * A.kp;
 
-  if constexpr (MB >= 4) {
+  if (MB >= 4 && cin % 32 == 0) {
     // wave w takes taps w, w + 4, w + 8. With 64 output channels or more its
-    // (tap, chunk) steps run as one loop whose operands are loaded a step ahead
-    // (the products still enter each accumulator in (tap, chunk) order): 12.3 ->
-    // 9.9 us for the 64-channel layers at a batch of three 128^2 grids; narrower
-    // layers keep the plain loop, which the extra registers slowed.
-    const int nchunk = cin >> 4;
+    // (tap, 32-wide chunk) steps run as one loop whose operands are loaded a
+    // step ahead (the products still enter each accumulator in (tap, chunk)
+    // order): 12.3 -> 9.9 us for the 64-channel layers at a batch of three
+    // 128^2 grids; narrower layers keep the plain loop, which the extra
+    // registers slowed.
+    const int nchunk = cin >> 5;
     const int ntap = (9 - wave + 3) / 4, nstep = ntap * nchunk;
-    auto load_b = [&](int st) -> bf16x4 {
-      const int i = st / nchunk, c0 = (st - i * nchunk) * 16, t = wave + 4 * i;
+    auto load_b = [&](int st) -> bf16x8 {
+      const int i = st / nchunk, c = (st - i * nchunk) * 32 + grp * 8, t = wave + 4 * i;
       const int dy = (t / 3 - 1) * A.dil, dx = (t % 3 - 1) * A.dil;
       const int yy = py + dy, xx = px + dx;
       const bool ok = pv && yy >= 0 && yy < A.h && xx >= 0 && xx < A.w;
       const int ys = ok ? yy : 0, xs = ok ? xx : 0;
-      if (!ok) return bf16x4{0, 0, 0, 0};
-      return (c0 + grp * 4 < A.ca)
-                 ? ld4(A.a + ((size_t)(pimg * A.h + ys) * A.w + xs) * A.ca + grp * 4 + c0)
-                 : ld4(A.b + ((size_t)(pimg * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb + grp * 4 -
-                       A.ca + c0);
+      if (!ok) return bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      return (c < A.ca) ? ld8(A.a + ((size_t)(pimg * A.h + ys) * A.w + xs) * A.ca + c)
+                        : ld8(A.b + ((size_t)(pimg * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb -
+                              A.ca + c);
     };
-    auto load_a = [&](int st, bf16x4 (&av)[MB]) {
-      const int i = st / nchunk, c0 = (st - i * nchunk) * 16, t = wave + 4 * i;
+    auto load_a = [&](int st, bf16x8 (&av)[MB]) {
+      const int i = st / nchunk, c = (st - i * nchunk) * 32 + grp * 8, t = wave + 4 * i;
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) av[mb] = ld4(wrow[mb] + t * cin + c0);
+      for (int mb = 0; mb < MB; ++mb) av[mb] = ld8(wrow[mb] + t * cin + c);
     };
     if (nstep > 0) {
-      bf16x4 bv = load_b(0), av[MB];
+      bf16x8 bv = load_b(0), av[MB];
       load_a(0, av);
       for (int st = 0; st < nstep; ++st) {
-        bf16x4 bn = bv, an[MB];
+        bf16x8 bn = bv, an[MB];
         if (st + 1 < nstep) {
           bn = load_b(st + 1);
           load_a(st + 1, an);
         }
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[mb] = mfma16(av[mb], bv, acc[mb]);
+        for (int mb = 0; mb < MB; ++mb) acc[mb] = mfma32(av[mb], bv, acc[mb]);
         if (st + 1 < nstep) {
           bv = bn;
 #pragma unroll
@@ -251,16 +300,24 @@ __global__ __launch_bounds__(256) void k_conv3_sk(UnConvArgs A) {
       const int yy = py + dy, xx = px + dx;
       const bool ok = pv && yy >= 0 && yy < A.h && xx >= 0 && xx < A.w;
       const int ys = ok ? yy : 0, xs = ok ? xx : 0;
-      const uint16_t *sa = A.a + ((size_t)(pimg * A.h + ys) * A.w + xs) * A.ca + grp * 4;
+      const uint16_t *sa = A.a + ((size_t)(pimg * A.h + ys) * A.w + xs) * A.ca;
       const uint16_t *sb =
-          A.cb ? A.b + ((size_t)(pimg * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb + grp * 4 - A.ca
-               : nullptr;
+          A.cb ? A.b + ((size_t)(pimg * h2 + (ys >> 1)) * w2 + (xs >> 1)) * A.cb - A.ca : nullptr;
       const int kt = t * cin;
-      for (int c0 = 0; c0 < cin; c0 += 16) {
-        bf16x4 bv = bf16x4{0, 0, 0, 0};
-        if (ok) bv = (c0 + grp * 4 < A.ca) ? ld4(sa + c0) : ld4(sb + c0);
+      int c0 = 0;
+      for (; c0 + 32 <= cin; c0 += 32) {
+        const int c = c0 + grp * 8;
+        bf16x8 bv = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (ok) bv = (c < A.ca) ? ld8(sa + c) : ld8(sb + c);
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[mb] = mfma16(ld4(wrow[mb] + kt + c0), bv, acc[mb]);
+        for (int mb = 0; mb < MB; ++mb) acc[mb] = mfma32(ld8(wrow[mb] + kt + c), bv, acc[mb]);
+      }
+      if (c0 < cin) {  // cin % 32 == 16
+        const int c = c0 + grp * 4;
+        bf16x4 bv = bf16x4{0, 0, 0, 0};
+        if (ok) bv = (c < A.ca) ? ld4(sa + c) : ld4(sb + c);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) acc[mb] = mfma16(ld4(wrow[mb] + kt + c), bv, acc[mb]);
       }
     }
   }
@@ -307,10 +364,11 @@ template __global__ void k_conv3<8, 2>(UnConvArgs);
 // Weight gradient: dW[co][k] = sum_p dY[p][co] * patch_k(p), db[co] = sum_p dY[p][co].
 // Block = (pixel chunk, K range). Each 64-pixel sub-tile of dY and of the
 // patch columns is staged transposed in LDS (pixels contiguous), so an MFMA
-// operand (four consecutive pixels of one row) is one 8-byte LDS read.
+// operand (eight consecutive pixels of one row: the pixels are this GEMM's K,
+// taken 32 at a time by v_mfma_f32_16x16x32_bf16) is one 16-byte LDS read.
 // ---------------------------------------------------------------------------
 constexpr int DW_PX = 64;
-constexpr int DW_LD = DW_PX + 4;  // LDS row pitch (bf16), keeps 8-byte alignment
+constexpr int DW_LD = DW_PX + 8;  // LDS row pitch (bf16), keeps 16-byte alignment
 
 __global__ __launch_bounds__(256) void k_conv3_dw(UnDwArgs A) {
   __shared__ __align__(16) uint16_t dyT[64 * DW_LD];
@@ -364,15 +422,15 @@ __global__ __launch_bounds__(256) void k_conv3_dw(UnDwArgs A) {
     }
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < DW_PX / 16; ++ks) {
+    for (int ks = 0; ks < DW_PX / 32; ++ks) {
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
         const int j = wave + 4 * jj;
         if (j < nblk) {
           const int mb = j % mbt, nb = j / mbt;
-          const bf16x4 av = *(const bf16x4 *)&dyT[(mb * 16 + col) * DW_LD + ks * 16 + grp * 4];
-          const bf16x4 bv = *(const bf16x4 *)&xT[(nb * 16 + col) * DW_LD + ks * 16 + grp * 4];
-          acc[jj] = mfma16(av, bv, acc[jj]);
+          const bf16x8 av = *(const bf16x8 *)&dyT[(mb * 16 + col) * DW_LD + ks * 32 + grp * 8];
+          const bf16x8 bv = *(const bf16x8 *)&xT[(nb * 16 + col) * DW_LD + ks * 32 + grp * 8];
+          acc[jj] = mfma32(av, bv, acc[jj]);
         }
       }
     }
