@@ -321,44 +321,92 @@ __global__ __launch_bounds__(256) void k_assemble_spans(const PhInst *__restrict
     const double xu = (double)tgt[(q - W) * C + c], xd = (double)tgt[(q + W) * C + c];
     return fma(dg, -a, ((xl + xr) + xu) + xd) - bp;
   };
-  for (int px = threadIdx.x; px < I0.pitch; px += 256) {
-    const size_t p = (size_t)py * I0.pitch + px;
-    const int ix = I0.ox + px, iy = I0.oy + py;
-    const bool inside = px < I0.pw && ix >= 0 && ix < W && iy >= 0 && iy < H;
-    const size_t q = inside ? (size_t)iy * W + ix : 0;
-    const uint8_t in = inside ? cls_img[q] : 0;
-    cls_plane[p] = in;
+  if constexpr (CC != 0) {
+    // a known channel count: each pixel of the stencil is loaded once for all
+    // of its channels (one 16-byte load for four channels when both images are
+    // 16-byte aligned), and every channel is formed from registers with the
+    // operation order of bprime / anchor_r
+    const bool vec = CC == 4 && ((((uintptr_t)src) | ((uintptr_t)tgt)) & 15) == 0;  // uniform
+    auto ldpx = [&](const float *__restrict__ base, size_t q, float (&v)[R]) {
+      if (CC == 4 && vec) {
+        const float4 t = *reinterpret_cast<const float4 *>(base + q * 4);
+        v[0] = t.x;
+        v[1 % R] = t.y;
+        v[2 % R] = t.z;
+        v[3 % R] = t.w;
+      } else {
 #pragma unroll
-    for (int c = 0; c < (CC ? CC : 64); ++c) {
-      if (!CC && c >= C) break;
-      double *x = CC ? xs[c] : insts[S.slot + c].x;
-      if (!x) continue;
-      double a = 0.0, bp = 0.0;
-      float gv = __builtin_nanf("");
-      if (inside) {
-        a = (double)tgt[q * C + c];
-        if (in) {
-          // interior cells are never on the image border: four in-image neighbours
-          bp = bprime(q, c, a);
-          if (CC) {
-            const double r = anchor_r(q, c, a, bp);
-            gv = (float)(-r);
-            rs[c] = fma(r, r, rs[c]);
-          }
-        }
+        for (int c = 0; c < R; ++c) v[c] = base[q * CC + c];
       }
-      x[p] = a;
-      if (CC) {
+    };
+    for (int px = threadIdx.x; px < I0.pitch; px += 256) {
+      const size_t p = (size_t)py * I0.pitch + px;
+      const int ix = I0.ox + px, iy = I0.oy + py;
+      const bool inside = px < I0.pw && ix >= 0 && ix < W && iy >= 0 && iy < H;
+      const size_t q = inside ? (size_t)iy * W + ix : 0;
+      const uint8_t in = inside ? cls_img[q] : 0;
+      cls_plane[p] = in;
+      float ta[R], sc[R], sl[R], sr[R], su[R], sd[R], tl[R], tr[R], tu[R], td[R];
+#pragma unroll
+      for (int c = 0; c < R; ++c) ta[c] = 0.0f;
+      if (inside) ldpx(tgt, q, ta);
+      if (in) {
+        // interior cells are never on the image border: four in-image neighbours
+        ldpx(src, q, sc);
+        ldpx(src, q - 1, sl);
+        ldpx(src, q + 1, sr);
+        ldpx(src, q - W, su);
+        ldpx(src, q + W, sd);
+        ldpx(tgt, q - 1, tl);
+        ldpx(tgt, q + 1, tr);
+        ldpx(tgt, q - W, tu);
+        ldpx(tgt, q + W, td);
+      }
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        if (!xs[c]) continue;
+        const double a = (double)ta[c];
+        double bp = 0.0;
+        float gv = __builtin_nanf("");
+        if (in) {
+          const double bb = fma(4.0, -(double)sc[c],
+                                (((double)sl[c] + (double)sr[c]) + (double)su[c]) + (double)sd[c]);
+          bp = fma(lambda, -a, bb);
+          const double r =
+              fma(dg, -a, (((double)tl[c] + (double)tr[c]) + (double)tu[c]) + (double)td[c]) - bp;
+          gv = (float)(-r);
+          rs[c] = fma(r, r, rs[c]);
+        }
+        xs[c][p] = a;
         bps[c][p] = bp;
         e0s[c][p] = 0.0f;
         e1s[c][p] = 0.0f;
         gs[c][p] = gv;
-      } else {
+      }
+    }
+  } else {
+    // any C: one channel at a time (the residual follows in a second pass)
+    for (int px = threadIdx.x; px < I0.pitch; px += 256) {
+      const size_t p = (size_t)py * I0.pitch + px;
+      const int ix = I0.ox + px, iy = I0.oy + py;
+      const bool inside = px < I0.pw && ix >= 0 && ix < W && iy >= 0 && iy < H;
+      const size_t q = inside ? (size_t)iy * W + ix : 0;
+      const uint8_t in = inside ? cls_img[q] : 0;
+      cls_plane[p] = in;
+      for (int c = 0; c < C; ++c) {
         const PhInst &I = insts[S.slot + c];
+        if (!I.x) continue;
+        double a = 0.0, bp = 0.0;
+        if (inside) {
+          a = (double)tgt[q * C + c];
+          // interior cells are never on the image border: four in-image neighbours
+          if (in) bp = bprime(q, c, a);
+        }
+        I.x[p] = a;
         I.bp[p] = bp;
         I.e[0][p] = 0.0f;
         I.e[1][p] = 0.0f;
-        I.g[p] = gv;
+        I.g[p] = __builtin_nanf("");
       }
     }
   }
