@@ -377,6 +377,40 @@ def train(X, Y, Wl, *, epochs=24, batch=32, lr=2e-3, lr_min=5e-5, warmup=300, se
     return params, dict(history=hist, val_loss=final_val, val_by_kind=by_kind, steps=step)
 
 
+def time_network(batch: int = 32, reps: int = 50, log=print) -> dict:
+    """Wall-clock ms of the shipped network's forward pass at batches 1, 3, 8
+    and `batch`, and of one training step (forward, multi-scale loss, backward,
+    no update) at `batch`, on RES x RES grids of seeded random inputs."""
+    from .device import DeviceArray, synchronize
+
+    rng = np.random.default_rng(0)
+    net = hip.UNet(batch, RES, params=load_weights())
+    x = DeviceArray.from_numpy(rng.standard_normal((batch, RES, RES, 4)).astype(np.float32))
+    y = DeviceArray.from_numpy(rng.standard_normal((batch, RES, RES)).astype(np.float32))
+    w = DeviceArray.from_numpy(np.ones((batch, RES, RES), np.float32))
+    o = DeviceArray((batch, RES, RES), np.float32)
+
+    def ms(fn, n):
+        for _ in range(5):
+            fn()
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    res = {}
+    for b in sorted({1, 3, 8, batch}):
+        if b <= batch:
+            res[f"forward_b{b}_ms"] = round(ms(lambda: net.forward(x.ptr, o.ptr, b), reps), 4)
+    res[f"train_step_b{batch}_ms"] = round(
+        ms(lambda: net.train_step(x.ptr, y.ptr, w.ptr, batch, 0.0, update=False, pyramid=1.0),
+           max(1, reps // 2)), 4)
+    log(json.dumps(res))
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m paintop_amd.warmstart")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -397,7 +431,12 @@ def main(argv=None):
                         "learned solver's own states (needs --init or the shipped weights)")
     t.add_argument("--init", default=None,
                    help="start from these weights (.npy); also the solver's for --state-frac")
+    tm = sub.add_parser("time", help="time the network's forward pass and training step")
+    tm.add_argument("--batch", type=int, default=32)
+    tm.add_argument("--reps", type=int, default=50)
     a = ap.parse_args(argv)
+    if a.cmd == "time":
+        return time_network(a.batch, a.reps)
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
         t0 = time.time()
