@@ -366,7 +366,7 @@ struct UnNet {
     hipLaunchKernelGGL(k_head_bwd, dim3(nhb), dim3(256), 0, st, out, d_target, d_weight,
                        act[UB_D0], master + g_lay.off_head, np0, 1.0f / (float)np0, pyr, res,
                        alpha, dz[13], head_part);
-    hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, st, head_part, nhb,
+    hipLaunchKernelGGL(k_head_reduce, dim3(UN_HEAD_C + 2), dim3(256), 0, st, head_part, nhb,
                        grad + g_lay.off_head, loss);
     // decoder level 0
     weight_grad(13, dz[13], batch, st);

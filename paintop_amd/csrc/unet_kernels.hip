@@ -463,13 +463,17 @@ __global__ void k_dw_reduce(const float *__restrict__ part, const float *__restr
                             float *__restrict__ gb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nw = cout * kp;
+  // (unrolled so that several chunks' loads are in flight; the adds keep
+  // chunk order)
   if (i < nw) {
     float s = 0.0f;
+#pragma unroll 8
     for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * nw + i];
     gw[i] = s;
   } else if (i < nw + cout) {
     const int co = i - nw;
     float s = 0.0f;
+#pragma unroll 8
     for (int c = 0; c < nchunk; ++c) s += part_db[(size_t)c * cout + co];
     gb[co] = s;
   }
@@ -664,16 +668,29 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ out,
 }
 
 // Fixed-order sum of the head partials: gh[0..16] = [dwh, dbh], loss[0] (FP64).
-__global__ void k_head_reduce(const float *__restrict__ part, int nblocks, float *__restrict__ gh,
-                              double *__restrict__ loss) {
-  const int j = threadIdx.x;
-  if (j >= UN_HEAD_PART) return;
+// One block per sum j (grid UN_HEAD_PART, 256 threads): thread t adds the
+// partials t, t + 256, ... in order, then the block adds its threads' sums by
+// a fixed tree (one thread walking all partials took ~0.3 ms at 2048 blocks:
+// a chain of dependent loads).
+__global__ __launch_bounds__(256) void k_head_reduce(const float *__restrict__ part, int nblocks,
+                                                     float *__restrict__ gh,
+                                                     double *__restrict__ loss) {
+  __shared__ double red[256];
+  const int j = blockIdx.x, t = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += (double)part[(size_t)b * UN_HEAD_PART + j];
-  if (j < UN_HEAD_C + 1)
-    gh[j] = (float)s;
-  else
-    loss[0] = s;
+  for (int b = t; b < nblocks; b += 256) s += (double)part[(size_t)b * UN_HEAD_PART + j];
+  red[t] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (j < UN_HEAD_C + 1)
+      gh[j] = (float)red[0];
+    else
+      loss[0] = red[0];
+  }
 }
 
 // ---------------------------------------------------------------------------

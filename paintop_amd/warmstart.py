@@ -436,7 +436,8 @@ def main(argv=None):
     tm.add_argument("--reps", type=int, default=50)
     a = ap.parse_args(argv)
     if a.cmd == "time":
-        return time_network(a.batch, a.reps)
+        time_network(a.batch, a.reps)
+        return 0
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.cmd == "train":
         t0 = time.time()
